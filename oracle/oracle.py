@@ -1,0 +1,469 @@
+"""ctypes binding of the CPU oracle (liborc_splat.so) and of the reference's
+own sources compiled into oracle/_ref/libparallax_ref.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+benchmark's CPU-baseline leg, as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liborc_splat.so")
+REF_SO = os.path.join(HERE, "_ref", "libparallax_ref.so")
+
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int)
+
+
+class CCloud(C.Structure):
+    _fields_ = [("n", C.c_int), ("sh_degree", C.c_int), ("rest_stride", C.c_int),
+                ("pos", f32p), ("log_scale", f32p), ("rot", f32p), ("opacity", f32p),
+                ("dc", f32p), ("rest", f32p)]
+
+
+class CCloudD(C.Structure):
+    _fields_ = [("n", C.c_int), ("sh_degree", C.c_int), ("rest_stride", C.c_int),
+                ("pos", f64p), ("log_scale", f64p), ("rot", f64p), ("opacity", f64p),
+                ("dc", f64p), ("rest", f64p)]
+
+
+class CView(C.Structure):
+    _fields_ = [("R", C.c_double * 9), ("t", C.c_double * 3), ("fx", C.c_double), ("fy", C.c_double),
+                ("cx", C.c_double), ("cy", C.c_double), ("width", C.c_int), ("height", C.c_int)]
+
+
+class COptions(C.Structure):
+    _fields_ = [("background", C.c_float * 3), ("sh_degree", C.c_int), ("min_alpha", C.c_float),
+                ("transmittance_cutoff", C.c_float), ("screen_dilation", C.c_float)]
+
+
+class CGrads(C.Structure):
+    _fields_ = [("pos", f32p), ("log_scale", f32p), ("rot", f32p), ("opacity", f32p), ("dc", f32p),
+                ("rest", f32p), ("screen_gradient", f32p), ("visible_count", i32p)]
+
+
+class CGradsD(C.Structure):
+    _fields_ = [("pos", f64p), ("log_scale", f64p), ("rot", f64p), ("opacity", f64p), ("dc", f64p),
+                ("rest", f64p), ("screen_gradient", f64p), ("visible_count", i32p)]
+
+
+CONFIG_FIELDS = [
+    ("iterations", C.c_int), ("position_lr_initial", C.c_float), ("position_lr_final", C.c_float),
+    ("dc_lr", C.c_float), ("sh_rest_lr_divider", C.c_float), ("opacity_lr", C.c_float),
+    ("scale_lr", C.c_float), ("rotation_lr", C.c_float), ("adam_beta1", C.c_float),
+    ("adam_beta2", C.c_float), ("adam_epsilon", C.c_float), ("ssim_weight", C.c_float),
+    ("sh_degree", C.c_int), ("sh_increase_interval", C.c_int), ("densify_from_iteration", C.c_int),
+    ("densify_until_iteration", C.c_int), ("densify_interval", C.c_int),
+    ("densify_grad_threshold", C.c_float), ("percent_dense", C.c_float),
+    ("split_scale_divider", C.c_float), ("opacity_reset_interval", C.c_int),
+    ("opacity_reset_value", C.c_float), ("prune_opacity", C.c_float),
+    ("prune_scale_fraction", C.c_float), ("max_gaussians", C.c_int), ("seed", C.c_uint)]
+
+
+class CConfig(C.Structure):
+    _fields_ = CONFIG_FIELDS
+
+
+def default_config(**overrides) -> CConfig:
+    c = CConfig()
+    lib().orc_default_config(C.byref(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+def default_options(**overrides) -> COptions:
+    o = COptions()
+    lib().orc_default_options(C.byref(o))
+    for k, v in overrides.items():
+        if k == "background":
+            o.background[:] = list(v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def ptr(a, t=f32p):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class _Keep:
+    """Holds numpy arrays alive alongside the ctypes struct that points at them."""
+
+    def __init__(self, struct, arrays):
+        self.struct = struct
+        self.arrays = arrays
+
+
+def ccloud(cloud) -> _Keep:
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in
+            (cloud.pos, cloud.log_scale, cloud.rot, cloud.opacity, cloud.dc, cloud.rest)]
+    s = CCloud(cloud.n, cloud.sh_degree, cloud.rest_stride, *[ptr(a) for a in arrs])
+    return _Keep(s, arrs)
+
+
+def ccloud_d(cloud) -> _Keep:
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (cloud.pos, cloud.log_scale, cloud.rot, cloud.opacity, cloud.dc, cloud.rest)]
+    s = CCloudD(cloud.n, cloud.sh_degree, cloud.rest_stride, *[ptr(a, f64p) for a in arrs])
+    return _Keep(s, arrs)
+
+
+def cview(v) -> CView:
+    s = CView()
+    s.R[:] = [float(x) for x in np.asarray(v.R, dtype=np.float64).ravel()]
+    s.t[:] = [float(x) for x in np.asarray(v.t, dtype=np.float64).ravel()]
+    s.fx, s.fy, s.cx, s.cy = float(v.fx), float(v.fy), float(v.cx), float(v.cy)
+    s.width, s.height = int(v.width), int(v.height)
+    return s
+
+
+class Grads:
+    """Host gradient arrays in the layout of SplatGradients."""
+
+    def __init__(self, n: int, rs: int, dtype=np.float32):
+        self.pos = np.zeros((n, 3), dtype)
+        self.log_scale = np.zeros((n, 3), dtype)
+        self.rot = np.zeros((n, 4), dtype)
+        self.opacity = np.zeros(n, dtype)
+        self.dc = np.zeros((n, 3), dtype)
+        self.rest = np.zeros((n, rs), dtype)
+        self.screen_gradient = np.zeros(n, dtype)
+        self.visible_count = np.zeros(n, np.int32)
+
+    def struct(self):
+        t = f32p if self.pos.dtype == np.float32 else f64p
+        S = CGrads if t is f32p else CGradsD
+        return S(ptr(self.pos, t), ptr(self.log_scale, t), ptr(self.rot, t), ptr(self.opacity, t),
+                 ptr(self.dc, t), ptr(self.rest, t), ptr(self.screen_gradient, t),
+                 ptr(self.visible_count, i32p))
+
+    def flat(self) -> np.ndarray:
+        return np.concatenate([self.pos.ravel(), self.log_scale.ravel(), self.rot.ravel(),
+                               self.opacity.ravel(), self.dc.ravel(), self.rest.ravel()])
+
+
+_LIB = None
+_REF = None
+
+
+def _declare(L):
+    L.orc_frame_create.restype = C.c_void_p
+    L.orc_frame_create.argtypes = [C.POINTER(CCloud), C.POINTER(CView), C.POINTER(COptions)]
+    L.orc_frame_destroy.argtypes = [C.c_void_p]
+    L.orc_frame_projected_count.argtypes = [C.c_void_p]
+    L.orc_frame_get_projected.argtypes = [C.c_void_p, i32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, i32p, i32p]
+    L.orc_frame_tile_grid.argtypes = [C.c_void_p, i32p, i32p]
+    L.orc_frame_get_tiles.argtypes = [C.c_void_p, i32p, i32p]
+    L.orc_frame_get_image.argtypes = [C.c_void_p, f32p, f32p, i32p, C.POINTER(C.c_int64)]
+    L.orc_frame_backward.argtypes = [C.c_void_p, C.POINTER(CCloud), f32p, C.POINTER(CGrads)]
+    L.orc_frame_backward_f64.argtypes = [C.c_void_p, C.POINTER(CCloudD), f32p, C.POINTER(CGradsD)]
+    L.orc_frame_render_f64.argtypes = [C.c_void_p, C.POINTER(CCloudD), f64p]
+    L.orc_frame_fd_f64.restype = C.c_double
+    L.orc_frame_fd_f64.argtypes = [C.c_void_p, C.POINTER(CCloudD), f64p, C.c_int, C.c_int, C.c_int, C.c_double]
+    L.orc_photometric_loss.restype = C.c_float
+    L.orc_photometric_loss.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_float, f32p]
+    L.orc_mean_ssim.restype = C.c_float
+    L.orc_mean_ssim.argtypes = [f32p, f32p, C.c_int, C.c_int, f32p]
+    L.orc_blur.argtypes = [f32p, C.c_int, C.c_int, f32p]
+    L.orc_evaluate_sh.argtypes = [f32p, f32p, C.c_int, C.c_int, f32p, f32p, i32p]
+    L.orc_det_exp.restype = C.c_float
+    L.orc_det_exp.argtypes = [C.c_float]
+    L.orc_trainer_create.restype = C.c_void_p
+    L.orc_trainer_create.argtypes = [C.POINTER(CCloud), C.c_int, C.POINTER(CView), C.POINTER(f32p),
+                                     C.POINTER(CConfig)]
+    L.orc_trainer_destroy.argtypes = [C.c_void_p]
+    L.orc_trainer_step.argtypes = [C.c_void_p, C.c_int]
+    L.orc_trainer_step_views.argtypes = [C.c_void_p, C.c_int, i32p]
+    for name in ("iteration", "count", "sh_degree_in_use"):
+        getattr(L, "orc_trainer_" + name).argtypes = [C.c_void_p]
+    for name in ("last_loss", "smoothed_loss", "extent"):
+        getattr(L, "orc_trainer_" + name).argtypes = [C.c_void_p]
+        getattr(L, "orc_trainer_" + name).restype = C.c_float
+    L.orc_trainer_densify_rng.argtypes = [C.c_void_p]
+    L.orc_trainer_densify_rng.restype = C.c_uint32
+    L.orc_trainer_last_views.argtypes = [C.c_void_p, i32p]
+    L.orc_trainer_get_cloud.argtypes = [C.c_void_p, C.POINTER(CCloud)]
+    L.orc_trainer_get_moments.argtypes = [C.c_void_p, f32p, f32p]
+    L.orc_trainer_get_stats.argtypes = [C.c_void_p, f32p, i32p]
+    L.orc_trainer_get_last_grads.argtypes = [C.c_void_p, C.POINTER(CGrads)]
+    L.orc_trainer_finish.argtypes = [C.c_void_p]
+    L.orc_adam_apply.argtypes = [C.POINTER(CCloud), C.POINTER(CGrads), f32p, f32p, C.c_int, C.c_float,
+                                 C.POINTER(CConfig)]
+    L.orc_densify.argtypes = [C.POINTER(CCloud), C.c_int, f32p, i32p, C.POINTER(CConfig), C.c_float, C.c_int,
+                              C.POINTER(C.c_uint32), i32p, i32p]
+    L.orc_camera_extent.restype = C.c_float
+    L.orc_camera_extent.argtypes = [C.c_int, C.POINTER(CView)]
+    L.orc_prune_and_sort.argtypes = [C.POINTER(CCloud), C.c_float, i32p]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        _LIB = C.CDLL(ORACLE_SO)
+        _declare(_LIB)
+    return _LIB
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _REF
+    if _REF is None:
+        if not os.path.exists(REF_SO):
+            raise RuntimeError(f"reference build missing: {REF_SO} (run make -C oracle ref)")
+        R = C.CDLL(REF_SO)
+        R.ref_project.argtypes = [C.POINTER(CCloud), C.POINTER(CView), C.POINTER(COptions), i32p, f32p, f32p,
+                                  f32p, f32p, f32p, f32p, f32p, i32p]
+        R.ref_render.argtypes = [C.POINTER(CCloud), C.POINTER(CView), C.POINTER(COptions), f32p, f32p]
+        R.ref_photometric_loss.restype = C.c_float
+        R.ref_photometric_loss.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_float, f32p]
+        R.ref_backward.argtypes = [C.POINTER(CCloud), C.POINTER(CView), C.POINTER(COptions), f32p, f32p, f32p,
+                                   f32p, f32p, f32p, f32p, f32p, i32p]
+        R.ref_densify.argtypes = [C.POINTER(CCloud), f32p, i32p, C.POINTER(CConfig), C.c_float, C.c_int,
+                                  C.POINTER(C.c_uint32), C.POINTER(CCloud), i32p, i32p]
+        R.ref_train.argtypes = [C.POINTER(CCloud), C.c_int, C.POINTER(CView), C.POINTER(f32p),
+                                C.POINTER(CConfig), C.c_int, f32p, C.POINTER(CCloud), C.c_int]
+        R.ref_time_steps.restype = C.c_double
+        R.ref_time_steps.argtypes = [C.POINTER(CCloud), C.c_int, C.POINTER(CView), C.POINTER(f32p),
+                                     C.POINTER(CConfig), C.c_int, f32p]
+        _REF = R
+    return _REF
+
+
+# ------------------------------------------------------------- frames --
+
+class Frame:
+    """projectSplats + buildTiles + rasterizeSplats of one view, kept alive
+    for the adjoint and the FP64 checks."""
+
+    def __init__(self, cloud, view, options: COptions | None = None):
+        self.cloud = cloud
+        self.view = view
+        self.options = options or default_options()
+        self._c = ccloud(cloud)
+        self._v = cview(view)
+        self.h = lib().orc_frame_create(C.byref(self._c.struct), C.byref(self._v), C.byref(self.options))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_frame_destroy(self.h)
+            self.h = None
+
+    @property
+    def count(self) -> int:
+        return lib().orc_frame_projected_count(self.h)
+
+    def projected(self) -> dict:
+        m = self.count
+        out = dict(index=np.zeros(m, np.int32), mean=np.zeros((m, 2), np.float32),
+                   cov2d=np.zeros((m, 3), np.float32), conic=np.zeros((m, 3), np.float32),
+                   depth=np.zeros(m, np.float32), radius=np.zeros(m, np.float32),
+                   color=np.zeros((m, 3), np.float32), opacity=np.zeros(m, np.float32),
+                   clamped=np.zeros((m, 3), np.int32), rect=np.zeros((m, 4), np.int32))
+        lib().orc_frame_get_projected(self.h, ptr(out["index"], i32p), ptr(out["mean"]), ptr(out["cov2d"]),
+                                      ptr(out["conic"]), ptr(out["depth"]), ptr(out["radius"]),
+                                      ptr(out["color"]), ptr(out["opacity"]), ptr(out["clamped"], i32p),
+                                      ptr(out["rect"], i32p))
+        return out
+
+    def tiles(self):
+        cols, rows = C.c_int(), C.c_int()
+        total = lib().orc_frame_tile_grid(self.h, C.byref(cols), C.byref(rows))
+        offsets = np.zeros(cols.value * rows.value + 1, np.int32)
+        entries = np.zeros(total, np.int32)
+        lib().orc_frame_get_tiles(self.h, ptr(offsets, i32p), ptr(entries, i32p))
+        return cols.value, rows.value, offsets, entries
+
+    def image(self):
+        px = self.view.width * self.view.height
+        rgb = np.zeros(px * 3, np.float32)
+        T = np.zeros(px, np.float32)
+        last = np.zeros(px, np.int32)
+        ev = (C.c_int64 * 2)()
+        lib().orc_frame_get_image(self.h, ptr(rgb), ptr(T), ptr(last, i32p), ev)
+        self.evals = (int(ev[0]), int(ev[1]))
+        return rgb, T, last
+
+    def backward(self, dL_drgb, grads: Grads | None = None) -> Grads:
+        g = grads or Grads(self.cloud.n, self.cloud.rest_stride)
+        s = g.struct()
+        dl = np.ascontiguousarray(dL_drgb, dtype=np.float32)
+        lib().orc_frame_backward(self.h, C.byref(self._c.struct), ptr(dl), C.byref(s))
+        ev = (C.c_int64 * 2)()
+        lib().orc_frame_get_image(self.h, None, None, None, ev)
+        self.evals = (int(ev[0]), int(ev[1]))
+        return g
+
+    def backward_f64(self, dL_drgb, cloud_d=None) -> Grads:
+        cd = cloud_d or ccloud_d(self.cloud)
+        g = Grads(self.cloud.n, self.cloud.rest_stride, np.float64)
+        s = g.struct()
+        dl = np.ascontiguousarray(dL_drgb, dtype=np.float32)
+        lib().orc_frame_backward_f64(self.h, C.byref(cd.struct), ptr(dl), C.byref(s))
+        return g
+
+    def render_f64(self, cloud_d=None) -> np.ndarray:
+        cd = cloud_d or ccloud_d(self.cloud)
+        rgb = np.zeros(self.view.width * self.view.height * 3, np.float64)
+        lib().orc_frame_render_f64(self.h, C.byref(cd.struct), ptr(rgb, f64p))
+        return rgb
+
+    def fd_f64(self, cloud_d, weights, gaussian: int, group: int, comp: int, h: float) -> float:
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        return lib().orc_frame_fd_f64(self.h, C.byref(cloud_d.struct), ptr(w, f64p), int(gaussian), int(group),
+                                      int(comp), float(h))
+
+
+def photometric_loss(rendered, target, width, height, ssim_weight, want_grad=True):
+    r = np.ascontiguousarray(rendered, dtype=np.float32)
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    dl = np.zeros_like(r) if want_grad else None
+    loss = lib().orc_photometric_loss(ptr(r), ptr(t), int(width), int(height), float(ssim_weight), ptr(dl))
+    return float(loss), dl
+
+
+def mean_ssim(rendered, target, width, height, want_grad=False):
+    r = np.ascontiguousarray(rendered, dtype=np.float32)
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    d = np.zeros_like(r) if want_grad else None
+    v = lib().orc_mean_ssim(ptr(r), ptr(t), int(width), int(height), ptr(d))
+    return float(v), d
+
+
+def det_exp(x: float) -> float:
+    return float(lib().orc_det_exp(float(x)))
+
+
+def camera_extent(views) -> float:
+    arr = (CView * len(views))(*[cview(v) for v in views])
+    return float(lib().orc_camera_extent(len(views), arr))
+
+
+def render(cloud, view, options=None):
+    f = Frame(cloud, view, options)
+    rgb, T, last = f.image()
+    return rgb, T
+
+
+# -------------------------------------------------------------- trainer --
+
+class Trainer:
+    """The oracle's SplatTrainer (SplatTrainer.cpp:326-383)."""
+
+    def __init__(self, cloud, views, images, config: CConfig):
+        self._c = ccloud(cloud)
+        self.views = views
+        self.rest_stride = cloud.rest_stride
+        self.sh_degree = cloud.sh_degree
+        self._v = (CView * max(1, len(views)))(*[cview(v) for v in views])
+        self._imgs = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
+        ip = (f32p * max(1, len(self._imgs)))(*[ptr(im) for im in self._imgs])
+        self.config = config
+        self.h = lib().orc_trainer_create(C.byref(self._c.struct), len(views), self._v, ip, C.byref(config))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_trainer_destroy(self.h)
+            self.h = None
+
+    def step(self, batch: int = 1) -> bool:
+        return bool(lib().orc_trainer_step(self.h, int(batch)))
+
+    def step_views(self, ids) -> bool:
+        a = np.ascontiguousarray(ids, dtype=np.int32)
+        return bool(lib().orc_trainer_step_views(self.h, len(a), ptr(a, i32p)))
+
+    iteration = property(lambda s: lib().orc_trainer_iteration(s.h))
+    count = property(lambda s: lib().orc_trainer_count(s.h))
+    sh_degree_in_use = property(lambda s: lib().orc_trainer_sh_degree_in_use(s.h))
+    last_loss = property(lambda s: lib().orc_trainer_last_loss(s.h))
+    smoothed_loss = property(lambda s: lib().orc_trainer_smoothed_loss(s.h))
+    extent = property(lambda s: lib().orc_trainer_extent(s.h))
+    densify_rng = property(lambda s: lib().orc_trainer_densify_rng(s.h))
+
+    def last_views(self):
+        out = np.zeros(1024, np.int32)
+        n = lib().orc_trainer_last_views(self.h, ptr(out, i32p))
+        return out[:n].copy()
+
+    def cloud(self):
+        from parallax_amd.synthetic import Cloud
+        n = self.count
+        rs = self.rest_stride
+        c = Cloud(np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32),
+                  np.zeros(n, np.float32), np.zeros((n, 3), np.float32), np.zeros((n, rs), np.float32),
+                  self.sh_degree)
+        k = ccloud(c)
+        lib().orc_trainer_get_cloud(self.h, C.byref(k.struct))
+        return Cloud(*[a.reshape(b.shape) for a, b in zip(k.arrays, (c.pos, c.log_scale, c.rot, c.opacity,
+                                                                     c.dc, c.rest))], self.sh_degree)
+
+    def moments(self):
+        n = self.count * (14 + self.rest_stride)
+        m = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        lib().orc_trainer_get_moments(self.h, ptr(m), ptr(v))
+        return m, v
+
+    def stats(self):
+        sg = np.zeros(self.count, np.float32)
+        vc = np.zeros(self.count, np.int32)
+        lib().orc_trainer_get_stats(self.h, ptr(sg), ptr(vc, i32p))
+        return sg, vc
+
+    def last_grads(self) -> Grads:
+        g = Grads(self.count, self.rest_stride)
+        s = g.struct()
+        lib().orc_trainer_get_last_grads(self.h, C.byref(s))
+        return g
+
+    def finish(self) -> int:
+        return lib().orc_trainer_finish(self.h)
+
+
+def adam_apply(cloud, grads: Grads, m, v, iteration: int, extent: float, config: CConfig):
+    k = ccloud(cloud)
+    s = grads.struct()
+    lib().orc_adam_apply(C.byref(k.struct), C.byref(s), ptr(m), ptr(v), int(iteration), float(extent),
+                         C.byref(config))
+    from parallax_amd.synthetic import Cloud
+    return Cloud(*[a.reshape(b.shape) for a, b in zip(k.arrays, (cloud.pos, cloud.log_scale, cloud.rot,
+                                                                 cloud.opacity, cloud.dc, cloud.rest))],
+                 cloud.sh_degree)
+
+
+def densify(cloud, screen_gradient, visible_count, config: CConfig, extent: float, iteration: int, rng_state: int):
+    """Returns (new cloud, source_index, counts, rng_state)."""
+    from parallax_amd.synthetic import Cloud
+    n = cloud.n
+    cap = 3 * n + 8
+    rs = cloud.rest_stride
+    work = Cloud(np.zeros((cap, 3), np.float32), np.zeros((cap, 3), np.float32), np.zeros((cap, 4), np.float32),
+                 np.zeros(cap, np.float32), np.zeros((cap, 3), np.float32), np.zeros((cap, rs), np.float32),
+                 cloud.sh_degree)
+    for a, b in zip((work.pos, work.log_scale, work.rot, work.opacity, work.dc, work.rest),
+                    (cloud.pos, cloud.log_scale, cloud.rot, cloud.opacity, cloud.dc, cloud.rest)):
+        a[:n] = b
+    k = ccloud(work)
+    k.struct.n = n
+    src = np.zeros(cap, np.int32)
+    counts = np.zeros(3, np.int32)
+    state = C.c_uint32(rng_state)
+    sg = np.ascontiguousarray(screen_gradient, dtype=np.float32)
+    vc = np.ascontiguousarray(visible_count, dtype=np.int32)
+    n1 = lib().orc_densify(C.byref(k.struct), cap, ptr(sg), ptr(vc, i32p), C.byref(config), float(extent),
+                           int(iteration), C.byref(state), ptr(src, i32p), ptr(counts, i32p))
+    out = Cloud(*[a.reshape(b.shape)[:n1].copy() for a, b in zip(k.arrays, (work.pos, work.log_scale, work.rot,
+                                                                           work.opacity, work.dc, work.rest))],
+                cloud.sh_degree)
+    return out, src[:n1].copy(), counts, int(state.value)
