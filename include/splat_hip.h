@@ -1,0 +1,221 @@
+/*
+ * splat_hip.h -- C ABI of the MI355X Gaussian-splat training step.
+ *
+ * A thin, POD-only boundary (no C++ or torch types) between host code and the
+ * HIP kernels in libparallax_splat.so. It replaces the engine's device path
+ * for splat training and exposes, one call each, the stages of the engine's
+ * CPU trainer so they can be run and checked separately:
+ *
+ *   reference interface                                  entry point here
+ *   -------------------                                  ----------------
+ *   splatGpuAvailability()  VulkanSplatTrainer.h:15-25   splat_availability
+ *   Context::create         vulkan/VulkanContext.cpp:65  splat_create / splat_destroy
+ *   GpuTrainer::setUp/allocate                           splat_set_capacity
+ *     VulkanSplatTrainer.cpp:290-405
+ *   HostParameters upload / download                     splat_upload_cloud / splat_download_cloud
+ *     VulkanSplatTrainer.cpp:147-195, 621-629, 863-866
+ *   SplatTrainingView images  SplatTrainer.h:20-23       splat_upload_views
+ *   projectSplats + rasterizeSplats  SplatRaster.h:123-131
+ *                                                        splat_render_forward (+ splat_download_image,
+ *                                                        splat_download_projection, splat_download_tiles)
+ *   photometricLoss  SplatRaster.h:153-154                splat_loss
+ *   rasterizeSplatsBackward  SplatRaster.h:138-142       splat_render_backward (+ splat_set_image_gradient,
+ *                                                        splat_download_gradients, splat_zero_gradients)
+ *   SplatTrainer applyGradients  SplatTrainer.cpp:218-256 splat_adam_step
+ *   SplatBackend::renderAndAccumulate + Adam              splat_train_step
+ *     SplatTrainer.cpp:337-349
+ *   densifyAndPrune + remapState                          splat_densify
+ *     SplatDensify.h:55-57, SplatTrainer.cpp:267-299
+ *   resetOpacity  SplatTrainer.cpp:301-309                splat_reset_opacity
+ *   accumulated statistics  SplatTrainer.cpp:153-154      splat_download_stats
+ *
+ * Conventions follow the engine: results are returned through out-parameters,
+ * every call returns a status and never throws, and on failure
+ * splat_last_error() holds a sentence for a person to read
+ * (engine/native/cpp CLAUDE.md:130-131, VulkanSplatTrainer.h:27-37). One
+ * context is used by one host thread at a time.
+ *
+ * Device data layout (HBM), all float32 unless noted:
+ *   parameters, gradients and both Adam moments each live in one buffer laid
+ *   out by capacity Cap: [pos 3Cap | log_scale 3Cap | rot 4Cap (w,x,y,z) |
+ *   opacity Cap | dc 3Cap | rest rs*Cap], rest channel-major per gaussian.
+ */
+#ifndef PARALLAX_SPLAT_HIP_H
+#define PARALLAX_SPLAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SPLAT_OK = 0,
+  SPLAT_ERROR_INVALID_ARGUMENT = 1,
+  SPLAT_ERROR_NO_DEVICE = 2,
+  SPLAT_ERROR_OUT_OF_MEMORY = 3,
+  SPLAT_ERROR_CAPACITY = 4,
+  SPLAT_ERROR_DEVICE = 5,
+  SPLAT_ERROR_STATE = 6
+} splat_status;
+
+typedef struct splat_context splat_context;
+
+/* SplatView (SplatRaster.h:26-29): camera-from-world pose p_cam = R p + t with
+ * R row-major, pinhole intrinsics (CameraIntrinsics, Types.h:30-94). */
+typedef struct {
+  double R[9];
+  double t[3];
+  double fx, fy, cx, cy;
+  int32_t width, height;
+} splat_view;
+
+/* RasterOptions (SplatRaster.h:31-50) plus the launch mode. */
+typedef struct {
+  float background[3];
+  int32_t sh_degree;
+  float min_alpha;             /* default 1/255 */
+  float transmittance_cutoff;  /* default 1e-4 */
+  float screen_dilation;       /* default 0.3 */
+  int32_t deterministic;       /* 1: fixed-order gradient reduction, bitwise repeatable */
+} splat_raster_options;
+
+/* A cloud in host memory, structure of arrays (Gaussian, GaussianCloud.h:41-72). */
+typedef struct {
+  int32_t n;
+  int32_t sh_degree;
+  int32_t rest_stride; /* 3 * shRestPerChannel(sh_degree) */
+  float* pos;          /* 3n */
+  float* log_scale;    /* 3n */
+  float* rot;          /* 4n, (w, x, y, z) */
+  float* opacity;      /* n, logits */
+  float* dc;           /* 3n */
+  float* rest;         /* rest_stride * n, channel-major per gaussian */
+} splat_cloud;
+
+/* SplatGradients (SplatRaster.h:87-114) in host memory; any pointer may be null. */
+typedef struct {
+  float* pos;
+  float* log_scale;
+  float* rot; /* (w, x, y, z) */
+  float* opacity;
+  float* dc;
+  float* rest;
+  float* screen_gradient;
+  int32_t* visible_count;
+} splat_gradients;
+
+/* The scalars of one Adam step (SplatTrainer.cpp:56-65, 218-256). */
+typedef struct {
+  float lr_position, lr_log_scale, lr_rotation, lr_opacity, lr_dc, lr_rest;
+  float beta1, beta2, epsilon;
+  float correction1; /* 1 - beta1^t */
+  float correction2; /* 1 - beta2^t */
+} splat_adam_params;
+
+/* SplatTrainingConfig fields used by densify (SplatTrainer.h:61-89). */
+typedef struct {
+  float densify_grad_threshold;
+  float percent_dense;
+  float split_scale_divider;
+  int32_t opacity_reset_interval;
+  float prune_opacity;
+  float prune_scale_fraction;
+  int32_t max_gaussians;
+} splat_densify_params;
+
+/* ---- availability, lifecycle ---- */
+splat_status splat_availability(char* device_name, size_t name_len, char* reason, size_t reason_len);
+splat_status splat_create(int32_t device, splat_context** out);
+void splat_destroy(splat_context* ctx);
+const char* splat_last_error(const splat_context* ctx);
+void splat_default_options(splat_raster_options* options);
+
+/* ---- scene ---- */
+/* Reserve device memory for `max_gaussians` at `sh_degree`; clears the cloud. */
+splat_status splat_set_capacity(splat_context* ctx, int32_t max_gaussians, int32_t sh_degree);
+/* Upload a cloud (parameters; gradients, moments and statistics are zeroed). */
+splat_status splat_upload_cloud(splat_context* ctx, const splat_cloud* cloud);
+/* Download into arrays sized for splat_count(). */
+splat_status splat_download_cloud(splat_context* ctx, splat_cloud* out);
+int32_t splat_count(const splat_context* ctx);
+/* Camera set and (optional) target images, 3*W*H floats each, linear RGB. */
+splat_status splat_upload_views(splat_context* ctx, int32_t n, const splat_view* views,
+                                const float* const* images);
+
+/* ---- forward / backward pair ---- */
+/* Project, bin, sort and composite the listed views in one batch. The batch
+ * keeps its sorted lists, final transmittance and last contributor per pixel
+ * for splat_render_backward; nothing per (splat, pixel) is stored. */
+splat_status splat_render_forward(splat_context* ctx, int32_t n_views, const int32_t* view_ids,
+                                  const splat_raster_options* options);
+/* Image of batch slot `slot` (3*W*H rgb, W*H transmittance, W*H last
+ * contributor relative to the tile's list); any pointer may be null. */
+splat_status splat_download_image(splat_context* ctx, int32_t slot, float* rgb, float* transmittance,
+                                  int32_t* last_contributor);
+/* Per projected splat of batch slot `slot` in depth order (gaussian index,
+ * mean, conic, depth, radius, colour, opacity, clamp flags, 16x16 tile rect);
+ * returns the count through *count. Arrays sized for splat_count(). */
+splat_status splat_download_projection(splat_context* ctx, int32_t slot, int32_t* count, int32_t* index,
+                                       float* mean2, float* conic3, float* depth, float* radius,
+                                       float* color3, float* opacity, int32_t* clamped3, int32_t* rect4);
+/* Per-tile lists of batch slot `slot` over 16x16 tiles: offsets (tiles+1),
+ * entries (gaussian indices); *entry_count set; entries may be null to query. */
+splat_status splat_download_tiles(splat_context* ctx, int32_t slot, int32_t* offsets, int32_t* entries,
+                                  int32_t* entry_count);
+/* Photometric loss (1-w) L1 + w (1-SSIM) of every batch view against its
+ * target; writes dL/drgb for the backward and the per-view losses. */
+splat_status splat_loss(splat_context* ctx, float ssim_weight, float* losses);
+/* Replace dL/drgb of batch slot `slot` (3*W*H) -- for objectives other than
+ * the photometric loss. */
+splat_status splat_set_image_gradient(splat_context* ctx, int32_t slot, const float* dL_drgb);
+/* dL/drgb of batch slot `slot` as the loss (or the caller) left it (3*W*H). */
+splat_status splat_download_image_gradient(splat_context* ctx, int32_t slot, float* dL_drgb);
+/* Adjoint of the last forward: adds parameter gradients, screen gradients and
+ * visible counts into the device gradient buffer (rasterizeSplatsBackward
+ * semantics: gradients are added, not overwritten). */
+splat_status splat_render_backward(splat_context* ctx);
+splat_status splat_download_gradients(splat_context* ctx, splat_gradients* out);
+splat_status splat_zero_gradients(splat_context* ctx);
+
+/* ---- optimiser and fused step ---- */
+/* One Adam step over every parameter; gradients are zeroed after use and the
+ * screen gradients / visible counts are added into the densify statistics. */
+splat_status splat_adam_step(splat_context* ctx, const splat_adam_params* params);
+/* renderAndAccumulate over the listed views + Adam: forward, loss, adjoint,
+ * Adam. losses: n_views floats (nullable). */
+splat_status splat_train_step(splat_context* ctx, int32_t n_views, const int32_t* view_ids,
+                              const splat_raster_options* options, float ssim_weight,
+                              const splat_adam_params* adam, float* losses);
+
+/* ---- adaptive density ---- */
+/* densifyAndPrune + remapState on the device. rng_state is advanced exactly
+ * as the CPU trainer advances it. source_index (nullable, sized for the new
+ * count) receives the old index or -1; counts = {cloned, split, pruned}. */
+splat_status splat_densify(splat_context* ctx, const splat_densify_params* params, float extent,
+                           int32_t iteration, uint32_t* rng_state, int32_t* source_index, int32_t* counts);
+/* resetOpacity: logit = min(logit, ceiling); opacity moments zeroed. */
+splat_status splat_reset_opacity(splat_context* ctx, float ceiling);
+/* Accumulated densify statistics since the last densify (or upload). */
+splat_status splat_download_stats(splat_context* ctx, float* screen_gradient, int32_t* visible_count);
+/* Prune (opacity < min_opacity, non-finite or > 1e3 scale) and stable sort by
+ * descending opacity logit (GaussianCloud.cpp:177-197), on the device. */
+splat_status splat_prune_and_sort(splat_context* ctx, float min_opacity);
+
+/* ---- measurement ---- */
+/* Per-kernel timing with HIP events on the context's stream: enable, then
+ * read the accumulated milliseconds and launch counts by kernel name. */
+splat_status splat_set_timing(splat_context* ctx, int32_t enabled);
+int32_t splat_timing_count(const splat_context* ctx);
+splat_status splat_timing_entry(splat_context* ctx, int32_t i, const char** name, double* total_ms,
+                                int64_t* launches);
+splat_status splat_reset_timing(splat_context* ctx);
+/* Number of (tile, splat) pairs of the last forward batch. */
+int64_t splat_last_pair_count(const splat_context* ctx);
+splat_status splat_synchronize(splat_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

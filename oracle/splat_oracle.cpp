@@ -1309,10 +1309,14 @@ int orc_densify(orc_cloud* cloud, int capacity, const float* screen_gradient, co
         rotationOf(cloud->rot + 4 * g, Rm);
         for (int child = 0; child < 2; ++child) {
           const int k = push(g);
+          // The reference draws the three normals as the arguments of one
+          // Vec3f constructor (SplatDensify.cpp:62-63), whose evaluation order
+          // C++ leaves unspecified; its GCC build evaluates them last to first,
+          // which the golden vectors pin.
           float sample[3];
-          sample[0] = nextNormal(rng_state) * scale[0];
-          sample[1] = nextNormal(rng_state) * scale[1];
           sample[2] = nextNormal(rng_state) * scale[2];
+          sample[1] = nextNormal(rng_state) * scale[1];
+          sample[0] = nextNormal(rng_state) * scale[0];
           for (int i = 0; i < 3; ++i)
             add.pos[3 * static_cast<size_t>(k) + i] =
                 cloud->pos[3 * g + i] + (Rm[i][0] * sample[0] + Rm[i][1] * sample[1] + Rm[i][2] * sample[2]);

@@ -1,0 +1,287 @@
+// Per-gaussian adjoint: screen-space gradients back to the parameters.
+//
+// The per-gaussian half of the engine's rasterizeSplatsBackward
+// (engine/native/cpp/src/splat/SplatRaster.cpp:435-576): opacity through the
+// sigmoid, colour through the SH clamp into the coefficients and the view
+// direction, the mean through the pinhole, the conic through -A^-1 dA A^-1
+// into the 2D covariance, then into the 3D covariance and the frustum-clamped
+// Jacobian (with the halved z-derivative of a clamped entry), scale, and the
+// normalised quaternion. One thread per gaussian; it loops over the views of
+// the batch, so gradients from all views are reduced in registers and written
+// once -- no atomics on parameter gradients. In deterministic mode the
+// screen-space terms of each (view, gaussian) are summed from that splat's
+// per-tile partials in pair order. Gradients are added (the engine's
+// accumulate semantics). HBM-bound: reads params (56 + 4 rs B) and 48 B of
+// screen terms per (view, gaussian), read-modify-writes gradients.
+#include "splat_common.h"
+#include "splat_kernels.h"
+
+namespace splat {
+namespace {
+
+constexpr float kShC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                            -1.0925484305920792f, 0.5462742152960396f};
+constexpr float kShC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                            0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                            -0.5900435899266435f};
+
+__device__ __forceinline__ int sh_basis(int degree, const float d[3], float* b) {
+  if (degree < 1) return 0;
+  const float x = d[0], y = d[1], z = d[2];
+  b[0] = -kShC1 * y;
+  b[1] = kShC1 * z;
+  b[2] = -kShC1 * x;
+  if (degree < 2) return 3;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float xy = x * y, yz = y * z, xz = x * z;
+  b[3] = kShC2[0] * xy;
+  b[4] = kShC2[1] * yz;
+  b[5] = kShC2[2] * (2.f * zz - xx - yy);
+  b[6] = kShC2[3] * xz;
+  b[7] = kShC2[4] * (xx - yy);
+  if (degree < 3) return 8;
+  b[8] = kShC3[0] * y * (3.f * xx - yy);
+  b[9] = kShC3[1] * xy * z;
+  b[10] = kShC3[2] * y * (4.f * zz - xx - yy);
+  b[11] = kShC3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
+  b[12] = kShC3[4] * x * (4.f * zz - xx - yy);
+  b[13] = kShC3[5] * z * (xx - yy);
+  b[14] = kShC3[6] * x * (xx - 3.f * yy);
+  return 15;
+}
+
+// d(basis_k)/d(direction), SplatRaster.cpp:55-76.
+__device__ __forceinline__ void sh_jacobian(int count, const float d[3], float j[15][3]) {
+  const float x = d[0], y = d[1], z = d[2];
+  auto set = [&](int k, float s, float a, float b, float c) {
+    j[k][0] = s * a;
+    j[k][1] = s * b;
+    j[k][2] = s * c;
+  };
+  set(0, 1.f, 0.f, -kShC1, 0.f);
+  set(1, 1.f, 0.f, 0.f, kShC1);
+  set(2, 1.f, -kShC1, 0.f, 0.f);
+  if (count < 8) return;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  set(3, kShC2[0], y, x, 0.f);
+  set(4, kShC2[1], 0.f, z, y);
+  set(5, kShC2[2], -2.f * x, -2.f * y, 4.f * z);
+  set(6, kShC2[3], z, 0.f, x);
+  set(7, kShC2[4], 2.f * x, -2.f * y, 0.f);
+  if (count < 15) return;
+  set(8, kShC3[0], 6.f * x * y, 3.f * xx - 3.f * yy, 0.f);
+  set(9, kShC3[1], y * z, x * z, x * y);
+  set(10, kShC3[2], -2.f * x * y, 4.f * zz - xx - 3.f * yy, 8.f * y * z);
+  set(11, kShC3[3], -6.f * x * z, -6.f * y * z, 6.f * zz - 3.f * xx - 3.f * yy);
+  set(12, kShC3[4], 4.f * zz - 3.f * xx - yy, -2.f * x * y, 8.f * x * z);
+  set(13, kShC3[5], 2.f * x * z, -2.f * y * z, xx - yy);
+  set(14, kShC3[6], 3.f * xx - 3.f * yy, -6.f * x * y, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc* __restrict__ views,
+                                                 const float* __restrict__ params, const SplatRec* __restrict__ rec,
+                                                 const float* __restrict__ screen,
+                                                 const uint32_t* __restrict__ pair_base,
+                                                 const uint32_t* __restrict__ counts,
+                                                 const float* __restrict__ partial, float* __restrict__ grads,
+                                                 float* __restrict__ sg, int* __restrict__ vc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= p.n) return;
+  const long long cap = p.cap;
+  const int rs = p.rest_stride;
+  const float pos[3] = {params[3 * g], params[3 * g + 1], params[3 * g + 2]};
+  const float ls[3] = {params[3 * cap + 3 * g], params[3 * cap + 3 * g + 1], params[3 * cap + 3 * g + 2]};
+  const float* rp = params + 6 * cap + 4 * (long long)g;
+  const float w0 = rp[0], x0 = rp[1], y0 = rp[2], z0 = rp[3];
+  const float* rest = params + 14 * cap + (long long)rs * g;
+  float* grest = grads + 14 * cap + (long long)rs * g;
+
+  // covariance3d pieces (SplatRaster.cpp:78-87, 539-541)
+  const float sqn = x0 * x0 + y0 * y0 + z0 * z0 + w0 * w0;
+  float qw = w0, qx = x0, qy = y0, qz = z0;
+  if (sqn > 0.f) {
+    const float s = __builtin_sqrtf(sqn);
+    qx = qx / s; qy = qy / s; qz = qz / s; qw = qw / s;
+  }
+  float R[3][3];
+  {
+    const float tx = 2.f * qx, ty = 2.f * qy, tz = 2.f * qz;
+    const float twx = tx * qw, twy = ty * qw, twz = tz * qw;
+    const float txx = tx * qx, txy = ty * qx, txz = tz * qx;
+    const float tyy = ty * qy, tyz = tz * qy, tzz = tz * qz;
+    R[0][0] = 1.f - (tyy + tzz); R[0][1] = txy - twz; R[0][2] = txz + twy;
+    R[1][0] = txy + twz; R[1][1] = 1.f - (txx + tzz); R[1][2] = tyz - twx;
+    R[2][0] = txz - twy; R[2][1] = tyz + twx; R[2][2] = 1.f - (txx + tyy);
+  }
+  const float sc[3] = {det_exp(ls[0]), det_exp(ls[1]), det_exp(ls[2])};
+  float M[3][3], S[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) M[i][j] = R[i][j] * sc[j];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
+  const float raw_norm = fmaxf(__builtin_sqrtf(sqn), 1e-12f);
+  const int per_channel = rs / 3;
+
+  float gpos[3] = {0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f}, grot[4] = {0.f, 0.f, 0.f, 0.f};
+  float gop = 0.f, gdc[3] = {0.f, 0.f, 0.f};
+  float sg_sum = 0.f;
+  int vc_sum = 0;
+
+  for (int v = 0; v < p.n_views; ++v) {
+    const long long slot = (long long)v * p.n + g;
+    const SplatRec& r = rec[slot];
+    if (!(r.flags & kProjected)) continue;
+    const ViewDesc& V = views[v];
+    vc_sum += 1;
+    float acc[11];
+    if (p.deterministic) {
+#pragma unroll
+      for (int k = 0; k < 11; ++k) acc[k] = 0.f;
+      const uint32_t base = pair_base[slot], cnt = counts[slot];
+      for (uint32_t q = 0; q < cnt; ++q) {
+        const float* src = partial + (long long)(base + q) * kScreenStride;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) acc[k] += src[k];
+      }
+    } else {
+      const float* src = screen + slot * kScreenStride;
+#pragma unroll
+      for (int k = 0; k < 11; ++k) acc[k] = src[k];
+    }
+    sg_sum += acc[6];
+    if (acc[10] == 0.f) continue;  // not touched
+
+    const float opacity = r.opacity;
+    gop += acc[3] * opacity * (1.f - opacity);
+    float dcol[3] = {acc[0], acc[1], acc[2]};
+    for (int c = 0; c < 3; ++c) {
+      if (r.flags & (1u << c)) dcol[c] = 0.f;
+      gdc[c] += kShC0 * dcol[c];
+    }
+    float ddir[3] = {0.f, 0.f, 0.f};
+    const float off[3] = {pos[0] - V.C[0], pos[1] - V.C[1], pos[2] - V.C[2]};
+    const float dist = __builtin_sqrtf(off[0] * off[0] + off[1] * off[1] + off[2] * off[2]);
+    if (p.sh_use >= 1 && dist > 1e-9f && per_channel > 0) {
+      const float dir[3] = {off[0] / dist, off[1] / dist, off[2] / dist};
+      float basis[15], jac[15][3];
+      const int count = sh_basis(p.sh_use, dir, basis);
+      const int usable = count < per_channel ? count : per_channel;
+      sh_jacobian(count, dir, jac);
+      for (int c = 0; c < 3; ++c) {
+        if (dcol[c] == 0.f) continue;
+        const float* rr = rest + c * per_channel;
+        float* dst = grest + c * per_channel;
+        for (int b = 0; b < usable; ++b) {
+          dst[b] += basis[b] * dcol[c];
+          const float k = rr[b] * dcol[c];
+          ddir[0] += jac[b][0] * k;
+          ddir[1] += jac[b][1] * k;
+          ddir[2] += jac[b][2] * k;
+        }
+      }
+    }
+    const float dmx = acc[4], dmy = acc[5];
+    float pc[3];
+    for (int i = 0; i < 3; ++i) pc[i] = V.W[3 * i] * pos[0] + V.W[3 * i + 1] * pos[1] + V.W[3 * i + 2] * pos[2] + V.t[i];
+    const float z = pc[2];
+    float dp[3] = {V.fx / z * dmx, V.fy / z * dmy, -(V.fx * pc[0] * dmx + V.fy * pc[1] * dmy) / (z * z)};
+    // conic -> cov2d: D = -(C G C)
+    const float C[2][2] = {{r.ca, r.cb}, {r.cb, r.cc}};
+    const float Gm[2][2] = {{acc[7], 0.5f * acc[8]}, {0.5f * acc[8], acc[9]}};
+    float CG[2][2], D[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) CG[i][j] = C[i][0] * Gm[0][j] + C[i][1] * Gm[1][j];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) D[i][j] = -(CG[i][0] * C[0][j] + CG[i][1] * C[1][j]);
+    // projection Jacobian at the clamped point
+    const float lx = V.thx * kSlack, ly = V.thy * kSlack;
+    const float qx_ = pc[0] / z, qy_ = pc[1] / z;
+    const float jxJ = ((qx_ < -lx) ? -lx : ((lx < qx_) ? lx : qx_)) * z;
+    const float jyJ = ((qy_ < -ly) ? -ly : ((ly < qy_) ? ly : qy_)) * z;
+    const float J00 = V.fx / z, J02 = -V.fx * jxJ / (z * z);
+    const float J11 = V.fy / z, J12 = -V.fy * jyJ / (z * z);
+    float T[2][3];
+    for (int c = 0; c < 3; ++c) {
+      T[0][c] = J00 * V.W[c] + 0.f * V.W[3 + c] + J02 * V.W[6 + c];
+      T[1][c] = 0.f * V.W[c] + J11 * V.W[3 + c] + J12 * V.W[6 + c];
+    }
+    float TtD[3][2], dS3[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 2; ++j) TtD[i][j] = T[0][i] * D[0][j] + T[1][i] * D[1][j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) dS3[i][j] = TtD[i][0] * T[0][j] + TtD[i][1] * T[1][j];
+    float D2T[2][3], dT[2][3], dJ[2][3];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 3; ++j) D2T[i][j] = (2.f * D[i][0]) * T[0][j] + (2.f * D[i][1]) * T[1][j];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 3; ++j) dT[i][j] = D2T[i][0] * S[0][j] + D2T[i][1] * S[1][j] + D2T[i][2] * S[2][j];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 3; ++j)
+        dJ[i][j] = dT[i][0] * V.W[3 * j] + dT[i][1] * V.W[3 * j + 1] + dT[i][2] * V.W[3 * j + 2];
+    const bool cx = (r.flags & 8u) != 0, cy = (r.flags & 16u) != 0;
+    const float jx = cx ? copysignf(lx, pc[0]) * z : pc[0];
+    const float jy = cy ? copysignf(ly, pc[1]) * z : pc[1];
+    if (!cx) dp[0] += dJ[0][2] * (-V.fx / (z * z));
+    if (!cy) dp[1] += dJ[1][2] * (-V.fy / (z * z));
+    const float dz_x = (cx ? 1.f : 2.f) * V.fx * jx / (z * z * z);
+    const float dz_y = (cy ? 1.f : 2.f) * V.fy * jy / (z * z * z);
+    dp[2] += dJ[0][0] * (-V.fx / (z * z)) + dJ[1][1] * (-V.fy / (z * z)) + dJ[0][2] * dz_x + dJ[1][2] * dz_y;
+    float dpos[3];
+    for (int i = 0; i < 3; ++i) dpos[i] = V.W[i] * dp[0] + V.W[3 + i] * dp[1] + V.W[6 + i] * dp[2];
+    const float ddsq = ddir[0] * ddir[0] + ddir[1] * ddir[1] + ddir[2] * ddir[2];
+    if (dist > 1e-9f && ddsq > 0.f) {
+      const float dir[3] = {off[0] / dist, off[1] / dist, off[2] / dist};
+      const float dot = dir[0] * ddir[0] + dir[1] * ddir[1] + dir[2] * ddir[2];
+      for (int i = 0; i < 3; ++i) dpos[i] += (ddir[i] - dir[i] * dot) / dist;
+    }
+    for (int i = 0; i < 3; ++i) gpos[i] += dpos[i];
+    // cov3d -> scale and rotation
+    float dM[3][3];
+    {
+      float sym[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sym[i][j] = 0.5f * (dS3[i][j] + dS3[j][i]);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          dM[i][j] = 2.f * sym[i][0] * M[0][j] + 2.f * sym[i][1] * M[1][j] + 2.f * sym[i][2] * M[2][j];
+    }
+    for (int a = 0; a < 3; ++a) gls[a] += (R[0][a] * dM[0][a] + R[1][a] * dM[1][a] + R[2][a] * dM[2][a]) * sc[a];
+    float dR[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) dR[i][j] = dM[i][j] * sc[j];
+    float dq[4];
+    dq[0] = 2.f * (-qz * dR[0][1] + qy * dR[0][2] + qz * dR[1][0] - qx * dR[1][2] - qy * dR[2][0] + qx * dR[2][1]);
+    dq[1] = 2.f * (qy * dR[0][1] + qz * dR[0][2] + qy * dR[1][0] - 2.f * qx * dR[1][1] - qw * dR[1][2] +
+                   qz * dR[2][0] + qw * dR[2][1] - 2.f * qx * dR[2][2]);
+    dq[2] = 2.f * (-2.f * qy * dR[0][0] + qx * dR[0][1] + qw * dR[0][2] + qx * dR[1][0] + qz * dR[1][2] -
+                   qw * dR[2][0] + qz * dR[2][1] - 2.f * qy * dR[2][2]);
+    dq[3] = 2.f * (-2.f * qz * dR[0][0] - qw * dR[0][1] + qx * dR[0][2] + qw * dR[1][0] - 2.f * qz * dR[1][1] +
+                   qy * dR[1][2] + qx * dR[2][0] + qy * dR[2][1]);
+    const float unit[4] = {qw, qx, qy, qz};
+    const float ud = unit[0] * dq[0] + unit[1] * dq[1] + unit[2] * dq[2] + unit[3] * dq[3];
+    for (int a = 0; a < 4; ++a) grot[a] += (dq[a] - unit[a] * ud) / raw_norm;
+  }
+  if (vc_sum == 0) return;
+  for (int i = 0; i < 3; ++i) {
+    grads[3 * g + i] += gpos[i];
+    grads[3 * cap + 3 * g + i] += gls[i];
+    grads[11 * cap + 3 * g + i] += gdc[i];
+  }
+  for (int a = 0; a < 4; ++a) grads[6 * cap + 4 * (long long)g + a] += grot[a];
+  grads[10 * cap + g] += gop;
+  sg[g] += sg_sum;
+  vc[g] += vc_sum;
+}
+
+}  // namespace
+
+hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
+                          const float* screen, const uint32_t* pair_base, const uint32_t* counts,
+                          const float* partial, float* grads, float* sg, int* vc, hipStream_t stream) {
+  if (p.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adjoint, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, views, params, rec, screen,
+                     pair_base, counts, partial, grads, sg, vc);
+  return hipGetLastError();
+}
+
+}  // namespace splat

@@ -1,0 +1,830 @@
+// C ABI of the splat trainer: context, device buffers, and the per-batch
+// launch sequence (projection -> depth sort -> pair emit -> tile sort ->
+// ranges -> forward -> loss -> adjoint -> per-gaussian adjoint -> Adam).
+// Host code only; the kernels live in project/sort/raster/loss/adjoint/adam.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/splat_hip.h"
+#include "splat_common.h"
+#include "splat_context.h"
+#include "splat_kernels.h"
+
+using namespace splat;
+
+namespace {
+
+thread_local std::string g_no_context_error;
+
+splat_status fail(splat_context* ctx, splat_status st, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  else g_no_context_error = msg;
+  return st;
+}
+
+splat_status hip_fail(splat_context* ctx, hipError_t e, const char* what) {
+  std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, e == hipErrorOutOfMemory ? SPLAT_ERROR_OUT_OF_MEMORY : SPLAT_ERROR_DEVICE, msg);
+}
+
+#define CK(expr, what)                                   \
+  do {                                                   \
+    hipError_t _e = (expr);                              \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
+  } while (0)
+
+// Float view as the projection uses it (SplatRaster.cpp:206-212).
+ViewDesc make_desc(const splat_view& v) {
+  ViewDesc d{};
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) d.W[3 * i + j] = static_cast<float>(v.R[3 * i + j]);
+    d.t[i] = static_cast<float>(v.t[i]);
+    const double c = (-v.R[0 * 3 + i]) * v.t[0] + (-v.R[1 * 3 + i]) * v.t[1] + (-v.R[2 * 3 + i]) * v.t[2];
+    d.C[i] = static_cast<float>(c);
+  }
+  d.fx = static_cast<float>(v.fx);
+  d.fy = static_cast<float>(v.fy);
+  d.cx = static_cast<float>(v.cx);
+  d.cy = static_cast<float>(v.cy);
+  d.thx = static_cast<float>(0.5 * v.width / v.fx);
+  d.thy = static_cast<float>(0.5 * v.height / v.fy);
+  d.width = v.width;
+  d.height = v.height;
+  d.cols = (v.width + kTile - 1) / kTile;
+  d.rows = (v.height + kTile - 1) / kTile;
+  return d;
+}
+
+int bit_width(unsigned long long x) {
+  int b = 0;
+  while (x) {
+    ++b;
+    x >>= 1;
+  }
+  return b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- buffers --
+hipError_t DeviceBuffer::reserve(size_t want) {
+  if (want <= bytes) return hipSuccess;
+  if (ptr) {
+    hipError_t e = hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (e != hipSuccess) return e;
+  }
+  size_t grow = want + want / 4 + 256;
+  hipError_t e = hipMalloc(&ptr, grow);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return e;
+  }
+  bytes = grow;
+  return hipSuccess;
+}
+
+void DeviceBuffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+splat_context::~splat_context() {
+  for (DeviceBuffer* b : all_buffers()) b->release();
+  for (auto& kv : timers) {
+    for (auto& pr : kv.second.pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  }
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+std::vector<DeviceBuffer*> splat_context::all_buffers() {
+  return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
+          &vals_alt, &counts, &offsets, &pair_base, &pair_keys, &pair_vals, &pair_keys_alt, &pair_vals_alt, &ranges,
+          &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &scan_ws, &hist_ws, &scalars,
+          &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src};
+}
+
+// ------------------------------------------------------------- timing --
+hipError_t splat_context::timed_begin(const char* name, hipEvent_t* start) {
+  if (!timing) return hipSuccess;
+  SPLAT_HIP_CHECK(hipEventCreate(start));
+  (void)name;
+  return hipEventRecord(*start, stream);
+}
+
+hipError_t splat_context::timed_end(const char* name, hipEvent_t start) {
+  if (!timing) return hipSuccess;
+  hipEvent_t stop;
+  SPLAT_HIP_CHECK(hipEventCreate(&stop));
+  SPLAT_HIP_CHECK(hipEventRecord(stop, stream));
+  timers[name].pending.push_back({start, stop});
+  return hipSuccess;
+}
+
+hipError_t splat_context::collect_timing() {
+  for (auto& kv : timers) {
+    for (auto& pr : kv.second.pending) {
+      SPLAT_HIP_CHECK(hipEventSynchronize(pr.second));
+      float ms = 0.f;
+      SPLAT_HIP_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+      kv.second.total_ms += ms;
+      kv.second.launches += 1;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    kv.second.pending.clear();
+  }
+  return hipSuccess;
+}
+
+#define TIMED(name, call)                                           \
+  do {                                                              \
+    hipEvent_t _t0 = nullptr;                                       \
+    CK(ctx->timed_begin(name, &_t0), "event");                      \
+    CK((call), name);                                               \
+    CK(ctx->timed_end(name, _t0), "event");                         \
+  } while (0)
+
+// ------------------------------------------------------------- pipeline --
+namespace {
+
+splat_status check_ready(splat_context* ctx) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  if (ctx->cap <= 0) return fail(ctx, SPLAT_ERROR_STATE, "no capacity reserved: call splat_set_capacity first");
+  return SPLAT_OK;
+}
+
+splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_ids, const splat_raster_options* o) {
+  if (n_views <= 0 || n_views > kMaxBatchViews || !view_ids || !o)
+    return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: 1..64 views and options required");
+  for (int k = 0; k < n_views; ++k)
+    if (view_ids[k] < 0 || view_ids[k] >= (int)ctx->views.size())
+      return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: view id out of range");
+  ctx->opts = *o;
+  ctx->batch_ids.assign(view_ids, view_ids + n_views);
+  ctx->batch_desc.clear();
+  int tiles = 0;
+  long long px = 0;
+  for (int k = 0; k < n_views; ++k) {
+    ViewDesc d = make_desc(ctx->views[view_ids[k]]);
+    d.tile_base = tiles;
+    d.pixel_base = px;
+    tiles += d.cols * d.rows;
+    px += (long long)d.width * d.height;
+    ctx->batch_desc.push_back(d);
+  }
+  ctx->total_tiles = tiles;
+  ctx->total_px = px;
+  ctx->sh_use = std::min(o->sh_degree, ctx->sh_degree);
+  ctx->forward_done = false;
+  ctx->loss_done = false;
+  const long long n = ctx->n;
+  const long long M = n * n_views;
+  ctx->M = M;
+  CK(ctx->d_views.reserve(sizeof(ViewDesc) * n_views), "alloc views");
+  CK(hipMemcpyAsync(ctx->d_views.ptr, ctx->batch_desc.data(), sizeof(ViewDesc) * n_views, hipMemcpyHostToDevice,
+                    ctx->stream),
+     "upload views");
+  CK(ctx->rgb.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc image");
+  CK(ctx->T.reserve(sizeof(float) * std::max(1LL, px)), "alloc image");
+  CK(ctx->last.reserve(sizeof(int) * std::max(1LL, px)), "alloc image");
+  CK(ctx->ranges.reserve(sizeof(uint32_t) * 2 * std::max(1, tiles)), "alloc ranges");
+  CK(ctx->scalars.reserve(64), "alloc scalars");
+  const long long Mc = std::max(1LL, M);
+  CK(ctx->rec.reserve(sizeof(SplatRec) * Mc), "alloc records");
+  CK(ctx->keys.reserve(4 * Mc), "alloc keys");
+  CK(ctx->vals.reserve(4 * Mc), "alloc keys");
+  CK(ctx->keys_alt.reserve(4 * Mc), "alloc keys");
+  CK(ctx->vals_alt.reserve(4 * Mc), "alloc keys");
+  CK(ctx->counts.reserve(4 * Mc), "alloc counts");
+  CK(ctx->offsets.reserve(4 * Mc), "alloc offsets");
+  CK(ctx->pair_base.reserve(4 * Mc), "alloc pair base");
+  CK(ctx->ensure_sort_workspace(std::max(M, 256LL * 4096)), "alloc sort workspace");
+
+  TIMED("project", launch_project(ctx->params.f(), ctx->cap, (int)n, ctx->rs, ctx->sh_use, ctx->views_dev(), n_views,
+                                  o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(), ctx->vals.u(),
+                                  ctx->counts.u(), ctx->stream));
+  bool alt = false;
+  TIMED("depth_sort", launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
+                                        ctx->sort_ws, &alt, ctx->stream));
+  uint32_t* sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
+  uint32_t* d_total = ctx->scalars.u();
+  TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), sorted_slots, ctx->offsets.u(), M, d_total,
+                                           ctx->sort_ws.scan, ctx->stream));
+  uint32_t P = 0;
+  if (M > 0) {
+    CK(hipMemcpyAsync(&P, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "read pair count");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+  }
+  ctx->P = P;
+  const long long Pc = std::max(1LL, (long long)P);
+  CK(ctx->pair_keys.reserve(4 * Pc), "alloc pairs");
+  CK(ctx->pair_vals.reserve(4 * Pc), "alloc pairs");
+  CK(ctx->pair_keys_alt.reserve(4 * Pc), "alloc pairs");
+  CK(ctx->pair_vals_alt.reserve(4 * Pc), "alloc pairs");
+  CK(ctx->ensure_sort_workspace(std::max((long long)P, M)), "alloc sort workspace");
+  TIMED("emit_pairs", launch_emit_pairs(sorted_slots, ctx->offsets.u(), ctx->counts.u(), ctx->recs(), ctx->views_dev(),
+                                        n_views, n, M, ctx->pair_keys.u(), ctx->pair_vals.u(), ctx->pair_base.u(),
+                                        ctx->stream));
+  const int tile_bits = std::max(1, bit_width((unsigned long long)std::max(0, tiles - 1)));
+  bool palt = false;
+  TIMED("tile_sort", launch_radix_sort(ctx->pair_keys.u(), ctx->pair_vals.u(), ctx->pair_keys_alt.u(),
+                                       ctx->pair_vals_alt.u(), P, 0, tile_bits, ctx->sort_ws, &palt, ctx->stream));
+  ctx->sorted_pair_keys = palt ? ctx->pair_keys_alt.u() : ctx->pair_keys.u();
+  ctx->sorted_pair_vals = palt ? ctx->pair_vals_alt.u() : ctx->pair_vals.u();
+  ctx->sorted_slots = sorted_slots;
+  TIMED("tile_ranges", launch_tile_ranges(ctx->sorted_pair_keys, P, ctx->ranges.u(), tiles, ctx->stream));
+  RasterParams rp{};
+  for (int c = 0; c < 3; ++c) rp.bg[c] = o->background[c];
+  rp.min_alpha = o->min_alpha;
+  rp.cutoff = o->transmittance_cutoff;
+  rp.n_views = n_views;
+  rp.total_tiles = tiles;
+  rp.per_view = n;
+  ctx->raster = rp;
+  TIMED("forward", launch_forward(rp, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
+                                  ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->stream));
+  ctx->forward_done = true;
+  return SPLAT_OK;
+}
+
+splat_status loss_impl(splat_context* ctx, float w, float* losses) {
+  if (!ctx->forward_done) return fail(ctx, SPLAT_ERROR_STATE, "loss: no forward batch");
+  for (int id : ctx->batch_ids)
+    if (!ctx->has_target[id]) return fail(ctx, SPLAT_ERROR_STATE, "loss: a batch view has no target image");
+  const long long px = ctx->total_px;
+  const int V = (int)ctx->batch_ids.size();
+  CK(ctx->dL.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc dL");
+  CK(ctx->block_sums.reserve(sizeof(float) * 2 * std::max(1, ctx->total_tiles)), "alloc loss");
+  CK(ctx->loss_sums.reserve(sizeof(float) * 2 * V), "alloc loss");
+  if (w > 0.f) CK(ctx->terms.reserve(sizeof(float) * 15 * std::max(1LL, px)), "alloc ssim terms");
+  // Gather the batch's targets into batch pixel order.
+  CK(ctx->batch_targets.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc targets");
+  for (int k = 0; k < V; ++k) {
+    const ViewDesc& d = ctx->batch_desc[k];
+    CK(hipMemcpyAsync(ctx->batch_targets.f() + 3 * d.pixel_base, ctx->targets.f() + ctx->target_offset[ctx->batch_ids[k]],
+                      sizeof(float) * 3 * (size_t)d.width * d.height, hipMemcpyDeviceToDevice, ctx->stream),
+       "gather targets");
+  }
+  LossParams lp{};
+  for (int i = 0; i < 11; ++i) lp.kernel[i] = ctx->ssim_kernel[i];
+  lp.ssim_weight = w;
+  lp.n_views = V;
+  lp.total_tiles = ctx->total_tiles;
+  TIMED("loss_forward", launch_loss_forward(lp, ctx->views_dev(), ctx->rgb.f(), ctx->batch_targets.f(), ctx->dL.f(),
+                                            ctx->terms.f(), px, ctx->block_sums.f(), ctx->stream));
+  TIMED("loss_backward", launch_loss_backward(lp, ctx->views_dev(), ctx->rgb.f(), ctx->batch_targets.f(),
+                                              ctx->terms.f(), px, ctx->dL.f(), ctx->stream));
+  CK(launch_loss_reduce(ctx->views_dev(), V, ctx->block_sums.f(), ctx->loss_sums.f(), ctx->stream), "loss reduce");
+  ctx->loss_done = true;
+  ctx->last_ssim_weight = w;
+  if (losses) {
+    std::vector<float> sums(2 * V);
+    CK(hipMemcpyAsync(sums.data(), ctx->loss_sums.f(), sizeof(float) * 2 * V, hipMemcpyDeviceToHost, ctx->stream),
+       "read loss");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+    for (int k = 0; k < V; ++k) {
+      const ViewDesc& d = ctx->batch_desc[k];
+      const float inverse = 1.f / static_cast<float>((long long)d.width * d.height * 3);
+      const float l1 = sums[2 * k] * inverse;
+      losses[k] = w <= 0.f ? l1 : (1.f - w) * l1 + w * (1.f - sums[2 * k + 1] * inverse);
+    }
+  }
+  return SPLAT_OK;
+}
+
+splat_status backward_impl(splat_context* ctx) {
+  if (!ctx->forward_done) return fail(ctx, SPLAT_ERROR_STATE, "render_backward: no forward batch");
+  if (!ctx->loss_done) return fail(ctx, SPLAT_ERROR_STATE, "render_backward: no image gradient (run splat_loss)");
+  const long long M = ctx->M;
+  const bool det = ctx->opts.deterministic != 0;
+  if (det) {
+    CK(ctx->partial.reserve(sizeof(float) * kScreenStride * std::max<long long>(1, ctx->P)), "alloc partials");
+  } else {
+    CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, M)), "alloc screen");
+    TIMED("screen_clear", launch_fill_f32(ctx->screen.f(), 0.f, kScreenStride * M, ctx->stream));
+  }
+  TIMED("backward", launch_backward(ctx->raster, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
+                                    ctx->dL.f(), ctx->T.f(), ctx->last.i(), ctx->screen.f(), ctx->pair_base.u(),
+                                    ctx->partial.f(), det, ctx->stream));
+  AdjointParams ap{};
+  ap.cap = ctx->cap;
+  ap.n = ctx->n;
+  ap.rest_stride = ctx->rs;
+  ap.sh_use = ctx->sh_use;
+  ap.n_views = (int)ctx->batch_ids.size();
+  ap.deterministic = det ? 1 : 0;
+  TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
+                                  ctx->pair_base.u(), ctx->counts.u(), ctx->partial.f(), ctx->grads.f(), ctx->sg.f(),
+                                  ctx->vc.i(), ctx->stream));
+  return SPLAT_OK;
+}
+
+splat_status adam_impl(splat_context* ctx, const splat_adam_params* a) {
+  AdamParams p{};
+  p.lr[0] = a->lr_position;
+  p.lr[1] = a->lr_log_scale;
+  p.lr[2] = a->lr_rotation;
+  p.lr[3] = a->lr_opacity;
+  p.lr[4] = a->lr_dc;
+  p.lr[5] = a->lr_rest;
+  p.beta1 = a->beta1;
+  p.beta2 = a->beta2;
+  p.eps = a->epsilon;
+  p.c1 = a->correction1;
+  p.c2 = a->correction2;
+  p.cap = ctx->cap;
+  p.n = ctx->n;
+  p.rest_stride = ctx->rs;
+  TIMED("adam", launch_adam(p, ctx->params.f(), ctx->grads.f(), ctx->m.f(), ctx->v.f(), ctx->sg.f(), ctx->vc.i(),
+                            ctx->stat_sg.f(), ctx->stat_vc.i(), ctx->stream));
+  return SPLAT_OK;
+}
+
+}  // namespace
+
+hipError_t splat_context::ensure_sort_workspace(long long max_n) {
+  if (max_n <= sort_ws.scan.max_n && sort_hist_elems(max_n) <= sort_ws.hist_capacity) return hipSuccess;
+  const long long hist_elems = sort_hist_elems(max_n);
+  // the scan workspace must cover both the histogram table and max_n items
+  const long long scan_n = std::max(max_n, hist_elems);
+  SPLAT_HIP_CHECK(hist_ws.reserve(sizeof(uint32_t) * hist_elems));
+  SPLAT_HIP_CHECK(scan_ws.reserve(sizeof(uint32_t) * scan_workspace_elems(scan_n)));
+  sort_ws.hist = hist_ws.u();
+  sort_ws.hist_capacity = (long long)(hist_ws.bytes / sizeof(uint32_t));
+  sort_ws.scan.buf = scan_ws.u();
+  sort_ws.scan.capacity = (long long)(scan_ws.bytes / sizeof(uint32_t));
+  sort_ws.scan.max_n = scan_n;
+  return hipSuccess;
+}
+
+// ================================================================ C ABI ==
+extern "C" {
+
+void splat_default_options(splat_raster_options* o) {
+  if (!o) return;
+  o->background[0] = o->background[1] = o->background[2] = 0.f;
+  o->sh_degree = 0;
+  o->min_alpha = 1.f / 255.f;
+  o->transmittance_cutoff = 1e-4f;
+  o->screen_dilation = 0.3f;
+  o->deterministic = 0;
+}
+
+splat_status splat_availability(char* name, size_t name_len, char* reason, size_t reason_len) {
+  auto put = [](char* dst, size_t len, const std::string& s) {
+    if (dst && len) {
+      std::snprintf(dst, len, "%s", s.c_str());
+    }
+  };
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    put(reason, reason_len, e != hipSuccess ? std::string("no HIP device: ") + hipGetErrorString(e)
+                                             : std::string("no HIP device visible"));
+    return SPLAT_ERROR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, 0);
+  if (e != hipSuccess) {
+    put(reason, reason_len, std::string("device query failed: ") + hipGetErrorString(e));
+    return SPLAT_ERROR_NO_DEVICE;
+  }
+  put(name, name_len, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    put(reason, reason_len, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+    return SPLAT_ERROR_NO_DEVICE;
+  }
+  put(reason, reason_len, "");
+  return SPLAT_OK;
+}
+
+splat_status splat_create(int32_t device, splat_context** out) {
+  if (!out) return fail(nullptr, SPLAT_ERROR_INVALID_ARGUMENT, "splat_create: null out");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(nullptr, SPLAT_ERROR_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= count) return fail(nullptr, SPLAT_ERROR_INVALID_ARGUMENT, "device index out of range");
+  splat_context* ctx = new splat_context;
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    std::string msg = std::string("stream creation failed: ") + hipGetErrorString(e);
+    delete ctx;
+    return fail(nullptr, SPLAT_ERROR_DEVICE, msg);
+  }
+  // ssimKernel (SplatRaster.cpp:587-600), host libm as the CPU trainer
+  float total = 0.f;
+  for (int i = -5; i <= 5; ++i) {
+    const float v = std::exp(-0.5f * static_cast<float>(i * i) / (1.5f * 1.5f));
+    ctx->ssim_kernel[i + 5] = v;
+    total += v;
+  }
+  for (float& v : ctx->ssim_kernel) v /= total;
+  *out = ctx;
+  return SPLAT_OK;
+}
+
+void splat_destroy(splat_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+const char* splat_last_error(const splat_context* ctx) {
+  return ctx ? ctx->error.c_str() : g_no_context_error.c_str();
+}
+
+splat_status splat_set_capacity(splat_context* ctx, int32_t max_gaussians, int32_t sh_degree) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  if (max_gaussians <= 0 || sh_degree < 0 || sh_degree > 3)
+    return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "set_capacity: max_gaussians > 0 and sh_degree in [0, 3]");
+  CK(hipSetDevice(ctx->device), "set device");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  const int per = sh_degree == 0 ? 0 : (sh_degree == 1 ? 3 : (sh_degree == 2 ? 8 : 15));
+  ctx->rs = 3 * per;
+  ctx->sh_degree = sh_degree;
+  ctx->cap = max_gaussians;
+  ctx->n = 0;
+  const size_t floats = (size_t)max_gaussians * (14 + ctx->rs);
+  for (DeviceBuffer* b : {&ctx->params, &ctx->grads, &ctx->m, &ctx->v}) CK(b->reserve(sizeof(float) * floats), "alloc parameters");
+  CK(ctx->sg.reserve(sizeof(float) * max_gaussians), "alloc stats");
+  CK(ctx->vc.reserve(sizeof(int) * max_gaussians), "alloc stats");
+  CK(ctx->stat_sg.reserve(sizeof(float) * max_gaussians), "alloc stats");
+  CK(ctx->stat_vc.reserve(sizeof(int) * max_gaussians), "alloc stats");
+  CK(launch_fill_f32(ctx->params.f(), 0.f, (long long)floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->grads.f(), 0.f, (long long)floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->m.f(), 0.f, (long long)floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->v.f(), 0.f, (long long)floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->sg.f(), 0.f, max_gaussians, ctx->stream), "clear");
+  CK(launch_fill_u32(ctx->vc.u(), 0u, max_gaussians, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->stat_sg.f(), 0.f, max_gaussians, ctx->stream), "clear");
+  CK(launch_fill_u32(ctx->stat_vc.u(), 0u, max_gaussians, ctx->stream), "clear");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->forward_done = ctx->loss_done = false;
+  return SPLAT_OK;
+}
+
+int32_t splat_count(const splat_context* ctx) { return ctx ? ctx->n : 0; }
+
+splat_status splat_upload_cloud(splat_context* ctx, const splat_cloud* c) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!c || c->n < 0 || c->n > ctx->cap)
+    return fail(ctx, SPLAT_ERROR_CAPACITY, "upload_cloud: count exceeds the reserved capacity");
+  if (c->rest_stride != ctx->rs)
+    return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "upload_cloud: rest_stride does not match the capacity's SH degree");
+  if (c->n > 0 && (!c->pos || !c->log_scale || !c->rot || !c->opacity || !c->dc || (ctx->rs > 0 && !c->rest)))
+    return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "upload_cloud: null parameter array");
+  CK(hipSetDevice(ctx->device), "set device");
+  const long long cap = ctx->cap, n = c->n;
+  float* P = ctx->params.f();
+  auto up = [&](long long off, const float* src, long long count) -> hipError_t {
+    if (count <= 0) return hipSuccess;
+    return hipMemcpyAsync(P + off, src, sizeof(float) * count, hipMemcpyHostToDevice, ctx->stream);
+  };
+  CK(up(0, c->pos, 3 * n), "upload");
+  CK(up(3 * cap, c->log_scale, 3 * n), "upload");
+  CK(up(6 * cap, c->rot, 4 * n), "upload");
+  CK(up(10 * cap, c->opacity, n), "upload");
+  CK(up(11 * cap, c->dc, 3 * n), "upload");
+  CK(up(14 * cap, c->rest, (long long)ctx->rs * n), "upload");
+  const long long floats = cap * (14 + ctx->rs);
+  CK(launch_fill_f32(ctx->grads.f(), 0.f, floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->m.f(), 0.f, floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->v.f(), 0.f, floats, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->sg.f(), 0.f, cap, ctx->stream), "clear");
+  CK(launch_fill_u32(ctx->vc.u(), 0u, cap, ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->stat_sg.f(), 0.f, cap, ctx->stream), "clear");
+  CK(launch_fill_u32(ctx->stat_vc.u(), 0u, cap, ctx->stream), "clear");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->n = (int)n;
+  ctx->cloud_degree = c->sh_degree;
+  ctx->forward_done = ctx->loss_done = false;
+  return SPLAT_OK;
+}
+
+splat_status splat_download_cloud(splat_context* ctx, splat_cloud* out) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!out) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "download_cloud: null out");
+  CK(hipSetDevice(ctx->device), "set device");
+  const long long cap = ctx->cap, n = ctx->n;
+  const float* P = ctx->params.f();
+  auto down = [&](float* dst, long long off, long long count) -> hipError_t {
+    if (!dst || count <= 0) return hipSuccess;
+    return hipMemcpyAsync(dst, P + off, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream);
+  };
+  CK(down(out->pos, 0, 3 * n), "download");
+  CK(down(out->log_scale, 3 * cap, 3 * n), "download");
+  CK(down(out->rot, 6 * cap, 4 * n), "download");
+  CK(down(out->opacity, 10 * cap, n), "download");
+  CK(down(out->dc, 11 * cap, 3 * n), "download");
+  CK(down(out->rest, 14 * cap, (long long)ctx->rs * n), "download");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  out->n = (int32_t)n;
+  out->rest_stride = ctx->rs;
+  out->sh_degree = ctx->cloud_degree;
+  return SPLAT_OK;
+}
+
+splat_status splat_upload_views(splat_context* ctx, int32_t n, const splat_view* views, const float* const* images) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  if (n < 0 || (n > 0 && !views)) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "upload_views: bad arguments");
+  for (int k = 0; k < n; ++k) {
+    const splat_view& v = views[k];
+    if (v.width <= 0 || v.height <= 0 || !(v.fx > 0.0) || !(v.fy > 0.0))
+      return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "upload_views: a view has invalid intrinsics");
+  }
+  CK(hipSetDevice(ctx->device), "set device");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->views.assign(views, views + n);
+  ctx->target_offset.assign(n, 0);
+  ctx->has_target.assign(n, false);
+  long long total = 0;
+  for (int k = 0; k < n; ++k) {
+    ctx->target_offset[k] = total;
+    total += 3LL * views[k].width * views[k].height;
+  }
+  CK(ctx->targets.reserve(sizeof(float) * std::max(1LL, total)), "alloc targets");
+  if (images) {
+    for (int k = 0; k < n; ++k) {
+      if (!images[k]) continue;
+      CK(hipMemcpyAsync(ctx->targets.f() + ctx->target_offset[k], images[k],
+                        sizeof(float) * 3 * (size_t)views[k].width * views[k].height, hipMemcpyHostToDevice,
+                        ctx->stream),
+         "upload targets");
+      ctx->has_target[k] = true;
+    }
+  }
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->forward_done = ctx->loss_done = false;
+  return SPLAT_OK;
+}
+
+splat_status splat_render_forward(splat_context* ctx, int32_t n_views, const int32_t* view_ids,
+                                  const splat_raster_options* options) {
+  if (splat_status s = check_ready(ctx)) return s;
+  CK(hipSetDevice(ctx->device), "set device");
+  return forward_impl(ctx, n_views, view_ids, options);
+}
+
+splat_status splat_download_image(splat_context* ctx, int32_t slot, float* rgb, float* transmittance,
+                                  int32_t* last_contributor) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!ctx->forward_done || slot < 0 || slot >= (int)ctx->batch_desc.size())
+    return fail(ctx, SPLAT_ERROR_STATE, "download_image: no such batch slot");
+  const ViewDesc& d = ctx->batch_desc[slot];
+  const size_t px = (size_t)d.width * d.height;
+  if (rgb) CK(hipMemcpyAsync(rgb, ctx->rgb.f() + 3 * d.pixel_base, 12 * px, hipMemcpyDeviceToHost, ctx->stream), "download");
+  if (transmittance)
+    CK(hipMemcpyAsync(transmittance, ctx->T.f() + d.pixel_base, 4 * px, hipMemcpyDeviceToHost, ctx->stream), "download");
+  if (last_contributor)
+    CK(hipMemcpyAsync(last_contributor, ctx->last.i() + d.pixel_base, 4 * px, hipMemcpyDeviceToHost, ctx->stream),
+       "download");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+splat_status splat_download_projection(splat_context* ctx, int32_t slot, int32_t* count, int32_t* index, float* mean2,
+                                       float* conic3, float* depth, float* radius, float* color3, float* opacity,
+                                       int32_t* clamped3, int32_t* rect4) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!ctx->forward_done || slot < 0 || slot >= (int)ctx->batch_desc.size())
+    return fail(ctx, SPLAT_ERROR_STATE, "download_projection: no such batch slot");
+  const long long n = ctx->n, M = ctx->M;
+  std::vector<SplatRec> recs((size_t)n);
+  std::vector<uint32_t> order((size_t)M);
+  if (n > 0) {
+    CK(hipMemcpyAsync(recs.data(), ctx->recs() + (long long)slot * n, sizeof(SplatRec) * n, hipMemcpyDeviceToHost,
+                      ctx->stream),
+       "download");
+    CK(hipMemcpyAsync(order.data(), ctx->sorted_slots, 4 * M, hipMemcpyDeviceToHost, ctx->stream), "download");
+  }
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  int k = 0;
+  for (long long s = 0; s < M; ++s) {
+    const long long sl = order[(size_t)s];
+    if (sl / n != slot) continue;
+    const int g = (int)(sl % n);
+    const SplatRec& r = recs[(size_t)g];
+    if (!(r.flags & kProjected)) continue;
+    if (index) index[k] = g;
+    if (mean2) { mean2[2 * k] = r.mx; mean2[2 * k + 1] = r.my; }
+    if (conic3) { conic3[3 * k] = r.ca; conic3[3 * k + 1] = r.cb; conic3[3 * k + 2] = r.cc; }
+    if (depth) depth[k] = r.depth;
+    if (radius) radius[k] = r.radius;
+    if (color3) { color3[3 * k] = r.cr; color3[3 * k + 1] = r.cg; color3[3 * k + 2] = r.cbl; }
+    if (opacity) opacity[k] = r.opacity;
+    if (clamped3) for (int c = 0; c < 3; ++c) clamped3[3 * k + c] = (r.flags >> c) & 1u;
+    if (rect4) { rect4[4 * k] = r.x0; rect4[4 * k + 1] = r.x1; rect4[4 * k + 2] = r.y0; rect4[4 * k + 3] = r.y1; }
+    ++k;
+  }
+  if (count) *count = k;
+  return SPLAT_OK;
+}
+
+splat_status splat_download_tiles(splat_context* ctx, int32_t slot, int32_t* offsets, int32_t* entries,
+                                  int32_t* entry_count) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!ctx->forward_done || slot < 0 || slot >= (int)ctx->batch_desc.size())
+    return fail(ctx, SPLAT_ERROR_STATE, "download_tiles: no such batch slot");
+  const ViewDesc& d = ctx->batch_desc[slot];
+  const int tiles = d.cols * d.rows;
+  std::vector<uint32_t> ranges(2 * (size_t)tiles);
+  CK(hipMemcpyAsync(ranges.data(), ctx->ranges.u() + 2 * (long long)d.tile_base, 8 * (size_t)tiles,
+                    hipMemcpyDeviceToHost, ctx->stream),
+     "download");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  long long total = 0;
+  for (int t = 0; t < tiles; ++t) total += ranges[2 * t + 1] - ranges[2 * t];
+  if (entry_count) *entry_count = (int32_t)total;
+  if (offsets) {
+    long long run = 0;
+    for (int t = 0; t < tiles; ++t) {
+      offsets[t] = (int32_t)run;
+      run += ranges[2 * t + 1] - ranges[2 * t];
+    }
+    offsets[tiles] = (int32_t)run;
+  }
+  if (entries && total > 0) {
+    // the view's pairs are contiguous in the sorted order
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    for (int t = 0; t < tiles; ++t)
+      if (ranges[2 * t + 1] > ranges[2 * t]) {
+        lo = std::min(lo, ranges[2 * t]);
+        hi = std::max(hi, ranges[2 * t + 1]);
+      }
+    std::vector<uint32_t> vals(hi - lo);
+    CK(hipMemcpyAsync(vals.data(), ctx->sorted_pair_vals + lo, 4 * (size_t)(hi - lo), hipMemcpyDeviceToHost,
+                      ctx->stream),
+       "download");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+    long long k = 0;
+    for (int t = 0; t < tiles; ++t)
+      for (uint32_t e = ranges[2 * t]; e < ranges[2 * t + 1]; ++e) entries[k++] = (int32_t)(vals[e - lo] % ctx->n);
+  }
+  return SPLAT_OK;
+}
+
+splat_status splat_loss(splat_context* ctx, float ssim_weight, float* losses) {
+  if (splat_status s = check_ready(ctx)) return s;
+  CK(hipSetDevice(ctx->device), "set device");
+  return loss_impl(ctx, ssim_weight, losses);
+}
+
+splat_status splat_set_image_gradient(splat_context* ctx, int32_t slot, const float* dL_drgb) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!ctx->forward_done || slot < 0 || slot >= (int)ctx->batch_desc.size() || !dL_drgb)
+    return fail(ctx, SPLAT_ERROR_STATE, "set_image_gradient: no such batch slot");
+  const ViewDesc& d = ctx->batch_desc[slot];
+  CK(ctx->dL.reserve(sizeof(float) * 3 * std::max(1LL, ctx->total_px)), "alloc dL");
+  CK(hipMemcpyAsync(ctx->dL.f() + 3 * d.pixel_base, dL_drgb, 12 * (size_t)d.width * d.height, hipMemcpyHostToDevice,
+                    ctx->stream),
+     "upload dL");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->loss_done = true;
+  return SPLAT_OK;
+}
+
+splat_status splat_download_image_gradient(splat_context* ctx, int32_t slot, float* dL_drgb) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!ctx->loss_done || slot < 0 || slot >= (int)ctx->batch_desc.size() || !dL_drgb)
+    return fail(ctx, SPLAT_ERROR_STATE, "download_image_gradient: no image gradient for that slot");
+  const ViewDesc& d = ctx->batch_desc[slot];
+  CK(hipMemcpyAsync(dL_drgb, ctx->dL.f() + 3 * d.pixel_base, 12 * (size_t)d.width * d.height, hipMemcpyDeviceToHost,
+                    ctx->stream),
+     "download dL");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+splat_status splat_render_backward(splat_context* ctx) {
+  if (splat_status s = check_ready(ctx)) return s;
+  CK(hipSetDevice(ctx->device), "set device");
+  return backward_impl(ctx);
+}
+
+splat_status splat_download_gradients(splat_context* ctx, splat_gradients* out) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!out) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "download_gradients: null out");
+  const long long cap = ctx->cap, n = ctx->n;
+  const float* G = ctx->grads.f();
+  auto down = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    if (!dst || bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  };
+  CK(down(out->pos, G, 12 * n), "download");
+  CK(down(out->log_scale, G + 3 * cap, 12 * n), "download");
+  CK(down(out->rot, G + 6 * cap, 16 * n), "download");
+  CK(down(out->opacity, G + 10 * cap, 4 * n), "download");
+  CK(down(out->dc, G + 11 * cap, 12 * n), "download");
+  CK(down(out->rest, G + 14 * cap, 4 * (size_t)ctx->rs * n), "download");
+  CK(down(out->screen_gradient, ctx->sg.f(), 4 * n), "download");
+  CK(down(out->visible_count, ctx->vc.i(), 4 * n), "download");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+splat_status splat_zero_gradients(splat_context* ctx) {
+  if (splat_status s = check_ready(ctx)) return s;
+  CK(launch_fill_f32(ctx->grads.f(), 0.f, ctx->cap * (14 + ctx->rs), ctx->stream), "clear");
+  CK(launch_fill_f32(ctx->sg.f(), 0.f, ctx->cap, ctx->stream), "clear");
+  CK(launch_fill_u32(ctx->vc.u(), 0u, ctx->cap, ctx->stream), "clear");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+splat_status splat_adam_step(splat_context* ctx, const splat_adam_params* params) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!params) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "adam_step: null params");
+  CK(hipSetDevice(ctx->device), "set device");
+  if (splat_status s = adam_impl(ctx, params)) return s;
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+splat_status splat_train_step(splat_context* ctx, int32_t n_views, const int32_t* view_ids,
+                              const splat_raster_options* options, float ssim_weight, const splat_adam_params* adam,
+                              float* losses) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (!adam) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "train_step: null adam params");
+  CK(hipSetDevice(ctx->device), "set device");
+  if (splat_status s = forward_impl(ctx, n_views, view_ids, options)) return s;
+  if (splat_status s = loss_impl(ctx, ssim_weight, nullptr)) return s;
+  if (splat_status s = backward_impl(ctx)) return s;
+  if (splat_status s = adam_impl(ctx, adam)) return s;
+  if (losses) {
+    std::vector<float> sums(2 * n_views);
+    CK(hipMemcpyAsync(sums.data(), ctx->loss_sums.f(), sizeof(float) * 2 * n_views, hipMemcpyDeviceToHost, ctx->stream),
+       "read loss");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+    for (int k = 0; k < n_views; ++k) {
+      const ViewDesc& d = ctx->batch_desc[k];
+      const float inverse = 1.f / static_cast<float>((long long)d.width * d.height * 3);
+      const float l1 = sums[2 * k] * inverse;
+      losses[k] = ssim_weight <= 0.f ? l1 : (1.f - ssim_weight) * l1 + ssim_weight * (1.f - sums[2 * k + 1] * inverse);
+    }
+  }
+  return SPLAT_OK;
+}
+
+splat_status splat_download_stats(splat_context* ctx, float* screen_gradient, int32_t* visible_count) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (screen_gradient)
+    CK(hipMemcpyAsync(screen_gradient, ctx->stat_sg.f(), 4 * (size_t)ctx->n, hipMemcpyDeviceToHost, ctx->stream),
+       "download");
+  if (visible_count)
+    CK(hipMemcpyAsync(visible_count, ctx->stat_vc.i(), 4 * (size_t)ctx->n, hipMemcpyDeviceToHost, ctx->stream),
+       "download");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+splat_status splat_set_timing(splat_context* ctx, int32_t enabled) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  ctx->timing = enabled != 0;
+  return SPLAT_OK;
+}
+
+int32_t splat_timing_count(const splat_context* ctx) { return ctx ? (int32_t)ctx->timers.size() : 0; }
+
+splat_status splat_timing_entry(splat_context* ctx, int32_t i, const char** name, double* total_ms, int64_t* launches) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  CK(ctx->collect_timing(), "timing");
+  if (i < 0 || i >= (int)ctx->timers.size()) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "timing index");
+  auto it = ctx->timers.begin();
+  std::advance(it, i);
+  if (name) *name = it->first.c_str();
+  if (total_ms) *total_ms = it->second.total_ms;
+  if (launches) *launches = it->second.launches;
+  return SPLAT_OK;
+}
+
+splat_status splat_reset_timing(splat_context* ctx) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  CK(ctx->collect_timing(), "timing");
+  ctx->timers.clear();
+  return SPLAT_OK;
+}
+
+int64_t splat_last_pair_count(const splat_context* ctx) { return ctx ? (int64_t)ctx->P : 0; }
+
+splat_status splat_synchronize(splat_context* ctx) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// The context behind the opaque splat_context handle (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/splat_hip.h"
+#include "splat_common.h"
+#include "splat_kernels.h"
+
+// A device allocation that only grows.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t want);
+  void release();
+  float* f() const { return static_cast<float*>(ptr); }
+  uint32_t* u() const { return static_cast<uint32_t*>(ptr); }
+  int* i() const { return static_cast<int*>(ptr); }
+};
+
+struct KernelTimer {
+  double total_ms = 0.0;
+  long long launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct splat_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+
+  // scene: parameters, gradients and moments laid out by capacity
+  long long cap = 0;
+  int sh_degree = 0;
+  int rs = 0;
+  int n = 0;
+  int cloud_degree = 0;
+  DeviceBuffer params, grads, m, v;
+  DeviceBuffer sg, vc;            // this step's screen gradient / visible count
+  DeviceBuffer stat_sg, stat_vc;  // accumulated since the last densify
+
+  // cameras and targets
+  std::vector<splat_view> views;
+  std::vector<long long> target_offset;
+  std::vector<bool> has_target;
+  DeviceBuffer targets;
+  DeviceBuffer batch_targets;
+
+  // the current batch
+  splat_raster_options opts{};
+  std::vector<int> batch_ids;
+  std::vector<splat::ViewDesc> batch_desc;
+  DeviceBuffer d_views;
+  int total_tiles = 0;
+  long long total_px = 0;
+  int sh_use = 0;
+  long long M = 0;
+  unsigned long long P = 0;
+  splat::RasterParams raster{};
+  bool forward_done = false;
+  bool loss_done = false;
+  float last_ssim_weight = 0.f;
+  float ssim_kernel[11] = {};
+
+  DeviceBuffer rec, keys, vals, keys_alt, vals_alt, counts, offsets, pair_base;
+  DeviceBuffer pair_keys, pair_vals, pair_keys_alt, pair_vals_alt, ranges;
+  uint32_t* sorted_slots = nullptr;
+  uint32_t* sorted_pair_keys = nullptr;
+  uint32_t* sorted_pair_vals = nullptr;
+  DeviceBuffer rgb, T, last, dL, terms, block_sums, loss_sums;
+  DeviceBuffer screen, partial;
+  DeviceBuffer scan_ws, hist_ws, scalars;
+  splat::SortWorkspace sort_ws;
+
+  // densify scratch
+  DeviceBuffer scratch_params, scratch_m, scratch_v;
+  DeviceBuffer dens_flags, dens_a, dens_b, dens_c, dens_src;
+
+  // timing
+  bool timing = false;
+  std::map<std::string, KernelTimer> timers;
+
+  ~splat_context();
+  std::vector<DeviceBuffer*> all_buffers();
+  hipError_t ensure_sort_workspace(long long max_n);
+  hipError_t timed_begin(const char* name, hipEvent_t* start);
+  hipError_t timed_end(const char* name, hipEvent_t start);
+  hipError_t collect_timing();
+  splat::SplatRec* recs() const { return static_cast<splat::SplatRec*>(rec.ptr); }
+  const splat::ViewDesc* views_dev() const { return static_cast<const splat::ViewDesc*>(d_views.ptr); }
+};

@@ -23,11 +23,12 @@
 
 #include <stdint.h>
 
+#include <cmath>
+#include <cstring>
+
 #if defined(__HIPCC__)
 #define SPLAT_EXP_QUAL __host__ __device__ __forceinline__
 #else
-#include <cmath>
-#include <cstring>
 #define SPLAT_EXP_QUAL inline
 #endif
 

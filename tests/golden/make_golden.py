@@ -1,0 +1,170 @@
+"""Writes the golden vectors under tests/golden/ from the reference engine's
+own splat sources, compiled by `make -C oracle ref` into
+oracle/_ref/libparallax_ref.so (its Release flags, -ffast-math included).
+
+Each fixture holds the inputs and the reference's outputs:
+  raster_*.npz   projectSplats, renderSplats, photometricLoss (+ dL/drgb) and
+                 rasterizeSplatsBackward for one seeded scene and view;
+  densify_*.npz  one densifyAndPrune call each (cloud, source_index, counts,
+                 RNG): one dominated by pruning, one that reaches the cap;
+  trainer.npz    SplatTrainer runs: per-step losses and the final cloud,
+                 including one with densification on.
+Run from the repository root: python tests/golden/make_golden.py
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from parallax_amd import synthetic as syn  # noqa: E402
+
+
+def cloud_arrays(prefix, c):
+    return {f"{prefix}pos": c.pos, f"{prefix}log_scale": c.log_scale, f"{prefix}rot": c.rot,
+            f"{prefix}opacity": c.opacity, f"{prefix}dc": c.dc, f"{prefix}rest": c.rest,
+            f"{prefix}sh_degree": np.int32(c.sh_degree)}
+
+
+def view_arrays(prefix, v):
+    return {f"{prefix}R": v.R, f"{prefix}t": v.t,
+            f"{prefix}K": np.array([v.fx, v.fy, v.cx, v.cy]), f"{prefix}size": np.array([v.width, v.height], np.int32)}
+
+
+def ref_raster(name, cloud, view, opt_kw, target, ssim_weight=0.2):
+    R = O.ref()
+    opt = O.default_options(**opt_kw)
+    k = O.ccloud(cloud)
+    v = O.cview(view)
+    n = max(1, cloud.n)
+    out = dict(index=np.zeros(n, np.int32), mean=np.zeros((n, 2), np.float32), cov2d=np.zeros((n, 3), np.float32),
+               conic=np.zeros((n, 3), np.float32), depth=np.zeros(n, np.float32), radius=np.zeros(n, np.float32),
+               color=np.zeros((n, 3), np.float32), opacity=np.zeros(n, np.float32), clamped=np.zeros((n, 3), np.int32))
+    m = R.ref_project(C.byref(k.struct), C.byref(v), C.byref(opt), O.ptr(out["index"], O.i32p), O.ptr(out["mean"]),
+                      O.ptr(out["cov2d"]), O.ptr(out["conic"]), O.ptr(out["depth"]), O.ptr(out["radius"]),
+                      O.ptr(out["color"]), O.ptr(out["opacity"]), O.ptr(out["clamped"], O.i32p))
+    proj = {f"proj_{kk}": vv[:m] for kk, vv in out.items()}
+    rgb = np.zeros(view.pixels * 3, np.float32)
+    T = np.zeros(view.pixels, np.float32)
+    R.ref_render(C.byref(k.struct), C.byref(v), C.byref(opt), O.ptr(rgb), O.ptr(T))
+    dl = np.zeros_like(rgb)
+    loss = R.ref_photometric_loss(O.ptr(rgb), O.ptr(target), view.width, view.height, float(ssim_weight), O.ptr(dl))
+    g = O.Grads(cloud.n, cloud.rest_stride)
+    R.ref_backward(C.byref(k.struct), C.byref(v), C.byref(opt), O.ptr(dl), O.ptr(g.pos), O.ptr(g.log_scale),
+                   O.ptr(g.rot), O.ptr(g.opacity), O.ptr(g.dc), O.ptr(g.rest), O.ptr(g.screen_gradient),
+                   O.ptr(g.visible_count, O.i32p))
+    data = {}
+    data.update(cloud_arrays("in_", cloud))
+    data.update(view_arrays("view_", view))
+    data.update({"opt_background": np.array(opt_kw.get("background", (0, 0, 0)), np.float32),
+                 "opt_sh_degree": np.int32(opt_kw.get("sh_degree", 0)),
+                 "opt_min_alpha": np.float32(opt_kw.get("min_alpha", 1.0 / 255.0)),
+                 "opt_cutoff": np.float32(opt_kw.get("transmittance_cutoff", 1e-4)),
+                 "target": target, "ssim_weight": np.float32(ssim_weight)})
+    data.update(proj)
+    data.update({"rgb": rgb, "T": T, "loss": np.float32(loss), "dL": dl})
+    data.update({f"grad_{kk}": getattr(g, kk) for kk in ("pos", "log_scale", "rot", "opacity", "dc", "rest",
+                                                          "screen_gradient", "visible_count")})
+    np.savez_compressed(os.path.join(HERE, f"raster_{name}.npz"), **data)
+    print(f"raster_{name}: {m} projected, loss {loss:.6f}")
+
+
+def ref_densify(name, seed, opacity_mu, max_gaussians, iteration):
+    R = O.ref()
+    rng = np.random.default_rng(seed)
+    cloud = syn.random_cloud(400, seed, 1, (-1, -1, -1), (1, 1, 1), log_scale_mu=-3.0, log_scale_sigma=1.0)
+    cloud.opacity[:] = rng.normal(opacity_mu, 2.0, size=cloud.n).astype(np.float32)
+    sg = rng.exponential(3e-4, size=cloud.n).astype(np.float32)
+    vc = rng.integers(0, 4, size=cloud.n).astype(np.int32)
+    cfg = O.default_config(max_gaussians=max_gaussians, opacity_reset_interval=3000)
+    extent, state0 = 2.0, 7
+    cap = 3 * cloud.n
+    out = syn.Cloud(np.zeros((cap, 3), np.float32), np.zeros((cap, 3), np.float32), np.zeros((cap, 4), np.float32),
+                    np.zeros(cap, np.float32), np.zeros((cap, 3), np.float32),
+                    np.zeros((cap, cloud.rest_stride), np.float32), 1)
+    ko = O.ccloud(out)
+    ki = O.ccloud(cloud)
+    src = np.zeros(cap, np.int32)
+    counts = np.zeros(3, np.int32)
+    st = C.c_uint32(state0)
+    n1 = R.ref_densify(C.byref(ki.struct), O.ptr(sg), O.ptr(vc, O.i32p), C.byref(cfg), extent, iteration, C.byref(st),
+                       C.byref(ko.struct), O.ptr(src, O.i32p), O.ptr(counts, O.i32p))
+    res = {k: a.reshape(b.shape)[:n1] for k, a, b in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"),
+                                                       ko.arrays, (out.pos, out.log_scale, out.rot, out.opacity,
+                                                                   out.dc, out.rest))}
+    data = cloud_arrays("in_", cloud)
+    data.update({"screen_gradient": sg, "visible_count": vc, "extent": np.float32(extent),
+                 "iteration": np.int32(iteration), "rng_in": np.uint32(state0),
+                 "max_gaussians": np.int32(max_gaussians),
+                 "rng_out": np.uint32(st.value), "source_index": src[:n1], "counts": counts})
+    data.update({f"out_{k}": v for k, v in res.items()})
+    np.savez_compressed(os.path.join(HERE, f"densify_{name}.npz"), **data)
+    print(f"densify_{name}: {cloud.n} -> {n1}, counts {counts.tolist()}")
+
+
+def ref_trainer():
+    R = O.ref()
+    data = {}
+    for tag, views, start, cfg_kw, steps in (
+        ("plain", syn.ring_of_views(6, 1.6, 32, 32), syn.rough_start(), dict(), 40),
+        ("densify", syn.ring_of_views(4, 1.6, 16, 16), syn.rough_start(8),
+         dict(densify_from_iteration=20, densify_until_iteration=60, densify_interval=20,
+              densify_grad_threshold=0.0), 60),
+    ):
+        truth = syn.truth_cloud()
+        images = [O.render(truth, v)[0] for v in views]
+        cfg = O.default_config(iterations=steps, sh_degree=0, sh_increase_interval=0,
+                               densify_from_iteration=steps + 1, densify_until_iteration=steps + 1,
+                               opacity_reset_interval=0, ssim_weight=0.2)
+        for kk, vv in cfg_kw.items():
+            setattr(cfg, kk, vv)
+        cap = 4096
+        out = syn.Cloud(np.zeros((cap, 3), np.float32), np.zeros((cap, 3), np.float32),
+                        np.zeros((cap, 4), np.float32), np.zeros(cap, np.float32), np.zeros((cap, 3), np.float32),
+                        np.zeros((cap, 0), np.float32), 0)
+        ko = O.ccloud(out)
+        ki = O.ccloud(start)
+        varr = (O.CView * len(views))(*[O.cview(v) for v in views])
+        imgs = [np.ascontiguousarray(im, np.float32) for im in images]
+        ip = (O.f32p * len(imgs))(*[O.ptr(im) for im in imgs])
+        losses = np.zeros(steps, np.float32)
+        n1 = R.ref_train(C.byref(ki.struct), len(views), varr, ip, C.byref(cfg), steps, O.ptr(losses),
+                         C.byref(ko.struct), 0)
+        data[f"{tag}_losses"] = losses
+        data[f"{tag}_images"] = np.stack(imgs)
+        data[f"{tag}_count"] = np.int32(n1)
+        for k, a, b in zip(("pos", "log_scale", "rot", "opacity", "dc"), ko.arrays,
+                           (out.pos, out.log_scale, out.rot, out.opacity, out.dc)):
+            data[f"{tag}_out_{k}"] = a.reshape(b.shape)[:n1]
+        data[f"{tag}_steps"] = np.int32(steps)
+        print(f"trainer {tag}: {steps} steps, {n1} gaussians, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    np.savez_compressed(os.path.join(HERE, "trainer.npz"), **data)
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("build the reference first: make -C oracle ref")
+    fd_target = np.random.default_rng(41).uniform(size=16 * 16 * 3).astype(np.float32)
+    ref_raster("fd16_sh2", syn.overlapping_cloud(2), syn.test_view(16, 16),
+               dict(sh_degree=2, background=(0.2, 0.3, 0.4)), fd_target)
+    c1 = syn.random_cloud(2000, 0, 0, (-1, -1, 2), (1, 1, 6))
+    v1 = syn.pinhole(128, 128, 150.0)
+    t1 = O.render(syn.random_cloud(2000, 1, 0, (-1, -1, 2), (1, 1, 6)), v1)[0]
+    ref_raster("c1_small_sh0", c1, v1, dict(), t1)
+    c3 = syn.random_cloud(1500, 5, 3, (-1, -1, 2), (1, 1, 6))
+    v3 = syn.pinhole(128, 80, 160.0)
+    t3 = O.render(syn.random_cloud(1500, 6, 3, (-1, -1, 2), (1, 1, 6)), v3, O.default_options(sh_degree=3))[0]
+    ref_raster("sh3_bg", c3, v3, dict(sh_degree=3, background=(0.1, 0.2, 0.3)), t3)
+    ref_densify("prune", 17, -2.0, 560, 3500)
+    ref_densify("cap", 23, 1.0, 470, 200)
+    ref_trainer()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,236 @@
+"""Parity of the HIP path with the CPU oracle on the same seeded inputs.
+
+Bars (from the task's contract): binning and per-tile lists bit-exact;
+images within 1e-4 max abs (the kernels reproduce the checker's float
+operations, so they are in fact compared for equality); gradients within
+1e-3 relative L2 of the FP64 oracle fed the same dL/drgb; Adam bit-exact
+given the same gradients.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from parallax_amd import splat as S
+from parallax_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-4
+GRAD_REL_L2 = 1e-3
+
+
+def scenes():
+    yield "c1_shape_10k_sh0", syn.config1()[0], syn.pinhole(256, 256, 300.0), 0, (0.0, 0.0, 0.0)
+    yield ("sh3_20k_320x180", syn.random_cloud(20_000, 3, 3, (-1, -1, 2), (1, 1, 6)),
+           syn.pinhole(320, 180, 250.0), 3, (0.1, 0.2, 0.3))
+    yield "fd_scene_16x16_sh2", syn.overlapping_cloud(2), syn.test_view(16, 16), 2, (0.2, 0.3, 0.4)
+    yield ("ragged_tiles_sh1", syn.random_cloud(3000, 9, 1, (-1, -1, 2), (1, 1, 6)),
+           syn.pinhole(97, 61, 80.0), 1, (0.0, 0.0, 0.0))
+
+
+def flat_grads(g):
+    return np.concatenate([np.asarray(g[k]).ravel() for k in ("pos", "log_scale", "rot", "opacity", "dc", "rest")])
+
+
+def oracle_flat(g):
+    return np.concatenate([g.pos.ravel(), g.log_scale.ravel(), g.rot.ravel(), g.opacity.ravel(), g.dc.ravel(),
+                           g.rest.ravel()]).astype(np.float64)
+
+
+def rel_l2(a, b):
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(np.asarray(a, dtype=np.float64) - b) / max(1e-30, np.linalg.norm(b)))
+
+
+def load(ctx, cloud, views, targets=None):
+    ctx.set_capacity(max(1, cloud.n), cloud.sh_degree)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views, targets)
+
+
+@pytest.mark.parametrize("case", list(scenes()), ids=lambda c: c[0])
+def test_forward_bit_exact(ctx, case):
+    name, cloud, view, deg, bg = case
+    print(f"[parity] forward {name}", flush=True)
+    load(ctx, cloud, [view])
+    ctx.render_forward([0], S.options(sh_degree=deg, background=bg))
+    f = O.Frame(cloud, view, O.default_options(sh_degree=deg, background=bg))
+    # projection: same splats, same depth order, same values bit for bit
+    gp, op = ctx.projection(0), f.projected()
+    assert np.array_equal(gp["index"], op["index"])
+    for key in ("mean", "conic", "depth", "radius", "color", "opacity", "clamped", "rect"):
+        assert np.array_equal(gp[key], op[key]), key
+    # binning: per-tile lists bit-exact
+    cols, rows, offs, ents = ctx.tiles(0)
+    ocols, orows, ooffs, oents = f.tiles()
+    assert (cols, rows) == (ocols, orows)
+    assert np.array_equal(offs, ooffs)
+    assert np.array_equal(ents, oents)
+    # image, transmittance, last contributor
+    rgb, T, last = ctx.image(0)
+    orgb, oT, olast = f.image()
+    assert np.abs(rgb - orgb).max() <= IMG_TOL
+    assert np.abs(T - oT).max() <= IMG_TOL
+    assert np.array_equal(rgb, orgb), "images differ below tolerance; expected identical float ops"
+    assert np.array_equal(last, olast)
+
+
+@pytest.mark.parametrize("case", list(scenes()), ids=lambda c: c[0])
+def test_loss_and_adjoint(ctx, case):
+    name, cloud, view, deg, bg = case
+    print(f"[parity] loss+adjoint {name}", flush=True)
+    target = syn.random_cloud(cloud.n, 101, cloud.sh_degree, (-1, -1, 2), (1, 1, 6)) if cloud.n > 100 else None
+    opts = O.default_options(sh_degree=deg, background=bg)
+    if target is not None:
+        tgt, _ = O.render(target, view, opts)
+    else:
+        tgt = np.random.default_rng(5).uniform(size=view.pixels * 3).astype(np.float32)
+    load(ctx, cloud, [view], [tgt])
+    ctx.zero_gradients()
+    ctx.render_forward([0], S.options(sh_degree=deg, background=bg))
+    loss = ctx.loss(0.2)[0]
+    ctx.render_backward()
+    g = ctx.gradients()
+
+    f = O.Frame(cloud, view, opts)
+    orgb, _, _ = f.image()
+    oloss, odl = O.photometric_loss(orgb, tgt, view.width, view.height, 0.2)
+    assert abs(loss - oloss) <= 1e-5 * max(1.0, abs(oloss))
+    gdl = ctx.image_gradient(0)
+    assert np.allclose(gdl, odl, rtol=1e-5, atol=1e-6 * np.abs(odl).max())
+    # the device's own dL/drgb goes into both checkers so the adjoint is compared alone
+    og32 = f.backward(gdl)
+    og64 = f.backward_f64(gdl)
+    ga = flat_grads(g)
+    r64 = rel_l2(ga, oracle_flat(og64))
+    r32 = rel_l2(ga, oracle_flat(og32))
+    print(f"  loss {loss:.7f} vs {oloss:.7f}; grad rel-L2 vs f64 {r64:.3g}, vs f32 {r32:.3g}", flush=True)
+    assert r64 <= GRAD_REL_L2
+    assert np.array_equal(g["visible_count"], og32.visible_count)
+    assert rel_l2(g["screen_gradient"], og64.screen_gradient) <= GRAD_REL_L2
+
+
+def test_image_gradient_linear_functional_matches_fp64(ctx):
+    """The engine's gradient-check objective: a fixed linear functional of the
+    pixels (SplatRasterTest.cpp:88-112, 341-415), thresholds off."""
+    cloud, view = syn.overlapping_cloud(2), syn.test_view(16, 16)
+    kw = dict(sh_degree=2, background=(0.2, 0.3, 0.4), min_alpha=0.0, transmittance_cutoff=0.0)
+    load(ctx, cloud, [view])
+    ctx.zero_gradients()
+    ctx.render_forward([0], S.options(**kw))
+    w = (np.random.default_rng(41).uniform(size=view.pixels * 3) - 0.5).astype(np.float32)
+    ctx.set_image_gradient(0, w)
+    ctx.render_backward()
+    g = ctx.gradients()
+    f = O.Frame(cloud, view, O.default_options(**kw))
+    og64 = f.backward_f64(w)
+    assert rel_l2(flat_grads(g), oracle_flat(og64)) <= GRAD_REL_L2
+
+
+def test_deterministic_mode_is_bitwise_repeatable(ctx):
+    cloud = syn.random_cloud(20_000, 4, 3, (-1, -1, 2), (1, 1, 6))
+    view = syn.pinhole(320, 240, 260.0)
+    tgt, _ = O.render(syn.random_cloud(20_000, 5, 3, (-1, -1, 2), (1, 1, 6)), view, O.default_options(sh_degree=3))
+    load(ctx, cloud, [view], [tgt])
+    runs = []
+    for det in (True, True, False):
+        ctx.zero_gradients()
+        ctx.render_forward([0], S.options(sh_degree=3, deterministic=det))
+        ctx.loss(0.2)
+        ctx.render_backward()
+        runs.append(ctx.gradients())
+    a, b, c = (flat_grads(r) for r in runs)
+    assert np.array_equal(a, b), "deterministic mode must reproduce gradients bit for bit"
+    assert rel_l2(c, a) <= 1e-4
+
+
+def test_backward_accumulates(ctx):
+    """Calling the adjoint twice adds twice (SplatRasterTest.cpp:417-442)."""
+    cloud, view = syn.overlapping_cloud(0, 3), syn.test_view(24, 20)
+    tgt = np.random.default_rng(51).uniform(size=view.pixels * 3).astype(np.float32)
+    load(ctx, cloud, [view], [tgt])
+    ctx.zero_gradients()
+    ctx.render_forward([0], S.options(deterministic=True))
+    ctx.loss(0.0)
+    ctx.render_backward()
+    once = ctx.gradients()
+    ctx.render_backward()
+    twice = ctx.gradients()
+    assert np.allclose(2 * once["opacity"], twice["opacity"], atol=1e-5)
+    assert np.array_equal(2 * once["visible_count"], twice["visible_count"])
+    assert np.allclose(2 * once["pos"], twice["pos"], atol=1e-5)
+
+
+def test_adam_bit_exact_given_the_same_gradients(ctx):
+    cloud = syn.random_cloud(5000, 7, 2, (-1, -1, 2), (1, 1, 6))
+    view = syn.pinhole(160, 120, 150.0)
+    tgt, _ = O.render(syn.random_cloud(5000, 8, 2, (-1, -1, 2), (1, 1, 6)), view, O.default_options(sh_degree=2))
+    load(ctx, cloud, [view], [tgt])
+    ctx.zero_gradients()
+    ctx.render_forward([0], S.options(sh_degree=2))
+    ctx.loss(0.2)
+    ctx.render_backward()
+    g = ctx.gradients()
+    cfg = O.default_config(iterations=100)
+    extent = 1e-3
+    it = 0
+    # the oracle's applyGradients with the GPU's own gradients
+    og = O.Grads(cloud.n, cloud.rest_stride)
+    for k in ("pos", "log_scale", "rot", "opacity", "dc", "rest"):
+        getattr(og, k)[...] = g[k]
+    m = np.zeros(cloud.n * (14 + cloud.rest_stride), np.float32)
+    v = np.zeros_like(m)
+    expect = O.adam_apply(cloud.copy(), og, m, v, it, extent, cfg)
+    ratio = np.float32(cfg.position_lr_final) / np.float32(cfg.position_lr_initial)
+    t = np.float32(it) / np.float32(cfg.iterations - 1)
+    lr_pos = np.float32(cfg.position_lr_initial) * np.power(ratio, t, dtype=np.float32) * np.float32(extent)
+    ctx.adam_step(S.adam_params(it, float(lr_pos)))
+    got = ctx.download_cloud()
+    for k in ("pos", "log_scale", "rot", "opacity", "dc", "rest"):
+        assert np.array_equal(getattr(got, k), getattr(expect, k)), k
+
+
+def test_batched_uneven_views_match_single_view_runs(ctx):
+    cloud = syn.random_cloud(30_000, 11, 3, (-2, -2, -2), (2, 2, 2))
+    views = syn.orbit_views(3, 6.0, [(160, 120), (200, 100), (96, 96)], seed=3)
+    tgt = [O.render(syn.random_cloud(30_000, 12, 3, (-2, -2, -2), (2, 2, 2)), v, O.default_options(sh_degree=3))[0]
+           for v in views]
+    load(ctx, cloud, views, tgt)
+    opts = S.options(sh_degree=3, deterministic=True)
+    # batched
+    ctx.zero_gradients()
+    ctx.render_forward([0, 1, 2], opts)
+    losses = ctx.loss(0.2)
+    images = [ctx.image(k)[0] for k in range(3)]
+    ctx.render_backward()
+    gb = ctx.gradients()
+    # one at a time
+    ctx.zero_gradients()
+    single = []
+    for k in range(3):
+        ctx.render_forward([k], opts)
+        single.append(ctx.loss(0.2)[0])
+        assert np.array_equal(ctx.image(0)[0], images[k])
+        ctx.render_backward()
+    gs = ctx.gradients()
+    assert np.allclose(losses, single, rtol=1e-6)
+    assert np.array_equal(gb["visible_count"], gs["visible_count"])
+    assert rel_l2(flat_grads(gb), flat_grads(gs)) <= 1e-5
+    # and the oracle on each view
+    for k, v in enumerate(views):
+        orgb, _ = O.render(cloud, v, O.default_options(sh_degree=3))
+        assert np.array_equal(images[k], orgb)
+
+
+def test_empty_cloud_renders_background(ctx):
+    """SplatRasterTest.cpp:220-232: T = 1 and rgb = background exactly."""
+    cloud = syn.random_cloud(0, 0, 0, (0, 0, 0), (1, 1, 1))
+    view = syn.test_view()
+    ctx.set_capacity(1, 0)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views([view])
+    ctx.render_forward([0], S.options(background=(0.25, 0.5, 0.75)))
+    rgb, T, last = ctx.image(0)
+    assert np.all(T == 1.0)
+    assert np.all(rgb.reshape(-1, 3) == np.array([0.25, 0.5, 0.75], np.float32))
+    assert np.all(last == 0)

@@ -1,0 +1,521 @@
+"""The engine's own rasteriser and trainer tests, re-expressed against the
+oracle (engine/native/cpp/tests/SplatRasterTest.cpp, SplatTrainerTest.cpp),
+plus the checks that pin what those leave open: finite differences of the
+FP64 adjoint, brute-force tile enumeration, closed-form Adam, and the
+deterministic exponential against libm."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parallax_amd import synthetic as syn
+
+C0 = syn.SH_C0
+
+
+def gaussian(pos=(0, 0, 0), log_scale=(-2.0, -2.0, -2.0), logit=0.0, dc=(0, 0, 0), rot=(1, 0, 0, 0), rest=()):
+    f = lambda a, s: np.asarray(a, np.float32).reshape(s)
+    rs = len(rest)
+    return syn.Cloud(f(pos, (1, 3)), f(log_scale, (1, 3)), f(rot, (1, 4)), f([logit], (1,)), f(dc, (1, 3)),
+                     f(rest, (1, rs)), {0: 0, 9: 1, 24: 2, 45: 3}[rs])
+
+
+def cat(*clouds):
+    return syn.Cloud(*[np.concatenate([getattr(c, k) for c in clouds]) for k in
+                       ("pos", "log_scale", "rot", "opacity", "dc", "rest")], clouds[0].sh_degree).contiguous()
+
+
+def sh(cloud, degree, direction):
+    color = np.zeros(3, np.float32)
+    clamped = np.zeros(3, np.int32)
+    O.lib().orc_evaluate_sh(O.ptr(np.ascontiguousarray(cloud.dc[0])), O.ptr(np.ascontiguousarray(cloud.rest[0])),
+                            cloud.rest_stride, degree, O.ptr(np.asarray(direction, np.float32)), O.ptr(color),
+                            O.ptr(clamped, O.i32p))
+    return color, clamped
+
+
+# ---------------------------------------------------------------- exp --
+def test_deterministic_exp_is_within_a_few_ulp_of_libm():
+    xs = np.concatenate([np.linspace(-79.9, 88.0, 20001), np.linspace(-1, 1, 4001), [0.0, -1e-30, 1e-30]])
+    worst = 0.0
+    for x in xs.astype(np.float32):
+        ours = O.det_exp(float(x))
+        exact = math.exp(float(x))
+        ulp = np.spacing(np.float32(exact))
+        worst = max(worst, abs(ours - exact) / float(ulp))
+    assert worst <= 4.0
+    assert O.det_exp(-80.5) == 0.0 and math.isinf(O.det_exp(89.0)) and O.det_exp(0.0) == 1.0
+
+
+# ---------------------------------------------------------------- SH --
+def test_degree_zero_colour_is_the_dc_conversion():
+    """SplatRasterTest.cpp:134-140."""
+    g = gaussian(dc=(0.3, 0.7, 1.1))
+    color, _ = sh(g, 0, (0, 0, 1))
+    np.testing.assert_allclose(color, np.clip(0.5 + C0 * np.array([0.3, 0.7, 1.1]), 0, 1), atol=1e-6)
+
+
+def test_harmonics_vary_with_view_direction_and_degree_zero_does_not():
+    """SplatRasterTest.cpp:142-155."""
+    rng = syn.Rng(2)
+    g = gaussian(dc=(0.5, 0.6, 0.7), rest=[rng.uniform(-0.3, 0.3) for _ in range(45)])
+    front, _ = sh(g, 3, (0, 0, 1))
+    side, _ = sh(g, 3, (1, 0, 0))
+    assert np.abs(front - side).max() > 1e-3
+    a, _ = sh(g, 0, (0, 0, 1))
+    b, _ = sh(g, 0, (1, 0, 0))
+    assert np.abs(a - b).max() <= 1e-7
+
+
+def test_negative_sums_are_clamped_and_reported():
+    """SplatRasterTest.cpp:157-169."""
+    color, clamped = sh(gaussian(dc=(-10.0, -10.0, 2.0)), 0, (0, 0, 1))
+    assert color[0] == 0.0 and color[1] == 0.0 and color[2] > 0.0
+    assert clamped.tolist() == [1, 1, 0]
+
+
+def test_sh_basis_matches_its_closed_form_on_axes():
+    """Band 1 is -C1 y, C1 z, -C1 x: along +y only the first term survives."""
+    rest = np.zeros(45, np.float32)
+    rest[0] = 1.0  # R channel, basis 0
+    g = gaussian(dc=(0, 0, 0), rest=rest)
+    color, _ = sh(g, 1, (0, -1, 0))
+    assert abs(color[0] - (0.5 + 0.4886025119029199)) < 1e-6
+
+
+# -------------------------------------------------------- projection --
+def test_on_axis_gaussian_projects_to_the_principal_point():
+    """SplatRasterTest.cpp:171-181."""
+    view = syn.test_view()
+    p = O.Frame(gaussian(logit=2.0, rot=(0.5, 0.5, 0.5, 0.5)), view).projected()
+    assert len(p["index"]) == 1
+    assert abs(p["mean"][0, 0] - view.cx) < 1e-4 and abs(p["mean"][0, 1] - view.cy) < 1e-4
+    assert abs(p["depth"][0] - 2.0) < 1e-5
+
+
+def test_isotropic_gaussian_is_a_circle_that_shrinks_with_distance():
+    """SplatRasterTest.cpp:183-209."""
+    g = gaussian(log_scale=[math.log(0.05)] * 3, logit=3.0)
+    opt = O.default_options(screen_dilation=0.0)
+    radii = []
+    for dist in (1.0, 2.0):
+        p = O.Frame(g, syn.test_view(48, 36, dist), opt).projected()
+        assert abs(p["cov2d"][0, 0] - p["cov2d"][0, 2]) < 1e-3 and abs(p["cov2d"][0, 1]) < 1e-3
+        radii.append(p["radius"][0])
+    assert abs(radii[1] / radii[0] - 0.5) < 0.01
+
+
+def test_gaussians_behind_the_camera_are_dropped():
+    """SplatRasterTest.cpp:211-218."""
+    c = cat(gaussian(logit=3.0), gaussian(pos=(0, 0, -10), logit=3.0))
+    assert O.Frame(c, syn.test_view()).count == 1
+
+
+def test_transparent_gaussians_are_culled_by_min_alpha():
+    c = cat(gaussian(logit=3.0), gaussian(pos=(0.1, 0, 0), logit=-8.0))
+    assert O.Frame(c, syn.test_view()).count == 1
+
+
+# ------------------------------------------------------ compositing --
+def test_empty_scene_shows_the_background():
+    """SplatRasterTest.cpp:220-232 (exact equality)."""
+    c = syn.random_cloud(0, 0, 0, (0, 0, 0), (1, 1, 1))
+    rgb, T = O.render(c, syn.test_view(), O.default_options(background=(0.25, 0.5, 0.75)))
+    assert np.all(T == 1.0)
+    assert np.all(rgb.reshape(-1, 3) == np.array([0.25, 0.5, 0.75], np.float32))
+
+
+def test_near_opaque_gaussian_hides_the_one_behind():
+    """SplatRasterTest.cpp:234-259."""
+    view = syn.test_view()
+    ls = [math.log(0.4)] * 3
+    front = gaussian(pos=(0, 0, -0.5), log_scale=ls, logit=8.0, dc=(0.5 / C0, -0.5 / C0, -0.5 / C0))
+    back = gaussian(pos=(0, 0, 0.5), log_scale=ls, logit=8.0, dc=(-0.5 / C0, -0.5 / C0, 0.5 / C0))
+    c = cat(back, front)
+    rgb, _ = O.render(c, view)
+    px = (18 * 48 + 24) * 3
+    assert rgb[px] > 0.9 and rgb[px + 2] < 0.1
+    c.pos[[0, 1]] = c.pos[[1, 0]]
+    rgb, _ = O.render(c, view)
+    assert rgb[px] < 0.1 and rgb[px + 2] > 0.9
+
+
+def test_opacity_controls_how_much_background_survives():
+    """SplatRasterTest.cpp:261-282."""
+    prev = 2.0
+    for logit in (-2.0, 0.0, 2.0, 4.0):
+        g = gaussian(log_scale=[math.log(0.4)] * 3, logit=logit)
+        rgb, _ = O.render(g, syn.test_view(), O.default_options(background=(1, 1, 1)))
+        v = rgb[(18 * 48 + 24) * 3]
+        assert v < prev
+        prev = v
+
+
+def test_compositing_matches_a_brute_force_fp64_walk():
+    """Every pixel re-walked in FP64 over the same sorted list (no tiles)."""
+    cloud = syn.random_cloud(300, 3, 1, (-1, -1, 2), (1, 1, 6))
+    view = syn.pinhole(64, 48, 60.0)
+    f = O.Frame(cloud, view, O.default_options(sh_degree=1))
+    rgb, T, _ = f.image()
+    p = f.projected()
+    cols, rows, offsets, entries = f.tiles()
+    slot = {g: s for s, g in enumerate(p["index"])}
+    for y in range(0, 48, 5):
+        for x in range(0, 64, 3):
+            tile = (y // 16) * cols + x // 16
+            Td, acc = 1.0, np.zeros(3)
+            for g in entries[offsets[tile]:offsets[tile + 1]]:
+                s = slot[g]
+                dx, dy = x + 0.5 - float(p["mean"][s, 0]), y + 0.5 - float(p["mean"][s, 1])
+                a, b, c = (float(v) for v in p["conic"][s])
+                power = -0.5 * (a * dx * dx + c * dy * dy) - b * dx * dy
+                if power > 0:
+                    continue
+                alpha = min(0.99, float(p["opacity"][s]) * math.exp(power))
+                if alpha < 1 / 255:
+                    continue
+                acc += Td * alpha * p["color"][s].astype(np.float64)
+                Td *= 1 - alpha
+                if Td < 1e-4:
+                    break
+            k = y * 64 + x
+            assert abs(T[k] - Td) < 1e-5
+            np.testing.assert_allclose(rgb[3 * k:3 * k + 3], acc, atol=1e-5)
+
+
+# ---------------------------------------------------------- binning --
+@pytest.mark.parametrize("w,h", [(128, 96), (97, 61)])
+def test_tile_lists_equal_brute_force_rect_enumeration(w, h):
+    cloud = syn.random_cloud(2000, 8, 0, (-1, -1, 2), (1, 1, 6))
+    f = O.Frame(cloud, syn.pinhole(w, h, 100.0))
+    p = f.projected()
+    cols, rows, offsets, entries = f.tiles()
+    lists = [[] for _ in range(cols * rows)]
+    order = np.lexsort((p["index"], p["depth"]))  # depth, ties by index
+    for s in order:
+        mx, my, r = float(p["mean"][s, 0]), float(p["mean"][s, 1]), float(p["radius"][s])
+        x0 = max(0, math.floor(np.float32(mx - r) / 16))
+        x1 = min(cols - 1, math.floor(np.float32(mx + r) / 16))
+        y0 = max(0, math.floor(np.float32(my - r) / 16))
+        y1 = min(rows - 1, math.floor(np.float32(my + r) / 16))
+        for ty in range(y0, y1 + 1):
+            for tx in range(x0, x1 + 1):
+                lists[ty * cols + tx].append(int(p["index"][s]))
+    for t in range(cols * rows):
+        assert entries[offsets[t]:offsets[t + 1]].tolist() == lists[t]
+
+
+# -------------------------------------------------------------- loss --
+def test_identical_images_have_unit_similarity():
+    """SplatRasterTest.cpp:284-287."""
+    img = np.random.default_rng(11).uniform(size=32 * 24 * 3).astype(np.float32)
+    assert abs(O.mean_ssim(img, img, 32, 24)[0] - 1.0) < 1e-5
+
+
+def test_blurring_lowers_similarity():
+    """SplatRasterTest.cpp:289-301."""
+    w, h = 32, 24
+    sharp = np.zeros((h, w, 3), np.float32)
+    for y in range(h):
+        for x in range(w):
+            sharp[y, x] = 1.0 if (x // 4 + y // 4) % 2 == 0 else 0.0
+    flat = np.full_like(sharp, 0.5)
+    assert O.mean_ssim(flat.ravel(), sharp.ravel(), w, h)[0] < 0.5
+    assert O.mean_ssim(sharp.ravel(), sharp.ravel(), w, h)[0] > 0.99
+
+
+def test_ssim_gradient_matches_finite_differences():
+    """SplatRasterTest.cpp:303-324."""
+    w, h = 16, 12
+
+    def make_target(values, seed):  # makeTarget, SplatRasterTest.cpp:61-66
+        r = syn.Rng(seed)
+        return np.array([r.uniform(0.0, 1.0) for _ in range(values)], np.float32)
+
+    rendered = make_target(w * h * 3, 21)
+    target = make_target(w * h * 3, 22)
+    _, analytic = O.mean_ssim(rendered, target, w, h, want_grad=True)
+    rng = syn.Rng(23)
+    for _ in range(20):
+        i = int(rng.uniform(0.0, float(rendered.size)))
+        orig = rendered[i]
+        rendered[i] = np.float32(orig + 1e-3)
+        plus = O.mean_ssim(rendered, target, w, h)[0]
+        rendered[i] = np.float32(orig - 1e-3)
+        minus = O.mean_ssim(rendered, target, w, h)[0]
+        rendered[i] = orig
+        num = (plus - minus) / 2e-3
+        assert abs(analytic[i] - num) / max(abs(analytic[i]), abs(num), 1e-3) < 2e-2
+
+
+def test_blur_is_self_adjoint_on_the_zero_padded_domain():
+    w, h = 23, 17
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=w * h).astype(np.float32)
+    b = rng.normal(size=w * h).astype(np.float32)
+    ba, bb = np.zeros_like(a), np.zeros_like(b)
+    O.lib().orc_blur(O.ptr(a), w, h, O.ptr(ba))
+    O.lib().orc_blur(O.ptr(b), w, h, O.ptr(bb))
+    assert abs(float(np.dot(ba.astype(np.float64), b)) - float(np.dot(a.astype(np.float64), bb))) < 1e-4
+
+
+def test_loss_is_zero_on_a_perfect_match_and_mixes_both_terms():
+    """SplatRasterTest.cpp:326-339."""
+    img = np.random.default_rng(31).uniform(size=16 * 12 * 3).astype(np.float32)
+    other = np.random.default_rng(32).uniform(size=16 * 12 * 3).astype(np.float32)
+    assert abs(O.photometric_loss(img, img, 16, 12, 0.2, False)[0]) < 1e-4
+    l1 = O.photometric_loss(img, other, 16, 12, 0.0, False)[0]
+    mixed = O.photometric_loss(img, other, 16, 12, 0.2, False)[0]
+    assert l1 > 0 and l1 != mixed
+
+
+# ----------------------------------------------------------- adjoint --
+GROUPS = [("pos", 0, 3), ("log_scale", 1, 3), ("rot", 2, 4), ("opacity", 3, 1), ("dc", 4, 3), ("rest", 5, None)]
+
+
+def test_float_backward_matches_finite_differences_for_every_parameter():
+    """SplatRasterTest.cpp:341-415: one 16x16 tile, SH2, background,
+    thresholds off, h = 2e-3, rel < 5e-2 (8e-2 for quaternions)."""
+    view = syn.test_view(16, 16)
+    opt = O.default_options(sh_degree=2, background=(0.2, 0.3, 0.4), min_alpha=0.0, transmittance_cutoff=0.0)
+    cloud = syn.overlapping_cloud(2)
+    w = (np.random.default_rng(41).uniform(size=16 * 16 * 3) - 0.5).astype(np.float32)
+
+    def loss(c):
+        rgb, _ = O.render(c, view, opt)
+        return float(np.dot(w.astype(np.float64), rgb.astype(np.float64)))
+
+    g = O.Frame(cloud, view, opt).backward(w)
+    for name, _, count in GROUPS:
+        arr = getattr(cloud, name)
+        grad = getattr(g, name).reshape(cloud.n, -1)
+        comps = range(arr.reshape(cloud.n, -1).shape[1]) if count else range(0, cloud.rest_stride, 2)
+        for i in range(cloud.n):
+            for k in comps:
+                c = cloud.copy()
+                flat = getattr(c, name).reshape(cloud.n, -1)
+                v0 = flat[i, k]
+                flat[i, k] = np.float32(v0 + 2e-3)
+                plus = loss(c)
+                flat[i, k] = np.float32(v0 - 2e-3)
+                minus = loss(c)
+                num = (plus - minus) / 4e-3
+                an = float(grad[i, k])
+                tol = 8e-2 if name == "rot" else 5e-2
+                assert abs(an - num) / max(abs(an), abs(num), 1e-3) < tol, (name, i, k, an, num)
+
+
+@pytest.mark.parametrize("seed,deg,n,size", [(5, 3, 1500, (96, 64)), (6, 1, 800, (64, 64))])
+def test_fp64_adjoint_matches_fp64_finite_differences(seed, deg, n, size):
+    """FP64 central differences (h = 1e-4) of the frozen-decision forward on
+    sampled parameters, with the real thresholds on."""
+    cloud = syn.random_cloud(n, seed, deg, (-1, -1, 2), (1, 1, 6))
+    view = syn.pinhole(size[0], size[1], 80.0)
+    f = O.Frame(cloud, view, O.default_options(sh_degree=deg, background=(0.1, 0.2, 0.3)))
+    w = (np.random.default_rng(seed).uniform(size=view.pixels * 3) - 0.5).astype(np.float32)
+    cd = O.ccloud_d(cloud)
+    gd = f.backward_f64(w, cd)
+    idx = f.projected()["index"]
+    rng = np.random.default_rng(seed + 100)
+    checked = 0
+    for _ in range(120):
+        g = int(idx[rng.integers(len(idx))])
+        name, gid, count = GROUPS[int(rng.integers(len(GROUPS)))]
+        count = count or cloud.rest_stride
+        if count == 0:
+            continue
+        comp = int(rng.integers(count))
+        num = f.fd_f64(cd, w.astype(np.float64), g, gid, comp, 1e-4)
+        an = float(getattr(gd, name).reshape(cloud.n, -1)[g, comp])
+        assert abs(an - num) <= 1e-3 * max(abs(an), abs(num), 1e-6), (name, g, comp, an, num)
+        checked += 1
+    assert checked > 80
+
+
+def test_fp64_and_float_adjoints_agree():
+    cloud = syn.random_cloud(2000, 12, 3, (-1, -1, 2), (1, 1, 6))
+    view = syn.pinhole(128, 96, 120.0)
+    f = O.Frame(cloud, view, O.default_options(sh_degree=3))
+    w = (np.random.default_rng(2).uniform(size=view.pixels * 3) - 0.5).astype(np.float32) * 1e-3
+    a, b = f.backward(w).flat(), f.backward_f64(w).flat()
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
+
+
+def test_backward_accumulates():
+    """SplatRasterTest.cpp:417-442."""
+    cloud, view = syn.overlapping_cloud(0, 3), syn.test_view(24, 20)
+    f = O.Frame(cloud, view)
+    rgb, _, _ = f.image()
+    tgt = np.random.default_rng(51).uniform(size=rgb.size).astype(np.float32)
+    _, dl = O.photometric_loss(rgb, tgt, 24, 20, 0.0)
+    once = f.backward(dl)
+    twice = f.backward(dl, f.backward(dl))
+    np.testing.assert_allclose(2 * once.opacity, twice.opacity, atol=1e-5)
+    assert np.array_equal(2 * once.visible_count, twice.visible_count)
+    np.testing.assert_allclose(2 * once.pos, twice.pos, atol=1e-5)
+
+
+def test_screen_gradient_is_larger_for_a_splat_the_loss_fights():
+    """SplatRasterTest.cpp:444-470."""
+    view = syn.test_view(48, 32)
+    c = cat(*[gaussian(pos=(x, 0, 0), log_scale=[math.log(0.1)] * 3, logit=2.0, dc=(1, 1, 1)) for x in (-0.5, 0.5)])
+    target, _ = O.render(c, view)
+
+    def grads(cloud):
+        f = O.Frame(cloud, view)
+        rgb, _, _ = f.image()
+        _, dl = O.photometric_loss(rgb, target, 48, 32, 0.0)
+        return f.backward(dl)
+
+    settled = grads(c)
+    assert settled.screen_gradient.max() < 1e-6
+    moved = c.copy()
+    moved.pos[1, 0] += 0.08
+    g = grads(moved)
+    assert g.screen_gradient[1] > 10 * max(g.screen_gradient[0], 1e-9)
+    assert g.visible_count.tolist() == [1, 1]
+
+
+# -------------------------------------------------------------- adam --
+def test_adam_matches_the_closed_form():
+    """Kingma & Ba with bias correction, per group learning rates."""
+    cloud = syn.random_cloud(7, 1, 1, (-1, -1, 2), (1, 1, 6))
+    g = O.Grads(cloud.n, cloud.rest_stride)
+    rng = np.random.default_rng(0)
+    for k in ("pos", "log_scale", "rot", "opacity", "dc", "rest"):
+        getattr(g, k)[...] = rng.normal(size=getattr(g, k).shape).astype(np.float32)
+    cfg = O.default_config(iterations=10)
+    m = np.zeros(cloud.n * (14 + cloud.rest_stride), np.float32)
+    v = np.zeros_like(m)
+    out = O.adam_apply(cloud.copy(), g, m, v, 0, 2.0, cfg)
+    # first step: m_hat = g, v_hat = g^2, delta = -lr g / (|g| + eps) = -lr sign(g)
+    # (the differences carry one rounding of the parameter itself: atol = 2 ulp)
+    np.testing.assert_allclose(out.opacity - cloud.opacity, -0.05 * np.sign(g.opacity), rtol=1e-5, atol=5e-7)
+    np.testing.assert_allclose(out.dc - cloud.dc, -2.5e-3 * np.sign(g.dc), rtol=1e-4, atol=5e-7)
+    np.testing.assert_allclose(out.rest - cloud.rest, -1.25e-4 * np.sign(g.rest), rtol=1e-4, atol=1e-7)
+    np.testing.assert_allclose(out.pos - cloud.pos, -1.6e-4 * 2.0 * np.sign(g.pos), rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(out.rot, axis=1), 1.0, rtol=1e-6)
+    np.testing.assert_allclose(m[:3 * cloud.n], 0.1 * g.pos.ravel(), rtol=1e-6)
+
+
+# ----------------------------------------------------------- trainer --
+def training_set(views):
+    truth = syn.truth_cloud()
+    return [O.render(truth, v)[0] for v in views]
+
+
+def short(iterations, **kw):
+    base = dict(iterations=iterations, sh_degree=0, sh_increase_interval=0, densify_from_iteration=iterations + 1,
+                densify_until_iteration=iterations + 1, opacity_reset_interval=0, ssim_weight=0.2)
+    base.update(kw)
+    return O.default_config(**base)
+
+
+def test_scene_extent_follows_the_camera_rig():
+    """SplatTrainerTest.cpp:129-136."""
+    for radius in (1.0, 4.0):
+        assert abs(O.camera_extent(syn.ring_of_views(4, radius, 16, 16)) - 1.1 * radius) < 1e-3
+
+
+def test_training_drives_the_loss_down():
+    """SplatTrainerTest.cpp:138-148 (shortened to 150 iterations)."""
+    views = syn.ring_of_views(6, 1.6, 32, 32)
+    t = O.Trainer(syn.rough_start(), views, training_set(views), short(150))
+    assert t.step()
+    first = t.last_loss
+    while t.step():
+        pass
+    assert t.smoothed_loss < 0.6 * first
+
+
+def test_training_is_deterministic_with_densify_on():
+    """SplatTrainerTest.cpp:171-193."""
+    views = syn.ring_of_views(4, 1.6, 16, 16)
+    imgs = training_set(views)
+    cfg = short(60, densify_from_iteration=20, densify_until_iteration=60, densify_interval=20,
+                densify_grad_threshold=0.0)
+    runs = []
+    for _ in range(2):
+        t = O.Trainer(syn.rough_start(8), views, imgs, cfg)
+        while t.step():
+            pass
+        t.finish()
+        runs.append(t.cloud())
+    assert runs[0].n == runs[1].n
+    assert np.array_equal(runs[0].pos, runs[1].pos) and np.array_equal(runs[0].opacity, runs[1].opacity)
+
+
+def test_sh_bands_unlock_on_schedule():
+    """SplatTrainerTest.cpp:211-228."""
+    views = syn.ring_of_views(3, 1.6, 12, 12)
+    start = syn.rough_start(4)
+    start = syn.Cloud(start.pos, start.log_scale, start.rot, start.opacity, start.dc,
+                      np.zeros((4, 45), np.float32), 3)
+    t = O.Trainer(start, views, training_set(views), short(31, sh_degree=3, sh_increase_interval=10))
+    assert t.sh_degree_in_use == 0
+    for target in (1, 2, 3):
+        while t.iteration < target * 10:
+            assert t.step()
+        assert t.sh_degree_in_use == target
+
+
+def test_densification_grows_only_when_the_gradient_asks_and_respects_the_cap():
+    """SplatTrainerTest.cpp:230-265."""
+    views = syn.ring_of_views(4, 1.6, 24, 24)
+    imgs = training_set(views)
+    kw = dict(densify_from_iteration=10, densify_until_iteration=20, densify_interval=10)
+    grown = O.Trainer(syn.rough_start(8), views, imgs, short(20, densify_grad_threshold=0.0, **kw))
+    while grown.step():
+        pass
+    assert grown.count > 8
+    same = O.Trainer(syn.rough_start(8), views, imgs, short(20, densify_grad_threshold=1e9, **kw))
+    while same.step():
+        pass
+    assert same.count == 8
+    capped = O.Trainer(syn.rough_start(10), views, imgs,
+                       short(40, densify_from_iteration=5, densify_until_iteration=40, densify_interval=5,
+                             densify_grad_threshold=0.0, max_gaussians=12))
+    while capped.step():
+        pass
+    assert capped.count <= 12
+
+
+def test_transparent_gaussians_are_pruned_and_opacity_reset_caps():
+    """SplatTrainerTest.cpp:267-302."""
+    views = syn.ring_of_views(4, 1.6, 16, 16)
+    imgs = training_set(views)
+    start = syn.rough_start(10)
+    start.opacity[:] = -20.0
+    t = O.Trainer(start, views, imgs, short(10, densify_from_iteration=10, densify_until_iteration=10,
+                                             densify_interval=10, densify_grad_threshold=1e9, opacity_lr=0.0))
+    while t.step():
+        pass
+    assert t.count == 0
+    start = syn.rough_start(6)
+    start.opacity[:] = 5.0
+    t = O.Trainer(start, views, imgs, short(60, opacity_reset_interval=20, densify_until_iteration=40,
+                                             densify_from_iteration=41, opacity_lr=0.0))
+    while t.iteration < 20:
+        assert t.step()
+    op = 1 / (1 + np.exp(-t.cloud().opacity.astype(np.float64)))
+    assert np.all(op < 0.011)
+
+
+def test_empty_inputs_finish_and_finish_sorts():
+    """SplatTrainerTest.cpp:304-339."""
+    views = syn.ring_of_views(2, 1.6, 8, 8)
+    imgs = training_set(views)
+    empty = O.Trainer(syn.random_cloud(0, 0, 0, (0, 0, 0), (1, 1, 1)), views, imgs, short(10))
+    assert not empty.step()
+    no_views = O.Trainer(syn.rough_start(4), [], [], short(10))
+    assert not no_views.step() and no_views.count == 4
+    start = syn.rough_start(6)
+    start.opacity[[0, 3]] = -30.0
+    views = syn.ring_of_views(3, 1.6, 12, 12)
+    t = O.Trainer(start, views, training_set(views), short(1, opacity_lr=0.0))
+    while t.step():
+        pass
+    assert t.finish() == 4
+    op = t.cloud().opacity
+    assert np.all(op[:-1] >= op[1:])
