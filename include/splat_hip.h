@@ -199,6 +199,12 @@ splat_status splat_densify(splat_context* ctx, const splat_densify_params* param
 splat_status splat_reset_opacity(splat_context* ctx, float ceiling);
 /* Accumulated densify statistics since the last densify (or upload). */
 splat_status splat_download_stats(splat_context* ctx, float* screen_gradient, int32_t* visible_count);
+/* Restore accumulated densify statistics (resume; n = splat_count() each). */
+splat_status splat_upload_stats(splat_context* ctx, const float* screen_gradient, const int32_t* visible_count);
+/* Adam moments, group-concatenated by count: [pos 3n | log_scale 3n | rot 4n
+ * | opacity n | dc 3n | rest rs n] (snapshot / resume of the optimiser). */
+splat_status splat_download_moments(splat_context* ctx, float* m, float* v);
+splat_status splat_upload_moments(splat_context* ctx, const float* m, const float* v);
 /* Prune (opacity < min_opacity, non-finite or > 1e3 scale) and stable sort by
  * descending opacity logit (GaussianCloud.cpp:177-197), on the device. */
 splat_status splat_prune_and_sort(splat_context* ctx, float min_opacity);

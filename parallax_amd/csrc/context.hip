@@ -792,6 +792,55 @@ splat_status splat_download_stats(splat_context* ctx, float* screen_gradient, in
   return SPLAT_OK;
 }
 
+splat_status splat_upload_stats(splat_context* ctx, const float* screen_gradient, const int32_t* visible_count) {
+  if (splat_status s = check_ready(ctx)) return s;
+  if (screen_gradient)
+    CK(hipMemcpyAsync(ctx->stat_sg.f(), screen_gradient, 4 * (size_t)ctx->n, hipMemcpyHostToDevice, ctx->stream),
+       "upload");
+  if (visible_count)
+    CK(hipMemcpyAsync(ctx->stat_vc.i(), visible_count, 4 * (size_t)ctx->n, hipMemcpyHostToDevice, ctx->stream),
+       "upload");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+
+namespace {
+// Copies between the capacity layout and the count-concatenated layout.
+splat_status moments_copy(splat_context* ctx, float* host_m, float* host_v, bool to_host) {
+  const long long cap = ctx->cap, n = ctx->n;
+  const long long offs[6] = {0, 3 * cap, 6 * cap, 10 * cap, 11 * cap, 14 * cap};
+  const long long widths[6] = {3, 3, 4, 1, 3, ctx->rs};
+  long long packed = 0;
+  for (int q = 0; q < 6; ++q) {
+    const size_t bytes = 4 * (size_t)(widths[q] * n);
+    if (bytes > 0) {
+      for (int which = 0; which < 2; ++which) {
+        float* host = which == 0 ? host_m : host_v;
+        float* dev = (which == 0 ? ctx->m.f() : ctx->v.f()) + offs[q];
+        if (!host) continue;
+        if (to_host)
+          CK(hipMemcpyAsync(host + packed, dev, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+        else
+          CK(hipMemcpyAsync(dev, host + packed, bytes, hipMemcpyHostToDevice, ctx->stream), "upload");
+      }
+    }
+    packed += widths[q] * n;
+  }
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return SPLAT_OK;
+}
+}  // namespace
+
+splat_status splat_download_moments(splat_context* ctx, float* m, float* v) {
+  if (splat_status s = check_ready(ctx)) return s;
+  return moments_copy(ctx, m, v, true);
+}
+
+splat_status splat_upload_moments(splat_context* ctx, const float* m, const float* v) {
+  if (splat_status s = check_ready(ctx)) return s;
+  return moments_copy(ctx, const_cast<float*>(m), const_cast<float*>(v), false);
+}
+
 splat_status splat_set_timing(splat_context* ctx, int32_t enabled) {
   if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
   ctx->timing = enabled != 0;

@@ -96,6 +96,9 @@ SIGNATURES = {
     "splat_reset_opacity": (C.c_int, [C.c_void_p, C.c_float]),
     "splat_download_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
     "splat_prune_and_sort": (C.c_int, [C.c_void_p, C.c_float]),
+    "splat_upload_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
+    "splat_download_moments": (C.c_int, [C.c_void_p, f32p, f32p]),
+    "splat_upload_moments": (C.c_int, [C.c_void_p, f32p, f32p]),
     "splat_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "splat_timing_count": (C.c_int32, [C.c_void_p]),
     "splat_timing_entry": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
@@ -344,6 +347,23 @@ class Context:
         vc = np.zeros(n, np.int32)
         self._ck(lib().splat_download_stats(self.h, _p(sg), _p(vc, i32p)))
         return sg, vc
+
+    def upload_stats(self, screen_gradient, visible_count):
+        sg = np.ascontiguousarray(screen_gradient, dtype=np.float32)
+        vc = np.ascontiguousarray(visible_count, dtype=np.int32)
+        self._ck(lib().splat_upload_stats(self.h, _p(sg), _p(vc, i32p)))
+
+    def moments(self):
+        k = self.count * (14 + self.rest_stride)
+        m = np.zeros(k, np.float32)
+        v = np.zeros(k, np.float32)
+        self._ck(lib().splat_download_moments(self.h, _p(m), _p(v)))
+        return m, v
+
+    def upload_moments(self, m, v):
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        self._ck(lib().splat_upload_moments(self.h, _p(m), _p(v)))
 
     def prune_and_sort(self, min_opacity: float):
         self._ck(lib().splat_prune_and_sort(self.h, float(min_opacity)))
