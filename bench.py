@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--deterministic", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
     return ap.parse_args()
 
 
@@ -72,17 +74,22 @@ def main():
     opts = S.options(sh_degree=3, deterministic=args.deterministic)
     ctx = S.Context(0)
     ctx.set_capacity(n, 3)
-    # targets: renders of the second seeded set, on the device
-    ctx.upload_cloud(target)
-    ctx.upload_views(views)
-    images = []
-    for k in range(len(views)):
-        ctx.render_forward([k], S.options(sh_degree=3))
-        images.append(ctx.image(0)[0])
+    if args.profile:
+        # no extra forward launches in the profile: fixed-seed noise targets
+        rng = np.random.default_rng(5)
+        images = [rng.uniform(size=v.pixels * 3).astype(np.float32) for v in views]
+    else:
+        # targets: renders of the second seeded set, on the device
+        ctx.upload_cloud(target)
+        ctx.upload_views(views)
+        images = []
+        for k in range(len(views)):
+            ctx.render_forward([k], S.options(sh_degree=3))
+            images.append(ctx.image(0)[0])
     ctx.upload_cloud(cloud)
     ctx.upload_views(views, images)
     del images
-    log("targets rendered; cloud resident")
+    log("targets ready; cloud resident")
 
     def adam(it):
         return S.adam_params(it, 1.6e-4 * 1.1 * 6.0)  # position lr x extent of a radius-6 rig
@@ -106,6 +113,10 @@ def main():
     ctx.synchronize()
     batched_s = (time.perf_counter() - t0) / args.steps
     log(f"batched: {batched_s * 1e3:.2f} ms per step of {B} views")
+    if args.profile:
+        ctx.close()
+        print(json.dumps({"profile": True, "batched_ms_per_step": round(batched_s * 1e3, 3), "batch_views": B}))
+        return
 
     # ---- plain per-view ----
     plain_steps = max(args.steps, 2 * B)

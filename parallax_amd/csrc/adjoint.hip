@@ -78,6 +78,7 @@ __device__ __forceinline__ void sh_jacobian(int count, const float d[3], float j
   set(14, kShC3[6], 3.f * xx - 3.f * yy, -6.f * x * y, 0.f);
 }
 
+template <int kDeg>
 __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc* __restrict__ views,
                                                  const float* __restrict__ params, const SplatRec* __restrict__ rec,
                                                  const float* __restrict__ screen,
@@ -121,6 +122,16 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
     for (int j = 0; j < 3; ++j) S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
   const float raw_norm = fmaxf(__builtin_sqrtf(sqn), 1e-12f);
   const int per_channel = rs / 3;
+  constexpr int kCount = kDeg == 0 ? 0 : (kDeg == 1 ? 3 : (kDeg == 2 ? 8 : 15));
+  constexpr int kC = kCount > 0 ? kCount : 1;
+  float rc[3][kC], grc[3][kC];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int k = 0; k < kCount; ++k) {
+      rc[c][k] = rest[c * per_channel + k];
+      grc[c][k] = 0.f;
+    }
 
   float gpos[3] = {0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f}, grot[4] = {0.f, 0.f, 0.f, 0.f};
   float gop = 0.f, gdc[3] = {0.f, 0.f, 0.f};
@@ -161,19 +172,18 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
     float ddir[3] = {0.f, 0.f, 0.f};
     const float off[3] = {pos[0] - V.C[0], pos[1] - V.C[1], pos[2] - V.C[2]};
     const float dist = __builtin_sqrtf(off[0] * off[0] + off[1] * off[1] + off[2] * off[2]);
-    if (p.sh_use >= 1 && dist > 1e-9f && per_channel > 0) {
+    if (kDeg >= 1 && dist > 1e-9f && per_channel > 0) {
       const float dir[3] = {off[0] / dist, off[1] / dist, off[2] / dist};
       float basis[15], jac[15][3];
-      const int count = sh_basis(p.sh_use, dir, basis);
-      const int usable = count < per_channel ? count : per_channel;
-      sh_jacobian(count, dir, jac);
+      sh_basis(kDeg, dir, basis);
+      sh_jacobian(kCount, dir, jac);
+#pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (dcol[c] == 0.f) continue;
-        const float* rr = rest + c * per_channel;
-        float* dst = grest + c * per_channel;
-        for (int b = 0; b < usable; ++b) {
-          dst[b] += basis[b] * dcol[c];
-          const float k = rr[b] * dcol[c];
+#pragma unroll
+        for (int b = 0; b < kCount; ++b) {
+          grc[c][b] += basis[b] * dcol[c];
+          const float k = rc[c][b] * dcol[c];
           ddir[0] += jac[b][0] * k;
           ddir[1] += jac[b][1] * k;
           ddir[2] += jac[b][2] * k;
@@ -269,6 +279,10 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
   }
   for (int a = 0; a < 4; ++a) grads[6 * cap + 4 * (long long)g + a] += grot[a];
   grads[10 * cap + g] += gop;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int k = 0; k < kCount; ++k) grest[c * per_channel + k] += grc[c][k];
   sg[g] += sg_sum;
   vc[g] += vc_sum;
 }
@@ -279,8 +293,27 @@ hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const f
                           const float* screen, const uint32_t* pair_base, const uint32_t* counts,
                           const float* partial, float* grads, float* sg, int* vc, hipStream_t stream) {
   if (p.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_adjoint, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, views, params, rec, screen,
-                     pair_base, counts, partial, grads, sg, vc);
+  if (3 * (p.sh_use == 0 ? 0 : (p.sh_use == 1 ? 3 : (p.sh_use == 2 ? 8 : 15))) > p.rest_stride)
+    return hipErrorInvalidValue;
+  const dim3 grid((p.n + 255) / 256);
+  switch (p.sh_use) {
+    case 0:
+      hipLaunchKernelGGL(k_adjoint<0>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
+                         partial, grads, sg, vc);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_adjoint<1>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
+                         partial, grads, sg, vc);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_adjoint<2>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
+                         partial, grads, sg, vc);
+      break;
+    default:
+      hipLaunchKernelGGL(k_adjoint<3>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
+                         partial, grads, sg, vc);
+      break;
+  }
   return hipGetLastError();
 }
 

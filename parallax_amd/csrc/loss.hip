@@ -191,20 +191,34 @@ __global__ __launch_bounds__(256) void k_loss_backward(LossParams p, const ViewD
   }
 }
 
-// One thread per view: the view's tiles' sums in tile order.
-__global__ void k_loss_reduce(const ViewDesc* __restrict__ views, int n_views, const float* __restrict__ block_sums,
-                              float* __restrict__ out) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n_views) return;
+// One workgroup per view: each thread sums a strided run of the view's tile
+// sums in tile order, then a fixed-shape tree combines the 256 partials, so
+// the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_loss_reduce(const ViewDesc* __restrict__ views,
+                                                     const float* __restrict__ block_sums, float* __restrict__ out) {
+  __shared__ float s_l1[256], s_ss[256];
+  const int v = blockIdx.x;
   const ViewDesc& V = views[v];
   const int tiles = V.cols * V.rows;
   float l1 = 0.f, ssim = 0.f;
-  for (int t = 0; t < tiles; ++t) {
+  for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
     l1 += block_sums[2 * (V.tile_base + t)];
     ssim += block_sums[2 * (V.tile_base + t) + 1];
   }
-  out[2 * v] = l1;
-  out[2 * v + 1] = ssim;
+  s_l1[threadIdx.x] = l1;
+  s_ss[threadIdx.x] = ssim;
+  __syncthreads();
+  for (int width = 128; width > 0; width >>= 1) {
+    if ((int)threadIdx.x < width) {
+      s_l1[threadIdx.x] += s_l1[threadIdx.x + width];
+      s_ss[threadIdx.x] += s_ss[threadIdx.x + width];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * v] = s_l1[0];
+    out[2 * v + 1] = s_ss[0];
+  }
 }
 
 }  // namespace
@@ -228,7 +242,7 @@ hipError_t launch_loss_backward(const LossParams& p, const ViewDesc* views, cons
 hipError_t launch_loss_reduce(const ViewDesc* views, int n_views, const float* block_sums, float* out,
                               hipStream_t stream) {
   if (n_views <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(64), 0, stream, views, n_views, block_sums, out);
+  hipLaunchKernelGGL(k_loss_reduce, dim3(n_views), dim3(256), 0, stream, views, block_sums, out);
   return hipGetLastError();
 }
 

@@ -53,8 +53,9 @@ __device__ __forceinline__ int sh_basis(int degree, float x, float y, float z, f
   return 15;
 }
 
+template <int kDeg>
 __global__ __launch_bounds__(256) void k_project(const float* __restrict__ params, long long cap, int n,
-                                                 int rest_stride, int sh_use, const ViewDesc* __restrict__ views,
+                                                 int rest_stride, const ViewDesc* __restrict__ views,
                                                  int n_views, float dilation, float min_alpha,
                                                  SplatRec* __restrict__ rec, uint32_t* __restrict__ keys,
                                                  uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
@@ -97,6 +98,13 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
   const float opacity = ((1.0f) / (1.0f + det_exp(-logit)));
   const float dc0 = dcp[0], dc1 = dcp[1], dc2 = dcp[2];
   const int per_channel = rest_stride / 3;
+  // the SH coefficients this degree uses, loaded once for all views
+  constexpr int kCount = kDeg == 0 ? 0 : (kDeg == 1 ? 3 : (kDeg == 2 ? 8 : 15));
+  float rc[3][kCount > 0 ? kCount : 1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int k = 0; k < kCount; ++k) rc[c][k] = rest[c * per_channel + k];
 
   for (int v = 0; v < n_views; ++v) {
     const ViewDesc& V = views[v];
@@ -164,13 +172,13 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
             d0 = ((d0) / (nrm)); d1 = ((d1) / (nrm)); d2 = ((d2) / (nrm));
           }
           float basis[15];
-          const int nb = sh_basis(sh_use, d0, d1, d2, basis);
-          const int usable = nb < per_channel ? nb : per_channel;
+          sh_basis(kDeg, d0, d1, d2, basis);
           float col[3] = {0.5f + kShC0 * dc0, 0.5f + kShC0 * dc1, 0.5f + kShC0 * dc2};
+#pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const float* r = rest + c * per_channel;
             float sum = 0.f;
-            for (int k = 0; k < usable; ++k) sum += basis[k] * r[k];
+#pragma unroll
+            for (int k = 0; k < kCount; ++k) sum += basis[k] * rc[c][k];
             col[c] += sum;
           }
           for (int c = 0; c < 3; ++c) {
@@ -221,8 +229,26 @@ hipError_t launch_project(const float* params, long long cap, int n, int rest_st
                           uint32_t* keys, uint32_t* vals, uint32_t* counts, hipStream_t stream) {
   if (n <= 0 || n_views <= 0) return hipSuccess;
   const int blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, sh_use, views,
-                     n_views, dilation, min_alpha, rec, keys, vals, counts);
+  // the cloud's coefficients must cover the degree in use
+  if (3 * (sh_use == 0 ? 0 : (sh_use == 1 ? 3 : (sh_use == 2 ? 8 : 15))) > rest_stride) return hipErrorInvalidValue;
+  switch (sh_use) {
+    case 0:
+      hipLaunchKernelGGL(k_project<0>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
+                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_project<1>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
+                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_project<2>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
+                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+      break;
+    default:
+      hipLaunchKernelGGL(k_project<3>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
+                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+      break;
+  }
   return hipGetLastError();
 }
 

@@ -16,8 +16,10 @@
 // pixel's list back to front from its last contributor, recomputing the
 // transmittance by division (nothing per splat and pixel is stored), and
 // accumulates per splat dL/dcolour, dL/dopacity, dL/dmean, |dL/dmean| and
-// dL/dconic. Per-pixel terms are summed across the wave64 with cross-lane
-// shuffles, the four waves' sums are combined in LDS in a fixed order, and
+// dL/dconic. Per-pixel terms are summed across the wave64 by a transposed
+// DPP butterfly (16 values in ~4 exchange levels, every lane ending with one
+// value's wave total), the four waves' sums are combined in LDS in a fixed
+// order, and
 // one workgroup result per (splat, tile) pair then goes out either as a
 // hardware float atomic (atomic mode) or as a plain store into that pair's
 // own slot for a later fixed-order per-gaussian sum (deterministic mode).
@@ -28,7 +30,8 @@
 namespace splat {
 namespace {
 
-constexpr int kBatch = 256;
+constexpr int kBatch = 256;   // forward: splats staged per LDS batch
+constexpr int kBatchB = 128;  // adjoint: smaller, so its per-wave partials fit 5 workgroups per CU
 constexpr int kVals = 11;  // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3, touched
 
 __device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int tile) {
@@ -105,10 +108,57 @@ __global__ __launch_bounds__(256) void k_forward(RasterParams p, const ViewDesc*
   }
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
+// Cross-lane exchange with the partner whose lane index differs in bit
+// kBit (0..3), through DPP: quad_perm for bits 0 and 1, row_half_mirror then
+// quad_perm [3,2,1,0] for bit 2, row_ror:8 for bit 3.
+template <int kCtrl>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
+template <int kBit>
+__device__ __forceinline__ float xchg(float v) {
+  if constexpr (kBit == 0) return dpp<0xB1>(v);
+  else if constexpr (kBit == 1) return dpp<0x4E>(v);
+  else if constexpr (kBit == 2) return dpp<0x1B>(dpp<0x141>(v));
+  else return dpp<0x128>(v);
+}
+
+// Sum 16 per-lane values over the wave64 in one transposed butterfly: each
+// level exchanges half of the values a lane holds with its partner and keeps
+// the other half, so 16 values -> 1 in four DPP levels (lane bits 0-3), then
+// two cross-row steps. Every lane ends with the wave total of one value: slot
+// bitrev4(lane & 15). The summation order is fixed, so it is deterministic.
+__device__ __forceinline__ float butterfly16(const float (&v)[16], int lane) {
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+  float w[8], x[4], y[2];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  for (int k = 0; k < 8; ++k) {
+    const float send = b0 ? v[k] : v[k + 8];
+    const float keep = b0 ? v[k + 8] : v[k];
+    w[k] = keep + xchg<0>(send);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float send = b1 ? w[k] : w[k + 4];
+    const float keep = b1 ? w[k + 4] : w[k];
+    x[k] = keep + xchg<1>(send);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const float send = b2 ? x[k] : x[k + 2];
+    const float keep = b2 ? x[k + 2] : x[k];
+    y[k] = keep + xchg<2>(send);
+  }
+  const float send = b3 ? y[0] : y[1];
+  const float keep = b3 ? y[1] : y[0];
+  float z = keep + xchg<3>(send);
+  z += __shfl_xor(z, 16, 64);
+  z += __shfl_xor(z, 32, 64);
+  return z;
+}
+
+__device__ __forceinline__ int butterfly_slot(int lane) {
+  return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
 }
 
 __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
@@ -119,13 +169,15 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
                                                   const int* __restrict__ last_in, float* __restrict__ screen,
                                                   const uint32_t* __restrict__ pair_base, float* __restrict__ partial,
                                                   int deterministic) {
-  __shared__ float4 s_a[kBatch];
-  __shared__ float4 s_b[kBatch];
-  __shared__ float s_c[kBatch];
-  __shared__ uint32_t s_dst[kBatch];        // slot (atomic) or pair id (deterministic)
-  __shared__ float s_part[4][kBatch][kVals];  // per-wave sums
+  __shared__ float4 s_a[kBatchB];
+  __shared__ float4 s_b[kBatchB];
+  __shared__ float s_c[kBatchB];
+  __shared__ uint32_t s_dst[kBatchB];          // slot (atomic) or pair id (deterministic)
+  __shared__ float s_part[4][kBatchB][kVals];  // per-wave sums
   __shared__ int s_maxlast;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int my_slot = butterfly_slot(lane);
+  const bool writer = lane < 16 && my_slot < kVals;
   const int tile = blockIdx.x;
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
@@ -167,7 +219,7 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
   uint32_t hi = stop;
   while (hi > begin) {
-    const uint32_t lo = (hi - begin) > (uint32_t)kBatch ? hi - kBatch : begin;
+    const uint32_t lo = (hi - begin) > (uint32_t)kBatchB ? hi - kBatchB : begin;
     const int cnt = (int)(hi - lo);
     __syncthreads();
     if (tid < cnt) {
@@ -186,15 +238,17 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
     for (int j = cnt - 1; j >= 0; --j) {
       const uint32_t e = lo + j;
       bool contrib = inside && (int)(e - begin) < lastp;
-      float val[kVals];
+      float val[16];
 #pragma unroll
-      for (int k = 0; k < kVals; ++k) val[k] = 0.f;
+      for (int k = 0; k < 16; ++k) val[k] = 0.f;
       const float4 a = s_a[j];
       const float4 b = s_b[j];
+      // the forward's decisions, recomputed with its exact float operations
+      float dx = 0.f, dy = 0.f, G = 0.f, raw = 0.f, alpha = 0.f;
       if (contrib) {
-        const float dx = fx - a.x, dy = fy - a.y;
+        dx = fx - a.x;
+        dy = fy - a.y;
         const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-        float G = 0.f, raw = 0.f, alpha = 0.f;
         if (power > 0.f) {
           contrib = false;
         } else {
@@ -203,49 +257,43 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
           alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
           if (alpha < p.min_alpha) contrib = false;
         }
-        if (contrib) {
-          const float c2 = s_c[j];
-          T = T / (1.f - alpha);
-          const float weight = alpha * T;
-          val[0] = weight * dL0;
-          val[1] = weight * dL1;
-          val[2] = weight * dL2;
-          suf0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
-          suf1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
-          suf2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
-          lc0 = b.z;
-          lc1 = b.w;
-          lc2 = c2;
-          float dL_dalpha = 0.f;
-          dL_dalpha += (b.z - suf0) * T * dL0;
-          dL_dalpha += (b.w - suf1) * T * dL1;
-          dL_dalpha += (c2 - suf2) * T * dL2;
-          last_alpha = alpha;
-          dL_dalpha += -T_final / (1.f - alpha) * bg_term;
-          if (raw < kMaxAlpha) {
-            val[3] = G * dL_dalpha;
-            const float dL_dweight = b.y * dL_dalpha;
-            const float gx = G * dx, gy = G * dy;
-            const float mx = dL_dweight * (-gx * a.z - gy * a.w);
-            const float my = dL_dweight * (-gy * b.x - gx * a.w);
-            val[4] = -mx;
-            val[5] = -my;
-            val[6] = __builtin_sqrtf(mx * mx + my * my);
-            val[7] = -0.5f * dL_dweight * gx * dx;
-            val[8] = -1.0f * dL_dweight * gx * dy;
-            val[9] = -0.5f * dL_dweight * gy * dy;
-            val[10] = 1.f;
-          }
+      }
+      if (contrib) {
+#pragma clang fp contract(fast)
+        const float c2 = s_c[j];
+        const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+        T = T * inv;
+        const float weight = alpha * T;
+        val[0] = weight * dL0;
+        val[1] = weight * dL1;
+        val[2] = weight * dL2;
+        suf0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
+        suf1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
+        suf2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
+        lc0 = b.z;
+        lc1 = b.w;
+        lc2 = c2;
+        float dL_dalpha = ((b.z - suf0) * dL0 + (b.w - suf1) * dL1 + (c2 - suf2) * dL2) * T;
+        last_alpha = alpha;
+        dL_dalpha -= T_final * inv * bg_term;
+        if (raw < kMaxAlpha) {
+          val[3] = G * dL_dalpha;
+          const float dL_dweight = b.y * dL_dalpha;
+          const float gx = G * dx, gy = G * dy;
+          const float mx = dL_dweight * (-gx * a.z - gy * a.w);
+          const float my = dL_dweight * (-gy * b.x - gx * a.w);
+          val[4] = -mx;
+          val[5] = -my;
+          val[6] = __builtin_sqrtf(mx * mx + my * my);
+          val[7] = -0.5f * dL_dweight * gx * dx;
+          val[8] = -dL_dweight * gx * dy;
+          val[9] = -0.5f * dL_dweight * gy * dy;
+          val[10] = 1.f;
         }
       }
-      if (__ballot(contrib) != 0ull) {
-#pragma unroll
-        for (int k = 0; k < kVals; ++k) val[k] = wave_sum(val[k]);
-      }
-      if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < kVals; ++k) s_part[wave][j][k] = val[k];
-      }
+      float z = 0.f;
+      if (__ballot(contrib) != 0ull) z = butterfly16(val, lane);
+      if (writer) s_part[wave][j][my_slot] = z;
     }
     __syncthreads();
     if (tid < cnt) {

@@ -9,13 +9,13 @@
 // pairs by tile id. Exact depth ties keep slot order, which is gaussian index
 // order within a view -- the checker's stable_sort gives the same order.
 //
-// The radix sort is a classic three-kernel LSD sort with 8-bit digits over
+// The radix sort is a three-kernel LSD sort with 8-bit digits over
 // 4096-element blocks: per-block digit histograms (per-wave LDS copies to
 // limit atomic contention), an exclusive scan of the digit-major histogram
 // table, and a stable scatter whose in-block ranks come from wave64 ballots
 // (one ballot per digit bit selects the lanes sharing a digit, and a popcount
-// below the lane gives the rank). All global loads and stores of keys and
-// values are coalesced; the scatter's stores are grouped by digit.
+// below the lane gives the rank). The scatter sorts each block by digit in
+// LDS first, so its global stores go out as contiguous per-digit runs.
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -104,56 +104,105 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
   hist[(long long)tid * nblocks + blockIdx.x] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
 }
 
+// Stable scatter of one 4096-item block for one digit. Each wave ranks its
+// own contiguous 1024 items with no block barriers: per round of 64, the
+// lanes sharing a digit are found with one ballot per digit bit, and a
+// wave-private LDS counter per digit carries the count across rounds. Then
+// per-digit wave offsets and block digit starts place every item at its
+// stable position in an LDS copy of the block, sorted by digit; the block is
+// written out from LDS so each digit's run goes to global memory contiguously.
+constexpr int kRsWaveItems = kScanTile / 4;  // 1024 items per wave
+constexpr int kRsRounds = kRsWaveItems / 64;  // 16
+
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ vals,
                                                        uint32_t* __restrict__ keys_out,
                                                        uint32_t* __restrict__ vals_out, long long n, int shift,
                                                        int bits, const uint32_t* __restrict__ hist, int nblocks) {
-  __shared__ uint32_t s_base[256];
-  __shared__ uint32_t s_cnt[4][256];
-  const int tid = threadIdx.x, wave = tid >> 6;
-  s_base[tid] = hist[(long long)tid * nblocks + blockIdx.x];
+  __shared__ uint32_t s_cnt[4][256];   // per-wave digit counts, then per-wave digit offsets
+  __shared__ uint32_t s_start[256];    // block-local start of each digit
+  __shared__ uint32_t s_global[256];   // global start of each digit for this block
+  __shared__ uint32_t s_keys[kScanTile];
+  __shared__ uint32_t s_vals[kScanTile];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t mask = (1u << bits) - 1u;
   const long long base = (long long)blockIdx.x * kScanTile;
-  const uint64_t below = __lanemask_lt();
-  for (int r = 0; r < kScanItems; ++r) {
-    const long long j = base + (long long)r * 256 + tid;
-    const bool valid = j < n;
-    const uint32_t k = valid ? keys[j] : 0u;
-    const uint32_t vv = valid ? vals[j] : 0u;
-    const uint32_t d = (k >> shift) & mask;
+  const long long wbase = base + (long long)wave * kRsWaveItems;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) s_cnt[w][tid] = 0;
-    __syncthreads();
+  for (int w = 0; w < 4; ++w) s_cnt[w][tid] = 0;
+  s_global[tid] = hist[(long long)tid * nblocks + blockIdx.x];
+  __syncthreads();
+  const uint64_t below = __lanemask_lt();
+  uint32_t k[kRsRounds], v[kRsRounds], rank[kRsRounds];
+#pragma unroll
+  for (int r = 0; r < kRsRounds; ++r) {
+    const long long j = wbase + (long long)r * 64 + lane;
+    const bool valid = j < n;
+    k[r] = valid ? keys[j] : 0xFFFFFFFFu;
+    v[r] = valid ? vals[j] : 0u;
+    const uint32_t d = (k[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(valid && bit);
       peers &= bit ? bb : ~bb;
     }
-    const uint32_t rank = (uint32_t)__popcll(peers & below);
-    if (valid && rank == 0) s_cnt[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      uint32_t run = s_base[tid];
+    const uint32_t before = s_cnt[wave][d];
+    rank[r] = before + (uint32_t)__popcll(peers & below);
+    if (valid && (peers & below) == 0ull) s_cnt[wave][d] = before + (uint32_t)__popcll(peers);
+    if (!valid) rank[r] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  // per digit: offsets of each wave inside the digit, and the digit's total
+  uint32_t total = 0;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        const uint32_t c = s_cnt[w][tid];
-        s_cnt[w][tid] = run;
-        run += c;
-      }
-      s_base[tid] = run;
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t c = s_cnt[w][tid];
+    s_cnt[w][tid] = total;
+    total += c;
+  }
+  // block-wide exclusive scan of the digit totals (thread = digit)
+  {
+    uint32_t x = total;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
     }
+    __shared__ uint32_t s_wsum[4];
+    if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
-    if (valid) {
-      const uint32_t pos = s_cnt[wave][d] + rank;
-      keys_out[pos] = k;
-      vals_out[pos] = vv;
-    }
-    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < wave; ++w) add += s_wsum[w];
+    s_start[tid] = x - total + add;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRsRounds; ++r) {
+    if (rank[r] == 0xFFFFFFFFu) continue;
+    const uint32_t d = (k[r] >> shift) & mask;
+    const uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
+    s_keys[pos] = k[r];
+    s_vals[pos] = v[r];
+  }
+  __syncthreads();
+  const long long remaining = n - base;
+  const int count = remaining < kScanTile ? (int)remaining : kScanTile;
+  for (int p = tid; p < count; p += 256) {
+    const uint32_t kk = s_keys[p];
+    const uint32_t d = (kk >> shift) & mask;
+    const uint32_t dst = s_global[d] + (uint32_t)p - s_start[d];
+    keys_out[dst] = kk;
+    vals_out[dst] = s_vals[p];
   }
 }
 
+// Pair emission, one wave per 64 consecutive depth-sorted splats. The
+// wave's pairs are one contiguous range of the output (the offsets are an
+// exclusive scan in the same order), so the lanes stride over that range
+// together -- coalesced stores, balanced however uneven the rects are -- and
+// each lane finds the splat owning its pair by a binary search over the
+// wave's 64 offsets.
 __global__ __launch_bounds__(256) void k_emit_pairs(const uint32_t* __restrict__ sorted_slots,
                                                     const uint32_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ counts,
@@ -163,29 +212,64 @@ __global__ __launch_bounds__(256) void k_emit_pairs(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ pair_vals,
                                                     uint32_t* __restrict__ pair_base) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= m) return;
-  const uint32_t slot = sorted_slots[s];
-  const uint32_t c = counts[slot];
-  if (c == 0) return;
-  const uint32_t off = offsets[s];
-  const int v = (int)(slot / per_view);
-  const SplatRec& r = rec[slot];
-  const int cols = views[v].cols, tile_base = views[v].tile_base;
-  pair_base[slot] = off;
-  uint32_t k = off;
-  for (int ty = r.y0; ty <= r.y1; ++ty)
-    for (int tx = r.x0; tx <= r.x1; ++tx) {
-      pair_keys[k] = (uint32_t)(tile_base + ty * cols + tx);
-      pair_vals[k] = slot;
-      ++k;
+  const int lane = threadIdx.x & 63;
+  uint32_t slot = 0, c = 0, off = 0xFFFFFFFFu;
+  int x0 = 0, w = 1, y0 = 0, cols = 0, tile_base = 0;
+  if (s < m) {
+    slot = sorted_slots[s];
+    c = counts[slot];
+    off = offsets[s];
+    if (c) {
+      const SplatRec& r = rec[slot];
+      const int v = (int)(slot / per_view);
+      x0 = r.x0;
+      y0 = r.y0;
+      w = r.x1 - r.x0 + 1;
+      cols = views[v].cols;
+      tile_base = views[v].tile_base;
+      pair_base[slot] = off;
     }
+  }
+  const uint32_t first = __shfl(off, 0, 64);
+  if (first == 0xFFFFFFFFu) return;  // the whole wave is past the end
+  // end of the wave's range: the last valid lane's offset + count
+  const uint64_t valid = __ballot(s < m);
+  const int last_lane = 63 - __builtin_clzll(valid);
+  const uint32_t stop = __shfl(off + c, last_lane, 64);
+  // Every lane runs every iteration (the trip count is wave-uniform), so all
+  // lanes are active for the shuffles; only the stores are predicated.
+  for (uint32_t chunk = first; chunk < stop; chunk += 64) {
+    const uint32_t i = chunk + (uint32_t)lane;
+    int owner = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const int cand = owner + step;
+      const uint32_t oc = __shfl(off, cand & 63, 64);
+      if (cand <= last_lane && oc <= i) owner = cand;
+    }
+    const uint32_t o_off = __shfl(off, owner, 64);
+    const int ow = __shfl(w, owner, 64);
+    const int oy0 = __shfl(y0, owner, 64);
+    const int ox0 = __shfl(x0, owner, 64);
+    const int obase = __shfl(tile_base, owner, 64);
+    const int ocols = __shfl(cols, owner, 64);
+    const uint32_t oslot = __shfl(slot, owner, 64);
+    if (i < stop) {
+      const uint32_t local = i - o_off;
+      const int ty = oy0 + (int)(local / (uint32_t)ow);
+      const int tx = ox0 + (int)(local % (uint32_t)ow);
+      pair_keys[i] = (uint32_t)(obase + ty * ocols + tx);
+      pair_vals[i] = oslot;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys, long long p,
-                                                     uint32_t* __restrict__ ranges) {
+                                                     uint32_t* __restrict__ ranges, int total_tiles) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p) return;
   const uint32_t k = keys[i];
+  if (k >= (uint32_t)total_tiles) return;  // cannot happen for keys the emitter wrote
   if (i == 0 || keys[i - 1] != k) ranges[2 * (long long)k] = (uint32_t)i;
   if (i == p - 1 || keys[i + 1] != k) ranges[2 * (long long)k + 1] = (uint32_t)(i + 1);
 }
@@ -268,7 +352,7 @@ hipError_t launch_tile_ranges(const uint32_t* sorted_keys, long long p, uint32_t
   SPLAT_HIP_CHECK(launch_fill_u32(ranges, 0u, 2LL * total_tiles, stream));
   if (p <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, stream, sorted_keys, p,
-                     ranges);
+                     ranges, total_tiles);
   return hipGetLastError();
 }
 
