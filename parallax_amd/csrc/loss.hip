@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_loss_forward(LossParams p, const ViewDe
   __shared__ float s_src[5][kHalo][kHalo];
   __shared__ float s_h[5][kHalo][kTile];
   __shared__ float s_red[2][4];
-  const int tile = blockIdx.x;
+  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
   const int local = tile - V.tile_base;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_loss_backward(LossParams p, const ViewD
                                                        float* __restrict__ dL) {
   __shared__ float s_src[5][kHalo][kHalo];
   __shared__ float s_h[5][kHalo][kTile];
-  const int tile = blockIdx.x;
+  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
   const int local = tile - V.tile_base;

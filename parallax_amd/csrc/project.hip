@@ -53,6 +53,34 @@ __device__ __forceinline__ int sh_basis(int degree, float x, float y, float z, f
   return 15;
 }
 
+// Half extents of a box that holds every pixel centre where the compositing
+// loop can find alpha >= min_alpha for this splat, so a workgroup whose pixels
+// lie outside the box may skip the splat without changing any result.
+//
+// alpha >= min_alpha needs o * exp(power) >= min_alpha, power = -Q(d)/2 with
+// Q(d) = a dx^2 + 2 b dx dy + c dy^2 (the conic), i.e. Q <= q = 2 ln(o/min_alpha).
+// The box of that ellipse is |dx| <= sqrt(q c / det), |dy| <= sqrt(q a / det),
+// det = ac - b^2. The loop evaluates Q in float: each term carries at most
+// ~3e-7 relative error, and a dx^2 + c dy^2 <= Q * 2ac/det, so the float Q is
+// within 1.2e-6 * (ac/det) * Q of the exact one. q is therefore raised by 1e-5
+// (the exponential's and the product's rounding), the box widened by
+// 1/(1 - 4e-6 ac/det) in q and by 1e-3 px, and a splat too elongated for the
+// bound to be meaningful gets no box (+inf). Evaluated in double.
+__device__ __forceinline__ void cull_extents(float a, float b, float c, float opacity, float min_alpha, float* ex,
+                                             float* ey) {
+  const double A = a, B = b, Cc = c;
+  const double det = A * Cc - B * B;
+  *ex = *ey = __builtin_inff();
+  if (!(det > 0.0) || !(min_alpha > 0.f)) return;
+  const double kappa = A * Cc / det;
+  const double shrink = 1.0 - 4e-6 * kappa;
+  if (!(shrink > 0.25)) return;
+  double q = 2.0 * (log((double)opacity / (double)min_alpha) + 1e-5);
+  if (q < 0.0) q = 0.0;
+  *ex = (float)(sqrt(q * Cc / (det * shrink)) * (1.0 + 1e-6) + 1e-3);
+  *ey = (float)(sqrt(q * A / (det * shrink)) * (1.0 + 1e-6) + 1e-3);
+}
+
 template <int kDeg>
 __global__ __launch_bounds__(256) void k_project(const float* __restrict__ params, long long cap, int n,
                                                  int rest_stride, const ViewDesc* __restrict__ views,
@@ -159,8 +187,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
         if (opacity < min_alpha) ok = false;
         if (ok) {
           o.opacity = opacity;
-          o.cov_xx = cov00;
-          o.cov_yy = cov11;
+          cull_extents(o.ca, o.cb, o.cc, opacity, min_alpha, &o.ex, &o.ey);
           const float lx2 = V.thx * kSlack, ly2 = V.thy * kSlack;
           if (fabsf(((pc[0]) / (z))) > lx2) o.flags |= 8u;
           if (fabsf(((pc[1]) / (z))) > ly2) o.flags |= 16u;
@@ -213,7 +240,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
       o.depth = 0.f;
       o.radius = 0.f;
       o.x0 = o.x1 = o.y0 = o.y1 = 0;
-      o.cov_xx = o.cov_yy = 0.f;
+      o.ex = o.ey = 0.f;
     }
     rec[slot] = o;
     keys[slot] = key;

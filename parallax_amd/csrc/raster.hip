@@ -1,37 +1,49 @@
 // Forward and adjoint compositing over 16x16 tiles.
 //
-// One 256-thread workgroup per (view, 16x16 tile) of the whole batch, one
-// thread per pixel; all views of a batch go through one launch. Each
-// workgroup walks its tile's near-to-far list in batches of 256 splats staged
-// in LDS as packed float4 records (one coalesced record fetch per splat per
-// tile, broadcast LDS reads per pixel).
+// One 256-thread workgroup per (view, 16x16 tile) of the whole batch; each of
+// its four waves owns one 8x8 quadrant of the tile, one lane per pixel. All
+// views of a batch go through one launch, and workgroups are mapped to tiles
+// so that each XCD works through one contiguous run of tiles.
+//
+// A wave skips every splat of its tile's list whose cull box (the box of the
+// ellipse where alpha >= min_alpha can hold, computed by the projection)
+// misses its quadrant's pixel centres. Such a splat is skipped by every pixel
+// of the quadrant anyway, so nothing changes but the work: about half of the
+// (wave, splat) steps of a 16x16 tile list fall away at 1080p.
 //
 // Forward: the engine's rasterizeSplats pixel loop (engine/native/cpp/src/
 // splat/SplatRaster.cpp:277-313) in IEEE float with no contraction and the
 // deterministic exponential, so every skip and termination decision -- and
 // therefore the image -- matches the CPU checker bit for bit. It also stores
-// the last contributor per pixel, so the adjoint needs no replay.
+// the last contributor per pixel, so the adjoint needs no replay. Each wave
+// streams its tile's list in chunks of 64 splats: one lane per splat fetches
+// the 48-byte record, tests the cull box, and the survivors are compacted in
+// order into the wave's own LDS slice (ballot + mbcnt). Waves never wait on
+// each other, and a wave stops as soon as all of its 64 pixels terminate.
 //
 // Adjoint: rasterizeSplatsBackward's pixel half (:346-433). Walks each
 // pixel's list back to front from its last contributor, recomputing the
 // transmittance by division (nothing per splat and pixel is stored), and
 // accumulates per splat dL/dcolour, dL/dopacity, dL/dmean, |dL/dmean| and
-// dL/dconic. Per-pixel terms are summed across the wave64 by a transposed
-// DPP butterfly (16 values in ~4 exchange levels, every lane ending with one
-// value's wave total), the four waves' sums are combined in LDS in a fixed
-// order, and
-// one workgroup result per (splat, tile) pair then goes out either as a
-// hardware float atomic (atomic mode) or as a plain store into that pair's
-// own slot for a later fixed-order per-gaussian sum (deterministic mode).
-// VALU-bound; the figure of merit is (pixel, splat) evaluations per second.
+// dL/dconic. The workgroup stages the list in batches of 128 records and
+// marks, per splat, the quadrants whose cull box it meets below that
+// quadrant's last contributor; each wave then visits only its marked splats,
+// in reverse order. Per-pixel terms are summed across the wave64 by a
+// transposed DPP butterfly (16 values in ~4 exchange levels, every lane
+// ending with one value's wave total), the marked waves' sums are combined in
+// LDS in a fixed order, and one workgroup result per (splat, tile) pair then
+// goes out either as a hardware float atomic (atomic mode) or as a plain
+// store into that pair's own slot for a later fixed-order per-gaussian sum
+// (deterministic mode). VALU-bound; the figure of merit is (pixel, splat)
+// evaluations per second.
 #include "splat_common.h"
 #include "splat_kernels.h"
 
 namespace splat {
 namespace {
 
-constexpr int kBatch = 256;   // forward: splats staged per LDS batch
-constexpr int kBatchB = 128;  // adjoint: smaller, so its per-wave partials fit 5 workgroups per CU
+constexpr int kChunk = 64;    // forward: splats fetched per wave step
+constexpr int kBatchB = 128;  // adjoint: splats staged per workgroup batch
 constexpr int kVals = 11;  // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3, touched
 
 __device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int tile) {
@@ -41,62 +53,103 @@ __device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int
   return v;
 }
 
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int lane_rank(uint64_t mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// The quadrant of a tile that wave `wave` owns, as pixel-centre bounds clipped
+// to the image. Returns false when the quadrant lies outside the image.
+struct Quad {
+  int x0, y0;
+  float bx0, bx1, by0, by1;
+};
+__device__ __forceinline__ bool quad_of(const ViewDesc& V, int tx, int ty, int wave, Quad* q) {
+  q->x0 = tx * kTile + 8 * (wave & 1);
+  q->y0 = ty * kTile + 8 * (wave >> 1);
+  if (q->x0 >= V.width || q->y0 >= V.height) return false;
+  q->bx0 = (float)q->x0 + 0.5f;
+  q->by0 = (float)q->y0 + 0.5f;
+  q->bx1 = (float)min(q->x0 + 7, V.width - 1) + 0.5f;
+  q->by1 = (float)min(q->y0 + 7, V.height - 1) + 0.5f;
+  return true;
+}
+__device__ __forceinline__ bool box_hits(float mx, float my, float ex, float ey, const Quad& q) {
+  return mx + ex >= q.bx0 && mx - ex <= q.bx1 && my + ey >= q.by0 && my - ey <= q.by1;
+}
+
 __global__ __launch_bounds__(256) void k_forward(RasterParams p, const ViewDesc* __restrict__ views,
                                                  const uint32_t* __restrict__ ranges,
                                                  const uint32_t* __restrict__ pair_vals,
                                                  const SplatRec* __restrict__ rec, float* __restrict__ rgb,
                                                  float* __restrict__ T_out, int* __restrict__ last_out) {
-  __shared__ float4 s_a[kBatch];  // mx, my, ca, cb
-  __shared__ float4 s_b[kBatch];  // cc, opacity, r, g
-  __shared__ float s_c[kBatch];   // b
-  const int tile = blockIdx.x;
+  __shared__ float4 s_a[4][kChunk];  // mx, my, ca, cb
+  __shared__ float4 s_b[4][kChunk];  // cc, opacity, r, g
+  __shared__ float2 s_c[4][kChunk];  // b, list index (int bits)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
   const int local = tile - V.tile_base;
   const int tx = local % V.cols, ty = local / V.cols;
-  const int px = tx * kTile + (threadIdx.x & 15), py = ty * kTile + (threadIdx.x >> 4);
+  Quad q;
+  if (!quad_of(V, tx, ty, wave, &q)) return;  // no barrier in this kernel: waves are independent
+  const int px = q.x0 + (lane & 7), py = q.y0 + (lane >> 3);
   const bool inside = px < V.width && py < V.height;
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
   float T = 1.f, acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
   int last = 0;
   bool done = !inside;
-  for (uint32_t start = begin; start < end; start += kBatch) {
-    if (__syncthreads_count(done) == (int)blockDim.x) break;
-    const uint32_t e = start + threadIdx.x;
+  for (uint32_t base = begin; base < end; base += kChunk) {
+    if (__ballot(!done) == 0ull) break;
+    const uint32_t e = base + lane;
+    bool hit = false;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
     if (e < end) {
-      const SplatRec* r = rec + pair_vals[e];
-      const float4 a = *reinterpret_cast<const float4*>(&r->mx);
-      const float4 b = *reinterpret_cast<const float4*>(&r->cc);
-      s_a[threadIdx.x] = a;
-      s_b[threadIdx.x] = b;
-      s_c[threadIdx.x] = r->cbl;
+      const float4* r = reinterpret_cast<const float4*>(rec + pair_vals[e]);
+      a = r[0];
+      b = r[1];
+      c = r[2];
+      hit = box_hits(a.x, a.y, c.y, c.z, q);
     }
-    __syncthreads();
-    const int cnt = (int)min((uint32_t)kBatch, end - start);
+    const uint64_t m = __ballot(hit);
+    if (hit) {
+      const int pos = lane_rank(m);
+      s_a[wave][pos] = a;
+      s_b[wave][pos] = b;
+      s_c[wave][pos] = make_float2(c.x, __int_as_float((int)(e - begin)));
+    }
+    wave_sync_lds();
+    const int cnt = __popcll(m);
     if (!done) {
       for (int j = 0; j < cnt; ++j) {
-        const float4 a = s_a[j];
-        const float4 b = s_b[j];
-        const float dx = fx - a.x, dy = fy - a.y;
-        const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+        const float4 sa = s_a[wave][j];
+        const float4 sb = s_b[wave][j];
+        const float dx = fx - sa.x, dy = fy - sa.y;
+        const float power = -0.5f * (sa.z * dx * dx + sb.x * dy * dy) - sa.w * dx * dy;
         if (power > 0.f) continue;
-        const float raw = b.y * det_exp(power);
+        const float raw = sb.y * det_exp_nonpos(power);
         const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
         if (alpha < p.min_alpha) continue;
+        const float2 sc = s_c[wave][j];
         const float w = T * alpha;
-        acc0 += w * b.z;
-        acc1 += w * b.w;
-        acc2 += w * s_c[j];
+        acc0 += w * sb.z;
+        acc1 += w * sb.w;
+        acc2 += w * sc.x;
         T *= 1.f - alpha;
-        last = (int)(start + j + 1 - begin);
+        last = __float_as_int(sc.y) + 1;
         if (T < p.cutoff) {
           done = true;
           break;
         }
       }
     }
-    __syncthreads();
+    wave_sync_lds();
   }
   if (inside) {
     const long long pixel = V.pixel_base + (long long)py * V.width + px;
@@ -173,17 +226,23 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
   __shared__ float4 s_b[kBatchB];
   __shared__ float s_c[kBatchB];
   __shared__ uint32_t s_dst[kBatchB];          // slot (atomic) or pair id (deterministic)
-  __shared__ float s_part[4][kBatchB][kVals];  // per-wave sums
-  __shared__ int s_maxlast;
+  __shared__ uint32_t s_mask[kBatchB];         // quadrants that visit the splat
+  __shared__ float s_part[4][kBatchB][kVals];  // per-wave sums (valid where s_mask says)
+  __shared__ int s_stop[4];                    // per-quadrant last contributor
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int my_slot = butterfly_slot(lane);
   const bool writer = lane < 16 && my_slot < kVals;
-  const int tile = blockIdx.x;
+  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
   const int local = tile - V.tile_base;
   const int tx = local % V.cols, ty = local / V.cols;
-  const int px = tx * kTile + (tid & 15), py = ty * kTile + (tid >> 4);
+  // the four quadrants' cull bounds, for the staging threads
+  Quad quads[4];
+  bool qvalid[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) qvalid[k] = quad_of(V, tx, ty, k, &quads[k]);
+  const int px = tx * kTile + 8 * (wave & 1) + (lane & 7), py = ty * kTile + 8 * (wave >> 1) + (lane >> 3);
   const bool inside = px < V.width && py < V.height;
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
@@ -197,11 +256,12 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
     T_final = T_in[pixel];
     lastp = last_in[pixel];
   }
-  if (tid == 0) s_maxlast = 0;
+  if (tid < 4) s_stop[tid] = 0;
   __syncthreads();
-  if (lastp > 0) atomicMax(&s_maxlast, lastp);
+  if (lastp > 0) atomicMax(&s_stop[wave], lastp);
   __syncthreads();
-  const uint32_t stop = begin + (uint32_t)s_maxlast;
+  const int stops[4] = {s_stop[0], s_stop[1], s_stop[2], s_stop[3]};
+  const uint32_t stop = begin + (uint32_t)max(max(stops[0], stops[1]), max(stops[2], stops[3]));
   // Pairs past every pixel's last contributor get zero partials.
   if (deterministic) {
     for (uint32_t e = stop + tid; e < end; e += blockDim.x) {
@@ -226,86 +286,113 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
       const uint32_t e = lo + tid;
       const uint32_t slot = pair_vals[e];
       const SplatRec* r = rec + slot;
-      s_a[tid] = *reinterpret_cast<const float4*>(&r->mx);
-      s_b[tid] = *reinterpret_cast<const float4*>(&r->cc);
-      s_c[tid] = r->cbl;
+      const float4* r4 = reinterpret_cast<const float4*>(r);
+      const float4 a = r4[0], b = r4[1], c = r4[2];
+      s_a[tid] = a;
+      s_b[tid] = b;
+      s_c[tid] = c.x;
+      const int idx = (int)(e - begin);
+      uint32_t m = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (qvalid[k] && idx < stops[k] && box_hits(a.x, a.y, c.y, c.z, quads[k])) m |= 1u << k;
+      s_mask[tid] = m;
       if (deterministic)
         s_dst[tid] = pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0));
       else
         s_dst[tid] = slot;
     }
     __syncthreads();
-    for (int j = cnt - 1; j >= 0; --j) {
-      const uint32_t e = lo + j;
-      bool contrib = inside && (int)(e - begin) < lastp;
-      float val[16];
+    // this wave's splats of the batch, visited from the back
+    uint64_t mine[2];
+    mine[0] = __ballot(lane < cnt && ((s_mask[lane] >> wave) & 1u));
+    mine[1] = __ballot(lane + 64 < cnt && ((s_mask[lane + 64] >> wave) & 1u));
 #pragma unroll
-      for (int k = 0; k < 16; ++k) val[k] = 0.f;
-      const float4 a = s_a[j];
-      const float4 b = s_b[j];
-      // the forward's decisions, recomputed with its exact float operations
-      float dx = 0.f, dy = 0.f, G = 0.f, raw = 0.f, alpha = 0.f;
-      if (contrib) {
-        dx = fx - a.x;
-        dy = fy - a.y;
-        const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-        if (power > 0.f) {
-          contrib = false;
-        } else {
-          G = det_exp(power);
-          raw = b.y * G;
-          alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-          if (alpha < p.min_alpha) contrib = false;
+    for (int h = 1; h >= 0; --h) {
+      uint64_t bits = mine[h];
+      while (bits) {
+        const int bit = 63 - __builtin_clzll(bits);
+        bits &= ~(1ull << bit);
+        const int j = 64 * h + bit;
+        const uint32_t e = lo + j;
+        bool contrib = inside && (int)(e - begin) < lastp;
+        float val[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) val[k] = 0.f;
+        const float4 a = s_a[j];
+        const float4 b = s_b[j];
+        // the forward's decisions, recomputed with its exact float operations
+        float dx = 0.f, dy = 0.f, G = 0.f, raw = 0.f, alpha = 0.f;
+        if (contrib) {
+          dx = fx - a.x;
+          dy = fy - a.y;
+          const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+          if (power > 0.f) {
+            contrib = false;
+          } else {
+            G = det_exp_nonpos(power);
+            raw = b.y * G;
+            alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
+            if (alpha < p.min_alpha) contrib = false;
+          }
         }
-      }
-      if (contrib) {
+        if (contrib) {
 #pragma clang fp contract(fast)
-        const float c2 = s_c[j];
-        const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-        T = T * inv;
-        const float weight = alpha * T;
-        val[0] = weight * dL0;
-        val[1] = weight * dL1;
-        val[2] = weight * dL2;
-        suf0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
-        suf1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
-        suf2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
-        lc0 = b.z;
-        lc1 = b.w;
-        lc2 = c2;
-        float dL_dalpha = ((b.z - suf0) * dL0 + (b.w - suf1) * dL1 + (c2 - suf2) * dL2) * T;
-        last_alpha = alpha;
-        dL_dalpha -= T_final * inv * bg_term;
-        if (raw < kMaxAlpha) {
-          val[3] = G * dL_dalpha;
-          const float dL_dweight = b.y * dL_dalpha;
-          const float gx = G * dx, gy = G * dy;
-          const float mx = dL_dweight * (-gx * a.z - gy * a.w);
-          const float my = dL_dweight * (-gy * b.x - gx * a.w);
-          val[4] = -mx;
-          val[5] = -my;
-          val[6] = __builtin_sqrtf(mx * mx + my * my);
-          val[7] = -0.5f * dL_dweight * gx * dx;
-          val[8] = -dL_dweight * gx * dy;
-          val[9] = -0.5f * dL_dweight * gy * dy;
-          val[10] = 1.f;
+          const float c2 = s_c[j];
+          const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+          T = T * inv;
+          const float weight = alpha * T;
+          val[0] = weight * dL0;
+          val[1] = weight * dL1;
+          val[2] = weight * dL2;
+          suf0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
+          suf1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
+          suf2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
+          lc0 = b.z;
+          lc1 = b.w;
+          lc2 = c2;
+          float dL_dalpha = ((b.z - suf0) * dL0 + (b.w - suf1) * dL1 + (c2 - suf2) * dL2) * T;
+          last_alpha = alpha;
+          dL_dalpha -= T_final * inv * bg_term;
+          if (raw < kMaxAlpha) {
+            val[3] = G * dL_dalpha;
+            const float dL_dweight = b.y * dL_dalpha;
+            const float gx = G * dx, gy = G * dy;
+            const float mx = dL_dweight * (-gx * a.z - gy * a.w);
+            const float my = dL_dweight * (-gy * b.x - gx * a.w);
+            val[4] = -mx;
+            val[5] = -my;
+            val[6] = __builtin_sqrtf(mx * mx + my * my);
+            val[7] = -0.5f * dL_dweight * gx * dx;
+            val[8] = -dL_dweight * gx * dy;
+            val[9] = -0.5f * dL_dweight * gy * dy;
+            val[10] = 1.f;
+          }
         }
+        float z = 0.f;
+        if (__ballot(contrib) != 0ull) z = butterfly16(val, lane);
+        if (writer) s_part[wave][j][my_slot] = z;
       }
-      float z = 0.f;
-      if (__ballot(contrib) != 0ull) z = butterfly16(val, lane);
-      if (writer) s_part[wave][j][my_slot] = z;
     }
     __syncthreads();
     if (tid < cnt) {
+      const uint32_t m = s_mask[tid];
       float sum[kVals];
 #pragma unroll
-      for (int k = 0; k < kVals; ++k) sum[k] = ((s_part[0][tid][k] + s_part[1][tid][k]) + s_part[2][tid][k]) + s_part[3][tid][k];
+      for (int k = 0; k < kVals; ++k) sum[k] = 0.f;
+      // fixed order over the quadrants that visited the splat
+#pragma unroll
+      for (int w = 0; w < 4; ++w)
+        if ((m >> w) & 1u) {
+#pragma unroll
+          for (int k = 0; k < kVals; ++k) sum[k] += s_part[w][tid][k];
+        }
       if (deterministic) {
         float* dst = partial + (long long)s_dst[tid] * kScreenStride;
 #pragma unroll
         for (int k = 0; k < kVals; ++k) dst[k] = sum[k];
         dst[kVals] = 0.f;
-      } else {
+      } else if (m) {
         float* dst = screen + (long long)s_dst[tid] * kScreenStride;
 #pragma unroll
         for (int k = 0; k < kVals; ++k)

@@ -34,23 +34,37 @@ struct ViewDesc {
 
 // A projected splat of one (view, gaussian) slot; 64 bytes. Written by the
 // projection for every slot; `flags` bit 31 marks a projected (not culled)
-// splat.
+// splat. The compositing kernels read the first 48 bytes as three float4.
 struct alignas(16) SplatRec {
   float mx, my;        // mean in pixels
   float ca, cb, cc;    // conic (inverse 2D covariance: xx, xy, yy)
   float opacity;       // sigmoid(logit)
   float cr, cg, cbl;   // SH colour, clamped at zero
+  float ex, ey;        // half extents (pixels) of a box holding every pixel centre
+                       // where alpha >= min_alpha can hold; +inf: no bound
   float depth;
   float radius;        // 3 sigma of the major axis, pixels
   uint32_t flags;      // bits 0-2 colour clamped, bits 3-4 frustum clamped, bit 31 projected
   int16_t x0, x1, y0, y1;  // 16x16 tile rect, inclusive
-  float cov_xx, cov_yy;    // unused by the kernels; kept for diagnostics
 };
 static_assert(sizeof(SplatRec) == 64, "SplatRec must be 64 bytes");
 
 constexpr uint32_t kProjected = 0x80000000u;
 
 __host__ __device__ __forceinline__ float det_exp(float x) { return splat_det::exp(x); }
+// det_exp for the compositing loop's falloff, whose argument is <= 0.
+__host__ __device__ __forceinline__ float det_exp_nonpos(float x) { return splat_det::exp_nonpos(x); }
+
+// Workgroups go to the 8 XCDs of the device round-robin. Give each XCD one
+// contiguous run of the n work items (tiles), so items that share data --
+// neighbouring tiles share splats and image halos -- meet in the same L2.
+// A bijection of [0, n) for any n.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  constexpr int kXcds = 8;
+  const int xcd = b % kXcds, k = b / kXcds;
+  const int per = n / kXcds, extra = n % kXcds;
+  return xcd * per + min(xcd, extra) + k;
+}
 
 // Depth in an unsigned key that sorts like the float (depth >= 0.01 > 0).
 __device__ __forceinline__ uint32_t depth_key(float d) { return __float_as_uint(d); }
