@@ -1,0 +1,359 @@
+// Two-level tile binning: per-tile lists without a sort over (tile, splat)
+// pairs.
+//
+// The lists are the engine's buildTiles (engine/native/cpp/src/splat/
+// SplatRaster.cpp:114-151): every projected splat enters each 16x16 tile of its
+// rect, and each tile's list runs near to far (the depth order of :260-261,
+// exact ties in gaussian index order).
+//
+// At 1080p a splat covers ~14 tiles, so sorting the (tile, splat) pairs moves
+// ~14 times more data than there are splats. Here only (super-tile, splat)
+// pairs are sorted -- a super-tile is 8x8 tiles, and most splats touch one or
+// two -- and the per-tile lists are then cut out of each super-tile's
+// depth-ordered list directly:
+//   1. emit:    for every depth-sorted slot, one (super-tile id, slot) pair per
+//               super-tile its rect touches (stable radix sort by id next);
+//   2. segment: each super-tile's list is cut into segments of 1024 splats;
+//   3. count:   one wave per segment builds, per splat, the 64-bit mask of the
+//               super-tile's tiles it covers (a row mask times a column mask)
+//               and counts, per tile, the segment's splats (ballot + popcount);
+//   4. scan:    an exclusive scan over the (super-tile, tile, segment) table
+//               gives every (tile, segment) run its place, and the tile ranges;
+//   5. write:   the same wave pass again, writing each splat into each covered
+//               tile's run at its rank among the lanes (ballot + mbcnt), in
+//               depth order.
+// Every tile's list thus lands contiguously and in the order of the super-tile
+// list, which is depth order; pair memory is 4 bytes per (tile, splat).
+#include "splat_common.h"
+#include "splat_kernels.h"
+
+namespace splat {
+namespace {
+
+__device__ __forceinline__ int find_view_st(const ViewDesc* views, int n_views, int st) {
+  int v = 0;
+  for (int k = 1; k < n_views; ++k)
+    if (views[k].st_base <= st) v = k;
+  return v;
+}
+
+// Super-tile pair emission, one wave per 64 consecutive depth-sorted slots.
+// The wave's pairs are one contiguous range of the output (soffsets is an
+// exclusive scan in the same order), so the lanes stride over that range
+// together and each lane finds the slot owning its pair by a binary search
+// over the wave's 64 offsets. Also records each slot's first tile-pair index
+// (pair_base), which the deterministic adjoint uses as the pair id base.
+__global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__ sorted_slots,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ soffsets,
+                                                    const uint32_t* __restrict__ scounts,
+                                                    const SplatRec* __restrict__ rec,
+                                                    const ViewDesc* __restrict__ views, long long per_view,
+                                                    long long m, uint32_t* __restrict__ skeys,
+                                                    uint32_t* __restrict__ svals, uint32_t* __restrict__ pair_base) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t slot = 0, c = 0, off = 0xFFFFFFFFu;
+  int sx0 = 0, w = 1, sy0 = 0, scols = 0, st_base = 0;
+  if (s < m) {
+    slot = sorted_slots[s];
+    c = scounts[slot];
+    off = soffsets[s];
+    if (c) {
+      const SplatRec& r = rec[slot];
+      const int v = (int)(slot / per_view);
+      sx0 = r.x0 / kSuper;
+      sy0 = r.y0 / kSuper;
+      w = r.x1 / kSuper - sx0 + 1;
+      scols = views[v].scols;
+      st_base = views[v].st_base;
+      pair_base[slot] = offsets[s];
+    }
+  }
+  const uint32_t first = __shfl(off, 0, 64);
+  if (first == 0xFFFFFFFFu) return;  // the whole wave is past the end
+  const uint64_t valid = __ballot(s < m);
+  const int last_lane = 63 - __builtin_clzll(valid);
+  const uint32_t stop = __shfl(off + c, last_lane, 64);
+  // The trip count is wave-uniform, so all lanes take part in every shuffle;
+  // only the stores are predicated.
+  for (uint32_t chunk = first; chunk < stop; chunk += 64) {
+    const uint32_t i = chunk + (uint32_t)lane;
+    int owner = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const int cand = owner + step;
+      const uint32_t oc = __shfl(off, cand & 63, 64);
+      if (cand <= last_lane && oc <= i) owner = cand;
+    }
+    const uint32_t o_off = __shfl(off, owner, 64);
+    const int ow = __shfl(w, owner, 64);
+    const int osy0 = __shfl(sy0, owner, 64);
+    const int osx0 = __shfl(sx0, owner, 64);
+    const int obase = __shfl(st_base, owner, 64);
+    const int ocols = __shfl(scols, owner, 64);
+    const uint32_t oslot = __shfl(slot, owner, 64);
+    if (i < stop) {
+      const uint32_t local = i - o_off;
+      const int sy = osy0 + (int)(local / (uint32_t)ow);
+      const int sx = osx0 + (int)(local % (uint32_t)ow);
+      skeys[i] = (uint32_t)(obase + sy * ocols + sx);
+      svals[i] = oslot;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_super_ranges(const uint32_t* __restrict__ keys, long long n,
+                                                      uint32_t* __restrict__ ranges, int n_st) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  if (k >= (uint32_t)n_st) return;  // cannot happen for keys the emitter wrote
+  if (i == 0 || keys[i - 1] != k) ranges[2 * (long long)k] = (uint32_t)i;
+  if (i == n - 1 || keys[i + 1] != k) ranges[2 * (long long)k + 1] = (uint32_t)(i + 1);
+}
+
+__global__ __launch_bounds__(256) void k_segment_counts(const uint32_t* __restrict__ sranges, int n_st,
+                                                        uint32_t* __restrict__ nseg) {
+  const int st = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st >= n_st) return;
+  const uint32_t len = sranges[2 * st + 1] - sranges[2 * st];
+  nseg[st] = (len + kBinSegment - 1) / kBinSegment;
+}
+
+__global__ __launch_bounds__(256) void k_segment_list(const uint32_t* __restrict__ nseg,
+                                                      const uint32_t* __restrict__ seg_base, int n_st,
+                                                      uint32_t* __restrict__ seg_st) {
+  const int st = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st >= n_st) return;
+  const uint32_t b = seg_base[st], c = nseg[st];
+  for (uint32_t s = 0; s < c; ++s) seg_st[b + s] = (uint32_t)st;
+}
+
+// One segment as seen by one wave: which super-tile, which slice of its list,
+// and where its tiles start.
+struct Segment {
+  uint32_t st, s, nsg, b0, b1;
+  uint32_t table0;  // 64 * seg_base[st]
+  int ox, oy;       // first tile (in the view's tile grid) of the super-tile
+};
+
+__device__ __forceinline__ Segment segment_of(uint32_t g, const ViewDesc* views, int n_views,
+                                              const uint32_t* sranges, const uint32_t* nseg,
+                                              const uint32_t* seg_base, const uint32_t* seg_st) {
+  Segment S;
+  S.st = seg_st[g];
+  S.s = g - seg_base[S.st];
+  S.nsg = nseg[S.st];
+  S.table0 = 64u * seg_base[S.st];
+  const uint32_t b = sranges[2 * S.st], e = sranges[2 * S.st + 1];
+  S.b0 = b + S.s * kBinSegment;
+  S.b1 = min(S.b0 + (uint32_t)kBinSegment, e);
+  const int v = find_view_st(views, n_views, (int)S.st);
+  const ViewDesc& V = views[v];
+  const int local = (int)S.st - V.st_base;
+  S.ox = (local % V.scols) * kSuper;
+  S.oy = (local / V.scols) * kSuper;
+  return S;
+}
+
+// Bit (ly * 8 + lx) set for every tile (lx, ly) of the super-tile that the
+// splat's rect covers.
+__device__ __forceinline__ uint64_t cover_mask(const SplatRec* rec, uint32_t slot, int ox, int oy) {
+  const int2 packed = *reinterpret_cast<const int2*>(&rec[slot].x0);  // x0, x1, y0, y1 as int16
+  const int x0 = (int)(int16_t)(packed.x & 0xFFFF), x1 = (int)(int16_t)(packed.x >> 16);
+  const int y0 = (int)(int16_t)(packed.y & 0xFFFF), y1 = (int)(int16_t)(packed.y >> 16);
+  const int lx0 = max(x0 - ox, 0), lx1 = min(x1 - ox, kSuper - 1);
+  const int ly0 = max(y0 - oy, 0), ly1 = min(y1 - oy, kSuper - 1);
+  if (lx0 > lx1 || ly0 > ly1) return 0ull;
+  const uint64_t all = 0x0101010101010101ull;
+  const uint64_t rows = (all << (8 * ly0)) & (all >> (8 * (kSuper - 1 - ly1)));
+  const uint64_t cols = (uint64_t)(((2u << lx1) - 1u) & ~((1u << lx0) - 1u));
+  return rows * cols;
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t m) {
+  uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    lo |= (uint32_t)__shfl_xor((int)lo, d, 64);
+    hi |= (uint32_t)__shfl_xor((int)hi, d, 64);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Per (tile, segment) splat counts into the table at
+// 64 * seg_base[st] + tile * nseg[st] + s; entries of segments past the last
+// real one (up to ns_max) are zeroed. One wave per segment, grid-stride.
+__global__ __launch_bounds__(256) void k_tile_count(const ViewDesc* __restrict__ views, int n_views,
+                                                    const uint32_t* __restrict__ svals,
+                                                    const uint32_t* __restrict__ sranges,
+                                                    const uint32_t* __restrict__ nseg,
+                                                    const uint32_t* __restrict__ seg_base,
+                                                    const uint32_t* __restrict__ seg_st,
+                                                    const uint32_t* __restrict__ ns_total, uint32_t ns_max,
+                                                    const SplatRec* __restrict__ rec, uint32_t* __restrict__ table) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  const uint32_t ns = *ns_total;
+  for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < ns_max; g += waves) {
+    if (g >= ns) {
+      table[64ull * g + lane] = 0u;
+      continue;
+    }
+    const Segment S = segment_of(g, views, n_views, sranges, nseg, seg_base, seg_st);
+    uint32_t cnt = 0;  // lane t: splats of this segment covering tile t
+    for (uint32_t c = S.b0; c < S.b1; c += 64) {
+      const uint32_t i = c + (uint32_t)lane;
+      const uint64_t mask = i < S.b1 ? cover_mask(rec, svals[i], S.ox, S.oy) : 0ull;
+      uint64_t u = wave_or(mask);
+      while (u) {
+        const int t = __builtin_ctzll(u);
+        u &= u - 1;
+        const uint64_t b = __ballot((mask >> t) & 1ull);
+        if (lane == t) cnt += (uint32_t)__popcll(b);
+      }
+    }
+    table[S.table0 + (uint64_t)lane * S.nsg + S.s] = cnt;
+  }
+}
+
+// Writes every splat of every segment into each covered tile's run. Same
+// traversal as k_tile_count; `scan` is the exclusive scan of its table.
+__global__ __launch_bounds__(256) void k_tile_write(const ViewDesc* __restrict__ views, int n_views,
+                                                    const uint32_t* __restrict__ svals,
+                                                    const uint32_t* __restrict__ sranges,
+                                                    const uint32_t* __restrict__ nseg,
+                                                    const uint32_t* __restrict__ seg_base,
+                                                    const uint32_t* __restrict__ seg_st,
+                                                    const uint32_t* __restrict__ ns_total,
+                                                    const SplatRec* __restrict__ rec,
+                                                    const uint32_t* __restrict__ scan,
+                                                    uint32_t* __restrict__ pair_vals) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  const uint32_t ns = *ns_total;
+  for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < ns; g += waves) {
+    const Segment S = segment_of(g, views, n_views, sranges, nseg, seg_base, seg_st);
+    uint32_t base = scan[S.table0 + (uint64_t)lane * S.nsg + S.s];  // lane t: next slot of tile t
+    for (uint32_t c = S.b0; c < S.b1; c += 64) {
+      const uint32_t i = c + (uint32_t)lane;
+      uint32_t slot = 0;
+      uint64_t mask = 0ull;
+      if (i < S.b1) {
+        slot = svals[i];
+        mask = cover_mask(rec, slot, S.ox, S.oy);
+      }
+      uint64_t u = wave_or(mask);
+      while (u) {
+        const int t = __builtin_ctzll(u);
+        u &= u - 1;
+        const bool mine = (mask >> t) & 1ull;
+        const uint64_t b = __ballot(mine);
+        const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
+        if (mine) pair_vals[at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] =
+            slot;
+        if (lane == t) base += (uint32_t)__popcll(b);
+      }
+    }
+  }
+}
+
+// [begin, end) of every global tile from the scanned table.
+__global__ __launch_bounds__(256) void k_tile_ranges_table(const ViewDesc* __restrict__ views, int n_views,
+                                                           int total_tiles, const uint32_t* __restrict__ scan,
+                                                           const uint32_t* __restrict__ nseg,
+                                                           const uint32_t* __restrict__ seg_base,
+                                                           uint32_t* __restrict__ ranges) {
+  const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= total_tiles) return;
+  int v = 0;
+  for (int k = 1; k < n_views; ++k)
+    if (views[k].tile_base <= tile) v = k;
+  const ViewDesc& V = views[v];
+  const int local = tile - V.tile_base;
+  const int tx = local % V.cols, ty = local / V.cols;
+  const int st = V.st_base + (ty / kSuper) * V.scols + tx / kSuper;
+  const int t = (ty % kSuper) * kSuper + tx % kSuper;
+  const uint64_t i0 = 64ull * seg_base[st] + (uint64_t)t * nseg[st];
+  ranges[2 * (long long)tile] = scan[i0];
+  ranges[2 * (long long)tile + 1] = scan[i0 + nseg[st]];
+}
+
+}  // namespace
+
+hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
+                             const uint32_t* scounts, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             long long m, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base, hipStream_t stream) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
+                     soffsets, scounts, rec, views, per_view, m, skeys, svals, pair_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, uint32_t* sranges, int n_st,
+                               hipStream_t stream) {
+  SPLAT_HIP_CHECK(launch_fill_u32(sranges, 0u, 2LL * n_st, stream));
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_super_ranges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sorted_keys, n,
+                     sranges, n_st);
+  return hipGetLastError();
+}
+
+hipError_t launch_segment_counts(const uint32_t* sranges, int n_st, uint32_t* nseg, hipStream_t stream) {
+  if (n_st <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_segment_counts, dim3((unsigned)((n_st + 255) / 256)), dim3(256), 0, stream, sranges, n_st,
+                     nseg);
+  return hipGetLastError();
+}
+
+hipError_t launch_segment_list(const uint32_t* nseg, const uint32_t* seg_base, int n_st, uint32_t* seg_st,
+                               hipStream_t stream) {
+  if (n_st <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_segment_list, dim3((unsigned)((n_st + 255) / 256)), dim3(256), 0, stream, nseg, seg_base,
+                     n_st, seg_st);
+  return hipGetLastError();
+}
+
+namespace {
+unsigned segment_grid(uint32_t ns_max) {
+  // four waves (segments) per workgroup; enough workgroups to fill the device
+  // several times over, grid-stride beyond that
+  const uint32_t want = (ns_max + 3) / 4;
+  return want < 1 ? 1u : (want > 8192u ? 8192u : want);
+}
+}  // namespace
+
+hipError_t launch_tile_count(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
+                             const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
+                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, uint32_t* table,
+                             hipStream_t stream) {
+  // the entry past the last segment's table block holds the grand total after the scan
+  SPLAT_HIP_CHECK(launch_fill_u32(table + 64ull * ns_max, 0u, 1, stream));
+  if (ns_max == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_count, dim3(segment_grid(ns_max)), dim3(256), 0, stream, views, n_views, svals, sranges,
+                     nseg, seg_base, seg_st, ns_total, ns_max, rec, table);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_write(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
+                             const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
+                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, const uint32_t* scan,
+                             uint32_t* pair_vals, hipStream_t stream) {
+  if (ns_max == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_write, dim3(segment_grid(ns_max)), dim3(256), 0, stream, views, n_views, svals, sranges,
+                     nseg, seg_base, seg_st, ns_total, rec, scan, pair_vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges_table(const ViewDesc* views, int n_views, int total_tiles, const uint32_t* scan,
+                                    const uint32_t* nseg, const uint32_t* seg_base, uint32_t* ranges,
+                                    hipStream_t stream) {
+  if (total_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_ranges_table, dim3((unsigned)((total_tiles + 255) / 256)), dim3(256), 0, stream, views,
+                     n_views, total_tiles, scan, nseg, seg_base, ranges);
+  return hipGetLastError();
+}
+
+}  // namespace splat

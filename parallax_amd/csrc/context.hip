@@ -57,6 +57,8 @@ ViewDesc make_desc(const splat_view& v) {
   d.height = v.height;
   d.cols = (v.width + kTile - 1) / kTile;
   d.rows = (v.height + kTile - 1) / kTile;
+  d.scols = (d.cols + kSuper - 1) / kSuper;
+  d.srows = (d.rows + kSuper - 1) / kSuper;
   return d;
 }
 
@@ -109,7 +111,8 @@ splat_context::~splat_context() {
 
 std::vector<DeviceBuffer*> splat_context::all_buffers() {
   return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
-          &vals_alt, &counts, &offsets, &pair_base, &pair_keys, &pair_vals, &pair_keys_alt, &pair_vals_alt, &ranges,
+          &vals_alt, &counts, &offsets, &pair_base, &scounts, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
+          &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
           &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &scan_ws, &hist_ws, &scalars,
           &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src};
 }
@@ -173,17 +176,20 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   ctx->opts = *o;
   ctx->batch_ids.assign(view_ids, view_ids + n_views);
   ctx->batch_desc.clear();
-  int tiles = 0;
+  int tiles = 0, supers = 0;
   long long px = 0;
   for (int k = 0; k < n_views; ++k) {
     ViewDesc d = make_desc(ctx->views[view_ids[k]]);
     d.tile_base = tiles;
+    d.st_base = supers;
     d.pixel_base = px;
     tiles += d.cols * d.rows;
+    supers += d.scols * d.srows;
     px += (long long)d.width * d.height;
     ctx->batch_desc.push_back(d);
   }
   ctx->total_tiles = tiles;
+  ctx->n_st = supers;
   ctx->total_px = px;
   ctx->sh_use = std::min(o->sh_degree, ctx->sh_degree);
   ctx->forward_done = false;
@@ -209,41 +215,73 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   CK(ctx->counts.reserve(4 * Mc), "alloc counts");
   CK(ctx->offsets.reserve(4 * Mc), "alloc offsets");
   CK(ctx->pair_base.reserve(4 * Mc), "alloc pair base");
+  CK(ctx->scounts.reserve(4 * Mc), "alloc counts");
+  CK(ctx->soffsets.reserve(4 * Mc), "alloc offsets");
+  CK(ctx->sranges.reserve(sizeof(uint32_t) * 2 * std::max(1, supers)), "alloc ranges");
+  CK(ctx->nseg.reserve(sizeof(uint32_t) * std::max(1, supers)), "alloc segments");
+  CK(ctx->seg_base.reserve(sizeof(uint32_t) * std::max(1, supers)), "alloc segments");
   CK(ctx->ensure_sort_workspace(std::max(M, 256LL * 4096)), "alloc sort workspace");
 
   TIMED("project", launch_project(ctx->params.f(), ctx->cap, (int)n, ctx->rs, ctx->sh_use, ctx->views_dev(), n_views,
                                   o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(), ctx->vals.u(),
-                                  ctx->counts.u(), ctx->stream));
+                                  ctx->counts.u(), ctx->scounts.u(), ctx->stream));
   bool alt = false;
   TIMED("depth_sort", launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
                                         ctx->sort_ws, &alt, ctx->stream));
   uint32_t* sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
+  // scalars: [0] tile pairs P, [1] super-tile pairs S, [2] segments
   uint32_t* d_total = ctx->scalars.u();
   TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), sorted_slots, ctx->offsets.u(), M, d_total,
                                            ctx->sort_ws.scan, ctx->stream));
-  uint32_t P = 0;
+  TIMED("pair_scan", launch_exclusive_scan(ctx->scounts.u(), sorted_slots, ctx->soffsets.u(), M, d_total + 1,
+                                           ctx->sort_ws.scan, ctx->stream));
+  uint32_t totals[2] = {0, 0};
   if (M > 0) {
-    CK(hipMemcpyAsync(&P, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "read pair count");
+    CK(hipMemcpyAsync(totals, d_total, sizeof(totals), hipMemcpyDeviceToHost, ctx->stream), "read pair count");
     CK(hipStreamSynchronize(ctx->stream), "sync");
   }
+  const uint32_t P = totals[0];
+  const uint32_t S = totals[1];
   ctx->P = P;
-  const long long Pc = std::max(1LL, (long long)P);
-  CK(ctx->pair_keys.reserve(4 * Pc), "alloc pairs");
+  ctx->S = S;
+  const long long Pc = std::max(1LL, (long long)P), Sc = std::max(1LL, (long long)S);
+  // segments: at most ceil(len / kBinSegment) per super-tile
+  const uint32_t ns_max = (uint32_t)(S / kBinSegment + (unsigned long long)supers);
   CK(ctx->pair_vals.reserve(4 * Pc), "alloc pairs");
-  CK(ctx->pair_keys_alt.reserve(4 * Pc), "alloc pairs");
-  CK(ctx->pair_vals_alt.reserve(4 * Pc), "alloc pairs");
-  CK(ctx->ensure_sort_workspace(std::max((long long)P, M)), "alloc sort workspace");
-  TIMED("emit_pairs", launch_emit_pairs(sorted_slots, ctx->offsets.u(), ctx->counts.u(), ctx->recs(), ctx->views_dev(),
-                                        n_views, n, M, ctx->pair_keys.u(), ctx->pair_vals.u(), ctx->pair_base.u(),
-                                        ctx->stream));
-  const int tile_bits = std::max(1, bit_width((unsigned long long)std::max(0, tiles - 1)));
-  bool palt = false;
-  TIMED("tile_sort", launch_radix_sort(ctx->pair_keys.u(), ctx->pair_vals.u(), ctx->pair_keys_alt.u(),
-                                       ctx->pair_vals_alt.u(), P, 0, tile_bits, ctx->sort_ws, &palt, ctx->stream));
-  ctx->sorted_pair_keys = palt ? ctx->pair_keys_alt.u() : ctx->pair_keys.u();
-  ctx->sorted_pair_vals = palt ? ctx->pair_vals_alt.u() : ctx->pair_vals.u();
+  CK(ctx->skeys.reserve(4 * Sc), "alloc super pairs");
+  CK(ctx->svals.reserve(4 * Sc), "alloc super pairs");
+  CK(ctx->skeys_alt.reserve(4 * Sc), "alloc super pairs");
+  CK(ctx->svals_alt.reserve(4 * Sc), "alloc super pairs");
+  CK(ctx->seg_st.reserve(4 * std::max(1u, ns_max)), "alloc segments");
+  CK(ctx->bin_table.reserve(4 * (64ull * ns_max + 1)), "alloc binning table");
+  CK(ctx->ensure_sort_workspace(std::max<long long>(std::max<long long>(S, M), 64LL * ns_max + 1)),
+     "alloc sort workspace");
+  TIMED("emit_super", launch_emit_super(sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), ctx->scounts.u(),
+                                        ctx->recs(), ctx->views_dev(), n, M, ctx->skeys.u(), ctx->svals.u(),
+                                        ctx->pair_base.u(), ctx->stream));
+  const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
+  bool salt = false;
+  TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(), S, 0,
+                                        st_bits, ctx->sort_ws, &salt, ctx->stream));
+  const uint32_t* skeys = salt ? ctx->skeys_alt.u() : ctx->skeys.u();
+  const uint32_t* svals = salt ? ctx->svals_alt.u() : ctx->svals.u();
+  TIMED("segments", launch_super_ranges(skeys, S, ctx->sranges.u(), supers, ctx->stream));
+  TIMED("segments", launch_segment_counts(ctx->sranges.u(), supers, ctx->nseg.u(), ctx->stream));
+  TIMED("segments", launch_exclusive_scan(ctx->nseg.u(), nullptr, ctx->seg_base.u(), supers, d_total + 2,
+                                          ctx->sort_ws.scan, ctx->stream));
+  TIMED("segments", launch_segment_list(ctx->nseg.u(), ctx->seg_base.u(), supers, ctx->seg_st.u(), ctx->stream));
+  TIMED("tile_count", launch_tile_count(ctx->views_dev(), n_views, svals, ctx->sranges.u(), ctx->nseg.u(),
+                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->recs(),
+                                        ctx->bin_table.u(), ctx->stream));
+  TIMED("tile_scan", launch_exclusive_scan(ctx->bin_table.u(), nullptr, ctx->bin_table.u(), 64LL * ns_max + 1,
+                                           nullptr, ctx->sort_ws.scan, ctx->stream));
+  TIMED("tile_write", launch_tile_write(ctx->views_dev(), n_views, svals, ctx->sranges.u(), ctx->nseg.u(),
+                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->recs(),
+                                        ctx->bin_table.u(), ctx->pair_vals.u(), ctx->stream));
+  TIMED("tile_ranges", launch_tile_ranges_table(ctx->views_dev(), n_views, tiles, ctx->bin_table.u(), ctx->nseg.u(),
+                                                ctx->seg_base.u(), ctx->ranges.u(), ctx->stream));
+  ctx->sorted_pair_vals = ctx->pair_vals.u();
   ctx->sorted_slots = sorted_slots;
-  TIMED("tile_ranges", launch_tile_ranges(ctx->sorted_pair_keys, P, ctx->ranges.u(), tiles, ctx->stream));
   RasterParams rp{};
   for (int c = 0; c < 3; ++c) rp.bg[c] = o->background[c];
   rp.min_alpha = o->min_alpha;

@@ -8,7 +8,7 @@
 // (:91-102) and tile rect (:121-126), in IEEE float with no contraction so that
 // culling, the radius and the tile rect -- the inputs of the binning -- match
 // the CPU checker bit for bit. HBM-bound: compulsory traffic is the parameter
-// read (56 + 4*rs bytes per gaussian) plus 76 bytes per (view, gaussian).
+// read (56 + 4*rs bytes per gaussian) plus 80 bytes per (view, gaussian).
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -86,7 +86,8 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
                                                  int rest_stride, const ViewDesc* __restrict__ views,
                                                  int n_views, float dilation, float min_alpha,
                                                  SplatRec* __restrict__ rec, uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
+                                                 uint32_t* __restrict__ vals, uint32_t* __restrict__ counts,
+                                                 uint32_t* __restrict__ scounts) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const float* P = params;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
     const long long slot = (long long)v * n + g;
     SplatRec o;
     o.flags = 0u;
-    uint32_t count = 0;
+    uint32_t count = 0, scount = 0;
     uint32_t key = 0xFFFFFFFFu;
     bool ok = true;
     float pc[3];
@@ -227,6 +228,8 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
           o.x0 = (int16_t)x0; o.x1 = (int16_t)x1; o.y0 = (int16_t)y0; o.y1 = (int16_t)y1;
           const int w = x1 - x0 + 1, h = y1 - y0 + 1;
           count = (w > 0 && h > 0) ? (uint32_t)(w * h) : 0u;
+          if (count)
+            scount = (uint32_t)((x1 / kSuper - x0 / kSuper + 1) * (y1 / kSuper - y0 / kSuper + 1));
           o.flags |= kProjected;
           key = depth_key(o.depth);
         }
@@ -246,6 +249,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
     keys[slot] = key;
     vals[slot] = (uint32_t)slot;
     counts[slot] = count;
+    scounts[slot] = scount;
   }
 }
 
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
 
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
-                          uint32_t* keys, uint32_t* vals, uint32_t* counts, hipStream_t stream) {
+                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, hipStream_t stream) {
   if (n <= 0 || n_views <= 0) return hipSuccess;
   const int blocks = (n + 255) / 256;
   // the cloud's coefficients must cover the degree in use
@@ -261,19 +265,19 @@ hipError_t launch_project(const float* params, long long cap, int n, int rest_st
   switch (sh_use) {
     case 0:
       hipLaunchKernelGGL(k_project<0>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
       break;
     case 1:
       hipLaunchKernelGGL(k_project<1>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
       break;
     case 2:
       hipLaunchKernelGGL(k_project<2>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
       break;
     default:
       hipLaunchKernelGGL(k_project<3>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
       break;
   }
   return hipGetLastError();

@@ -1,13 +1,10 @@
-// Device scan, stable LSD radix sort, and tile binning.
+// Device scan and stable LSD radix sort.
 //
-// Binning follows the engine's buildTiles (engine/native/cpp/src/splat/
-// SplatRaster.cpp:114-151): every projected splat is entered into each
-// 16x16 tile of its rect, and each tile's list runs near to far. The engine
-// sorts all projected splats by depth (:260-261) and then scatters them
-// stably into tiles. Here the same order comes from two stable sorts: the
-// (view, gaussian) slots by float depth bits, then the emitted (tile, slot)
-// pairs by tile id. Exact depth ties keep slot order, which is gaussian index
-// order within a view -- the checker's stable_sort gives the same order.
+// The binning uses two stable sorts: the (view, gaussian) slots by float depth
+// bits (the engine sorts its projected splats by depth, engine/native/cpp/src/
+// splat/SplatRaster.cpp:260-261; exact ties keep slot order, which is gaussian
+// index order within a view, as the checker's stable_sort), then the
+// (super-tile, slot) pairs by super-tile id.
 //
 // The radix sort is a three-kernel LSD sort with 8-bit digits over
 // 4096-element blocks: per-block digit histograms (per-wave LDS copies to
@@ -197,83 +194,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   }
 }
 
-// Pair emission, one wave per 64 consecutive depth-sorted splats. The
-// wave's pairs are one contiguous range of the output (the offsets are an
-// exclusive scan in the same order), so the lanes stride over that range
-// together -- coalesced stores, balanced however uneven the rects are -- and
-// each lane finds the splat owning its pair by a binary search over the
-// wave's 64 offsets.
-__global__ __launch_bounds__(256) void k_emit_pairs(const uint32_t* __restrict__ sorted_slots,
-                                                    const uint32_t* __restrict__ offsets,
-                                                    const uint32_t* __restrict__ counts,
-                                                    const SplatRec* __restrict__ rec,
-                                                    const ViewDesc* __restrict__ views, long long per_view,
-                                                    long long m, uint32_t* __restrict__ pair_keys,
-                                                    uint32_t* __restrict__ pair_vals,
-                                                    uint32_t* __restrict__ pair_base) {
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  uint32_t slot = 0, c = 0, off = 0xFFFFFFFFu;
-  int x0 = 0, w = 1, y0 = 0, cols = 0, tile_base = 0;
-  if (s < m) {
-    slot = sorted_slots[s];
-    c = counts[slot];
-    off = offsets[s];
-    if (c) {
-      const SplatRec& r = rec[slot];
-      const int v = (int)(slot / per_view);
-      x0 = r.x0;
-      y0 = r.y0;
-      w = r.x1 - r.x0 + 1;
-      cols = views[v].cols;
-      tile_base = views[v].tile_base;
-      pair_base[slot] = off;
-    }
-  }
-  const uint32_t first = __shfl(off, 0, 64);
-  if (first == 0xFFFFFFFFu) return;  // the whole wave is past the end
-  // end of the wave's range: the last valid lane's offset + count
-  const uint64_t valid = __ballot(s < m);
-  const int last_lane = 63 - __builtin_clzll(valid);
-  const uint32_t stop = __shfl(off + c, last_lane, 64);
-  // Every lane runs every iteration (the trip count is wave-uniform), so all
-  // lanes are active for the shuffles; only the stores are predicated.
-  for (uint32_t chunk = first; chunk < stop; chunk += 64) {
-    const uint32_t i = chunk + (uint32_t)lane;
-    int owner = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-      const int cand = owner + step;
-      const uint32_t oc = __shfl(off, cand & 63, 64);
-      if (cand <= last_lane && oc <= i) owner = cand;
-    }
-    const uint32_t o_off = __shfl(off, owner, 64);
-    const int ow = __shfl(w, owner, 64);
-    const int oy0 = __shfl(y0, owner, 64);
-    const int ox0 = __shfl(x0, owner, 64);
-    const int obase = __shfl(tile_base, owner, 64);
-    const int ocols = __shfl(cols, owner, 64);
-    const uint32_t oslot = __shfl(slot, owner, 64);
-    if (i < stop) {
-      const uint32_t local = i - o_off;
-      const int ty = oy0 + (int)(local / (uint32_t)ow);
-      const int tx = ox0 + (int)(local % (uint32_t)ow);
-      pair_keys[i] = (uint32_t)(obase + ty * ocols + tx);
-      pair_vals[i] = oslot;
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys, long long p,
-                                                     uint32_t* __restrict__ ranges, int total_tiles) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p) return;
-  const uint32_t k = keys[i];
-  if (k >= (uint32_t)total_tiles) return;  // cannot happen for keys the emitter wrote
-  if (i == 0 || keys[i - 1] != k) ranges[2 * (long long)k] = (uint32_t)i;
-  if (i == p - 1 || keys[i + 1] != k) ranges[2 * (long long)k + 1] = (uint32_t)(i + 1);
-}
-
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -334,26 +254,6 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
     *result_in_alt = !*result_in_alt;
   }
   return hipSuccess;
-}
-
-hipError_t launch_emit_pairs(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* counts,
-                             const SplatRec* rec, const ViewDesc* views, int n_views, long long per_view,
-                             long long m, uint32_t* pair_keys, uint32_t* pair_vals, uint32_t* pair_base,
-                             hipStream_t stream) {
-  (void)n_views;
-  if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
-                     counts, rec, views, per_view, m, pair_keys, pair_vals, pair_base);
-  return hipGetLastError();
-}
-
-hipError_t launch_tile_ranges(const uint32_t* sorted_keys, long long p, uint32_t* ranges, int total_tiles,
-                              hipStream_t stream) {
-  SPLAT_HIP_CHECK(launch_fill_u32(ranges, 0u, 2LL * total_tiles, stream));
-  if (p <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, stream, sorted_keys, p,
-                     ranges, total_tiles);
-  return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* ptr, uint32_t value, long long n, hipStream_t stream) {

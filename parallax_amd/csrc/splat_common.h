@@ -9,6 +9,7 @@
 namespace splat {
 
 constexpr int kTile = 16;             // logical tile edge (SplatRaster.cpp:18)
+constexpr int kSuper = 8;             // super-tile edge, in tiles (binning only)
 constexpr float kMaxAlpha = 0.99f;    // SplatRaster.cpp:22
 constexpr float kSlack = 1.3f;        // SplatRaster.cpp:26
 constexpr float kShC0 = 0.28209479177387814f;
@@ -28,6 +29,8 @@ struct ViewDesc {
   int width, height;
   int cols, rows;      // 16x16 tiles
   int tile_base;       // first global tile id of this view in the batch
+  int st_base;         // first global super-tile id (8x8 tiles) of this view
+  int scols, srows;    // super-tiles
   int pad0;
   long long pixel_base;  // first pixel of this view in the batch image buffers
 };

@@ -68,9 +68,13 @@ struct splat_context {
   float ssim_kernel[11] = {};
 
   DeviceBuffer rec, keys, vals, keys_alt, vals_alt, counts, offsets, pair_base;
-  DeviceBuffer pair_keys, pair_vals, pair_keys_alt, pair_vals_alt, ranges;
+  DeviceBuffer scounts, soffsets;                   // super-tile pairs per slot, their scan
+  DeviceBuffer skeys, svals, skeys_alt, svals_alt;  // (super-tile, slot) pairs
+  DeviceBuffer sranges, nseg, seg_base, seg_st, bin_table;
+  DeviceBuffer pair_vals, ranges;                   // per-tile lists (slots), [begin, end) per tile
+  unsigned long long S = 0;                         // super-tile pairs of the batch
+  int n_st = 0;                                     // super-tiles of the batch
   uint32_t* sorted_slots = nullptr;
-  uint32_t* sorted_pair_keys = nullptr;
   uint32_t* sorted_pair_vals = nullptr;
   DeviceBuffer rgb, T, last, dL, terms, block_sums, loss_sums;
   DeviceBuffer screen, partial;

@@ -9,9 +9,10 @@
 namespace splat {
 
 // ---- projection ----
+// counts[slot] = tiles of the slot's rect, scounts[slot] = super-tiles of it.
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
-                          uint32_t* keys, uint32_t* vals, uint32_t* counts, hipStream_t stream);
+                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, hipStream_t stream);
 
 // ---- scan / sort ----
 constexpr int kScanThreads = 256;
@@ -44,16 +45,39 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
                              hipStream_t stream);
 
-// ---- binning ----
-// For each depth-sorted slot s: writes the pair keys (global tile id) and
-// values (slot) of its tile rect at offsets[s], and pair_base[slot].
-hipError_t launch_emit_pairs(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* counts,
-                             const SplatRec* rec, const ViewDesc* views, int n_views, long long per_view,
-                             long long m, uint32_t* pair_keys, uint32_t* pair_vals, uint32_t* pair_base,
+// ---- binning (binning.hip) ----
+constexpr int kBinSegment = 1024;  // splats per segment of a super-tile list
+
+// For each depth-sorted slot s: writes one (global super-tile id, slot) pair
+// per super-tile of its rect at soffsets[s], and pair_base[slot] = offsets[s].
+hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
+                             const uint32_t* scounts, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             long long m, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base, hipStream_t stream);
+// sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted pairs.
+hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, uint32_t* sranges, int n_st,
+                               hipStream_t stream);
+// nseg[st] = segments of kBinSegment splats in super-tile st's list.
+hipError_t launch_segment_counts(const uint32_t* sranges, int n_st, uint32_t* nseg, hipStream_t stream);
+// seg_st[seg_base[st] + s] = st for every segment s of st.
+hipError_t launch_segment_list(const uint32_t* nseg, const uint32_t* seg_base, int n_st, uint32_t* seg_st,
+                               hipStream_t stream);
+// Per (super-tile, tile, segment) counts into table[0 .. 64*ns_max] (the last
+// entry and those of missing segments zeroed); *ns_total = real segments
+// (device), ns_max >= *ns_total.
+hipError_t launch_tile_count(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
+                             const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
+                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, uint32_t* table,
                              hipStream_t stream);
-// ranges[2*t] / ranges[2*t+1] = [begin, end) of global tile t in the sorted pairs.
-hipError_t launch_tile_ranges(const uint32_t* sorted_keys, long long p, uint32_t* ranges, int total_tiles,
-                              hipStream_t stream);
+// Every tile's list, in depth order, into pair_vals at the runs `scan` (the
+// exclusive scan of the count table) assigns.
+hipError_t launch_tile_write(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
+                             const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
+                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, const uint32_t* scan,
+                             uint32_t* pair_vals, hipStream_t stream);
+// ranges[2*t] / ranges[2*t+1] = [begin, end) of global tile t in pair_vals.
+hipError_t launch_tile_ranges_table(const ViewDesc* views, int n_views, int total_tiles, const uint32_t* scan,
+                                    const uint32_t* nseg, const uint32_t* seg_base, uint32_t* ranges,
+                                    hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* ptr, uint32_t value, long long n, hipStream_t stream);
 hipError_t launch_fill_f32(float* ptr, float value, long long n, hipStream_t stream);
 

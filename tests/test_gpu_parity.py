@@ -200,6 +200,13 @@ def test_batched_uneven_views_match_single_view_runs(ctx):
     # batched
     ctx.zero_gradients()
     ctx.render_forward([0, 1, 2], opts)
+    # every view's tile lists, binned in one batch, equal the checker's
+    for k, v in enumerate(views):
+        f = O.Frame(cloud, v, O.default_options(sh_degree=3))
+        cols, rows, offs, ents = ctx.tiles(k)
+        ocols, orows, ooffs, oents = f.tiles()
+        assert (cols, rows) == (ocols, orows)
+        assert np.array_equal(offs, ooffs) and np.array_equal(ents, oents)
     losses = ctx.loss(0.2)
     images = [ctx.image(k)[0] for k in range(3)]
     ctx.render_backward()
