@@ -994,7 +994,10 @@ const std::vector<float>& ssimKernel() {
 }
 }  // namespace
 
-// blur, SplatRaster.cpp:614-642: separable, zero padded.
+// blur, SplatRaster.cpp:614-642: separable, zero padded. Each tap is one
+// fused multiply-add, in tap order -5..5 (a fixed rounding both this checker
+// and the device kernels follow; the engine's own build leaves the
+// association to the compiler).
 void orc_blur(const float* src, int width, int height, float* dst) {
   const std::vector<float>& k = ssimKernel();
   const size_t n = static_cast<size_t>(width) * height;
@@ -1005,7 +1008,7 @@ void orc_blur(const float* src, int width, int height, float* dst) {
       for (int i = -kSsimRadius; i <= kSsimRadius; ++i) {
         const int sx = x + i;
         if (sx < 0 || sx >= width) continue;
-        sum += k[static_cast<size_t>(i + kSsimRadius)] * src[static_cast<size_t>(y) * width + sx];
+        sum = std::fma(k[static_cast<size_t>(i + kSsimRadius)], src[static_cast<size_t>(y) * width + sx], sum);
       }
       horizontal[static_cast<size_t>(y) * width + x] = sum;
     }
@@ -1015,7 +1018,7 @@ void orc_blur(const float* src, int width, int height, float* dst) {
       for (int i = -kSsimRadius; i <= kSsimRadius; ++i) {
         const int sy = y + i;
         if (sy < 0 || sy >= height) continue;
-        sum += k[static_cast<size_t>(i + kSsimRadius)] * horizontal[static_cast<size_t>(sy) * width + x];
+        sum = std::fma(k[static_cast<size_t>(i + kSsimRadius)], horizontal[static_cast<size_t>(sy) * width + x], sum);
       }
       dst[static_cast<size_t>(y) * width + x] = sum;
     }

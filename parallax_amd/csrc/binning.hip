@@ -15,8 +15,9 @@
 //               super-tile its rect touches (stable radix sort by id next);
 //   2. segment: each super-tile's list is cut into segments of 1024 splats;
 //   3. count:   one wave per segment builds, per splat, the 64-bit mask of the
-//               super-tile's tiles it covers (a row mask times a column mask)
-//               and counts, per tile, the segment's splats (ballot + popcount);
+//               super-tile's tiles it covers (a row mask times a column mask,
+//               from the rect bits the emitter packed into the key) and counts,
+//               per tile, the segment's splats (ballot + popcount);
 //   4. scan:    an exclusive scan over the (super-tile, tile, segment) table
 //               gives every (tile, segment) run its place, and the tile ranges;
 //   5. write:   the same wave pass again, writing each splat into each covered
@@ -39,14 +40,17 @@ __device__ __forceinline__ int find_view_st(const ViewDesc* views, int n_views, 
 
 // Super-tile pair emission, one wave per 64 consecutive depth-sorted slots.
 // The wave's pairs are one contiguous range of the output (soffsets is an
-// exclusive scan in the same order), so the lanes stride over that range
-// together and each lane finds the slot owning its pair by a binary search
-// over the wave's 64 offsets. Also records each slot's first tile-pair index
-// (pair_base), which the deterministic adjoint uses as the pair id base.
+// exclusive scan in the same order, *s_total its sum), so the lanes stride
+// over that range together and each lane finds the slot owning its pair by a
+// binary search over the wave's 64 offsets. A key is the global super-tile id
+// in its low kStBits bits and the splat's rect inside that super-tile (three
+// bits per bound) above them, where the sort by id carries it along. With
+// pair_base, also records each slot's first tile-pair index, the
+// deterministic adjoint's pair id base.
 __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__ sorted_slots,
                                                     const uint32_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ soffsets,
-                                                    const uint32_t* __restrict__ scounts,
+                                                    const uint32_t* __restrict__ s_total,
                                                     const SplatRec* __restrict__ rec,
                                                     const ViewDesc* __restrict__ views, long long per_view,
                                                     long long m, uint32_t* __restrict__ skeys,
@@ -54,20 +58,22 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   uint32_t slot = 0, c = 0, off = 0xFFFFFFFFu;
-  int sx0 = 0, w = 1, sy0 = 0, scols = 0, st_base = 0;
+  int x0 = 0, x1 = 0, y0 = 0, y1 = 0, w = 1, scols = 0, st_base = 0;
   if (s < m) {
     slot = sorted_slots[s];
-    c = scounts[slot];
     off = soffsets[s];
+    c = (s + 1 < m ? soffsets[s + 1] : *s_total) - off;
     if (c) {
-      const SplatRec& r = rec[slot];
+      const int2 packed = *reinterpret_cast<const int2*>(&rec[slot].x0);  // x0, x1, y0, y1 as int16
+      x0 = (int)(int16_t)(packed.x & 0xFFFF);
+      x1 = (int)(int16_t)(packed.x >> 16);
+      y0 = (int)(int16_t)(packed.y & 0xFFFF);
+      y1 = (int)(int16_t)(packed.y >> 16);
       const int v = (int)(slot / per_view);
-      sx0 = r.x0 / kSuper;
-      sy0 = r.y0 / kSuper;
-      w = r.x1 / kSuper - sx0 + 1;
+      w = x1 / kSuper - x0 / kSuper + 1;
       scols = views[v].scols;
       st_base = views[v].st_base;
-      pair_base[slot] = offsets[s];
+      if (pair_base) pair_base[slot] = offsets[s];
     }
   }
   const uint32_t first = __shfl(off, 0, 64);
@@ -88,16 +94,19 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
     }
     const uint32_t o_off = __shfl(off, owner, 64);
     const int ow = __shfl(w, owner, 64);
-    const int osy0 = __shfl(sy0, owner, 64);
-    const int osx0 = __shfl(sx0, owner, 64);
+    const int ox0 = __shfl(x0, owner, 64), ox1 = __shfl(x1, owner, 64);
+    const int oy0 = __shfl(y0, owner, 64), oy1 = __shfl(y1, owner, 64);
     const int obase = __shfl(st_base, owner, 64);
     const int ocols = __shfl(scols, owner, 64);
     const uint32_t oslot = __shfl(slot, owner, 64);
     if (i < stop) {
       const uint32_t local = i - o_off;
-      const int sy = osy0 + (int)(local / (uint32_t)ow);
-      const int sx = osx0 + (int)(local % (uint32_t)ow);
-      skeys[i] = (uint32_t)(obase + sy * ocols + sx);
+      const int sy = oy0 / kSuper + (int)(local / (uint32_t)ow);
+      const int sx = ox0 / kSuper + (int)(local % (uint32_t)ow);
+      const int lx0 = max(ox0 - sx * kSuper, 0), lx1 = min(ox1 - sx * kSuper, kSuper - 1);
+      const int ly0 = max(oy0 - sy * kSuper, 0), ly1 = min(oy1 - sy * kSuper, kSuper - 1);
+      skeys[i] = (uint32_t)(obase + sy * ocols + sx) | ((uint32_t)lx0 << kStBits) | ((uint32_t)lx1 << (kStBits + 3)) |
+                 ((uint32_t)ly0 << (kStBits + 6)) | ((uint32_t)ly1 << (kStBits + 9));
       svals[i] = oslot;
     }
   }
@@ -107,10 +116,10 @@ __global__ __launch_bounds__(256) void k_super_ranges(const uint32_t* __restrict
                                                       uint32_t* __restrict__ ranges, int n_st) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t k = keys[i];
+  const uint32_t k = keys[i] & kStMask;
   if (k >= (uint32_t)n_st) return;  // cannot happen for keys the emitter wrote
-  if (i == 0 || keys[i - 1] != k) ranges[2 * (long long)k] = (uint32_t)i;
-  if (i == n - 1 || keys[i + 1] != k) ranges[2 * (long long)k + 1] = (uint32_t)(i + 1);
+  if (i == 0 || (keys[i - 1] & kStMask) != k) ranges[2 * (long long)k] = (uint32_t)i;
+  if (i == n - 1 || (keys[i + 1] & kStMask) != k) ranges[2 * (long long)k + 1] = (uint32_t)(i + 1);
 }
 
 __global__ __launch_bounds__(256) void k_segment_counts(const uint32_t* __restrict__ sranges, int n_st,
@@ -131,15 +140,13 @@ __global__ __launch_bounds__(256) void k_segment_list(const uint32_t* __restrict
 }
 
 // One segment as seen by one wave: which super-tile, which slice of its list,
-// and where its tiles start.
+// and where its table entries start.
 struct Segment {
   uint32_t st, s, nsg, b0, b1;
   uint32_t table0;  // 64 * seg_base[st]
-  int ox, oy;       // first tile (in the view's tile grid) of the super-tile
 };
 
-__device__ __forceinline__ Segment segment_of(uint32_t g, const ViewDesc* views, int n_views,
-                                              const uint32_t* sranges, const uint32_t* nseg,
+__device__ __forceinline__ Segment segment_of(uint32_t g, const uint32_t* sranges, const uint32_t* nseg,
                                               const uint32_t* seg_base, const uint32_t* seg_st) {
   Segment S;
   S.st = seg_st[g];
@@ -149,23 +156,14 @@ __device__ __forceinline__ Segment segment_of(uint32_t g, const ViewDesc* views,
   const uint32_t b = sranges[2 * S.st], e = sranges[2 * S.st + 1];
   S.b0 = b + S.s * kBinSegment;
   S.b1 = min(S.b0 + (uint32_t)kBinSegment, e);
-  const int v = find_view_st(views, n_views, (int)S.st);
-  const ViewDesc& V = views[v];
-  const int local = (int)S.st - V.st_base;
-  S.ox = (local % V.scols) * kSuper;
-  S.oy = (local / V.scols) * kSuper;
   return S;
 }
 
 // Bit (ly * 8 + lx) set for every tile (lx, ly) of the super-tile that the
-// splat's rect covers.
-__device__ __forceinline__ uint64_t cover_mask(const SplatRec* rec, uint32_t slot, int ox, int oy) {
-  const int2 packed = *reinterpret_cast<const int2*>(&rec[slot].x0);  // x0, x1, y0, y1 as int16
-  const int x0 = (int)(int16_t)(packed.x & 0xFFFF), x1 = (int)(int16_t)(packed.x >> 16);
-  const int y0 = (int)(int16_t)(packed.y & 0xFFFF), y1 = (int)(int16_t)(packed.y >> 16);
-  const int lx0 = max(x0 - ox, 0), lx1 = min(x1 - ox, kSuper - 1);
-  const int ly0 = max(y0 - oy, 0), ly1 = min(y1 - oy, kSuper - 1);
-  if (lx0 > lx1 || ly0 > ly1) return 0ull;
+// splat's rect covers, from the rect bits of its super-tile key.
+__device__ __forceinline__ uint64_t cover_mask(uint32_t key) {
+  const int lx0 = (key >> kStBits) & 7, lx1 = (key >> (kStBits + 3)) & 7;
+  const int ly0 = (key >> (kStBits + 6)) & 7, ly1 = (key >> (kStBits + 9)) & 7;
   const uint64_t all = 0x0101010101010101ull;
   const uint64_t rows = (all << (8 * ly0)) & (all >> (8 * (kSuper - 1 - ly1)));
   const uint64_t cols = (uint64_t)(((2u << lx1) - 1u) & ~((1u << lx0) - 1u));
@@ -185,14 +183,13 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t m) {
 // Per (tile, segment) splat counts into the table at
 // 64 * seg_base[st] + tile * nseg[st] + s; entries of segments past the last
 // real one (up to ns_max) are zeroed. One wave per segment, grid-stride.
-__global__ __launch_bounds__(256) void k_tile_count(const ViewDesc* __restrict__ views, int n_views,
-                                                    const uint32_t* __restrict__ svals,
+__global__ __launch_bounds__(256) void k_tile_count(const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ sranges,
                                                     const uint32_t* __restrict__ nseg,
                                                     const uint32_t* __restrict__ seg_base,
                                                     const uint32_t* __restrict__ seg_st,
                                                     const uint32_t* __restrict__ ns_total, uint32_t ns_max,
-                                                    const SplatRec* __restrict__ rec, uint32_t* __restrict__ table) {
+                                                    uint32_t* __restrict__ table) {
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   const uint32_t ns = *ns_total;
@@ -201,11 +198,11 @@ __global__ __launch_bounds__(256) void k_tile_count(const ViewDesc* __restrict__
       table[64ull * g + lane] = 0u;
       continue;
     }
-    const Segment S = segment_of(g, views, n_views, sranges, nseg, seg_base, seg_st);
+    const Segment S = segment_of(g, sranges, nseg, seg_base, seg_st);
     uint32_t cnt = 0;  // lane t: splats of this segment covering tile t
     for (uint32_t c = S.b0; c < S.b1; c += 64) {
       const uint32_t i = c + (uint32_t)lane;
-      const uint64_t mask = i < S.b1 ? cover_mask(rec, svals[i], S.ox, S.oy) : 0ull;
+      const uint64_t mask = i < S.b1 ? cover_mask(skeys[i]) : 0ull;
       uint64_t u = wave_or(mask);
       while (u) {
         const int t = __builtin_ctzll(u);
@@ -220,21 +217,20 @@ __global__ __launch_bounds__(256) void k_tile_count(const ViewDesc* __restrict__
 
 // Writes every splat of every segment into each covered tile's run. Same
 // traversal as k_tile_count; `scan` is the exclusive scan of its table.
-__global__ __launch_bounds__(256) void k_tile_write(const ViewDesc* __restrict__ views, int n_views,
+__global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ svals,
                                                     const uint32_t* __restrict__ sranges,
                                                     const uint32_t* __restrict__ nseg,
                                                     const uint32_t* __restrict__ seg_base,
                                                     const uint32_t* __restrict__ seg_st,
                                                     const uint32_t* __restrict__ ns_total,
-                                                    const SplatRec* __restrict__ rec,
                                                     const uint32_t* __restrict__ scan,
                                                     uint32_t* __restrict__ pair_vals) {
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   const uint32_t ns = *ns_total;
   for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < ns; g += waves) {
-    const Segment S = segment_of(g, views, n_views, sranges, nseg, seg_base, seg_st);
+    const Segment S = segment_of(g, sranges, nseg, seg_base, seg_st);
     uint32_t base = scan[S.table0 + (uint64_t)lane * S.nsg + S.s];  // lane t: next slot of tile t
     for (uint32_t c = S.b0; c < S.b1; c += 64) {
       const uint32_t i = c + (uint32_t)lane;
@@ -242,7 +238,7 @@ __global__ __launch_bounds__(256) void k_tile_write(const ViewDesc* __restrict__
       uint64_t mask = 0ull;
       if (i < S.b1) {
         slot = svals[i];
-        mask = cover_mask(rec, slot, S.ox, S.oy);
+        mask = cover_mask(skeys[i]);
       }
       uint64_t u = wave_or(mask);
       while (u) {
@@ -284,11 +280,11 @@ __global__ __launch_bounds__(256) void k_tile_ranges_table(const ViewDesc* __res
 }  // namespace
 
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* scounts, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
                              long long m, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base, hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
-                     soffsets, scounts, rec, views, per_view, m, skeys, svals, pair_base);
+                     soffsets, s_total, rec, views, per_view, m, skeys, svals, pair_base);
   return hipGetLastError();
 }
 
@@ -325,25 +321,24 @@ unsigned segment_grid(uint32_t ns_max) {
 }
 }  // namespace
 
-hipError_t launch_tile_count(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
+hipError_t launch_tile_count(const uint32_t* skeys, const uint32_t* sranges,
                              const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
-                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, uint32_t* table,
-                             hipStream_t stream) {
+                             const uint32_t* ns_total, uint32_t ns_max, uint32_t* table, hipStream_t stream) {
   // the entry past the last segment's table block holds the grand total after the scan
   SPLAT_HIP_CHECK(launch_fill_u32(table + 64ull * ns_max, 0u, 1, stream));
   if (ns_max == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tile_count, dim3(segment_grid(ns_max)), dim3(256), 0, stream, views, n_views, svals, sranges,
-                     nseg, seg_base, seg_st, ns_total, ns_max, rec, table);
+  hipLaunchKernelGGL(k_tile_count, dim3(segment_grid(ns_max)), dim3(256), 0, stream, skeys, sranges,
+                     nseg, seg_base, seg_st, ns_total, ns_max, table);
   return hipGetLastError();
 }
 
-hipError_t launch_tile_write(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
-                             const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
-                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, const uint32_t* scan,
+hipError_t launch_tile_write(const uint32_t* skeys, const uint32_t* svals,
+                             const uint32_t* sranges, const uint32_t* nseg, const uint32_t* seg_base,
+                             const uint32_t* seg_st, const uint32_t* ns_total, uint32_t ns_max, const uint32_t* scan,
                              uint32_t* pair_vals, hipStream_t stream) {
   if (ns_max == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tile_write, dim3(segment_grid(ns_max)), dim3(256), 0, stream, views, n_views, svals, sranges,
-                     nseg, seg_base, seg_st, ns_total, rec, scan, pair_vals);
+  hipLaunchKernelGGL(k_tile_write, dim3(segment_grid(ns_max)), dim3(256), 0, stream, skeys, svals,
+                     sranges, nseg, seg_base, seg_st, ns_total, scan, pair_vals);
   return hipGetLastError();
 }
 

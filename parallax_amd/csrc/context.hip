@@ -176,19 +176,23 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   ctx->opts = *o;
   ctx->batch_ids.assign(view_ids, view_ids + n_views);
   ctx->batch_desc.clear();
-  int tiles = 0, supers = 0;
+  int tiles = 0, supers = 0, ltiles = 0;
   long long px = 0;
   for (int k = 0; k < n_views; ++k) {
     ViewDesc d = make_desc(ctx->views[view_ids[k]]);
     d.tile_base = tiles;
     d.st_base = supers;
     d.pixel_base = px;
+    d.target_base = ctx->has_target[view_ids[k]] ? ctx->target_offset[view_ids[k]] : 0;
+    d.lt_base = ltiles;
+    ltiles += loss_tiles(d.width, d.height);
     tiles += d.cols * d.rows;
     supers += d.scols * d.srows;
     px += (long long)d.width * d.height;
     ctx->batch_desc.push_back(d);
   }
   ctx->total_tiles = tiles;
+  ctx->loss_tiles = ltiles;
   ctx->n_st = supers;
   ctx->total_px = px;
   ctx->sh_use = std::min(o->sh_degree, ctx->sh_degree);
@@ -256,9 +260,10 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   CK(ctx->bin_table.reserve(4 * (64ull * ns_max + 1)), "alloc binning table");
   CK(ctx->ensure_sort_workspace(std::max<long long>(std::max<long long>(S, M), 64LL * ns_max + 1)),
      "alloc sort workspace");
-  TIMED("emit_super", launch_emit_super(sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), ctx->scounts.u(),
-                                        ctx->recs(), ctx->views_dev(), n, M, ctx->skeys.u(), ctx->svals.u(),
-                                        ctx->pair_base.u(), ctx->stream));
+  if (supers > (int)kStMask) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: batch too large");
+  TIMED("emit_super", launch_emit_super(sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), d_total + 1, ctx->recs(),
+                                        ctx->views_dev(), n, M, ctx->skeys.u(), ctx->svals.u(),
+                                        o->deterministic ? ctx->pair_base.u() : nullptr, ctx->stream));
   const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
   bool salt = false;
   TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(), S, 0,
@@ -270,14 +275,14 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   TIMED("segments", launch_exclusive_scan(ctx->nseg.u(), nullptr, ctx->seg_base.u(), supers, d_total + 2,
                                           ctx->sort_ws.scan, ctx->stream));
   TIMED("segments", launch_segment_list(ctx->nseg.u(), ctx->seg_base.u(), supers, ctx->seg_st.u(), ctx->stream));
-  TIMED("tile_count", launch_tile_count(ctx->views_dev(), n_views, svals, ctx->sranges.u(), ctx->nseg.u(),
-                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->recs(),
-                                        ctx->bin_table.u(), ctx->stream));
+  TIMED("tile_count", launch_tile_count(skeys, ctx->sranges.u(), ctx->nseg.u(),
+                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->bin_table.u(),
+                                        ctx->stream));
   TIMED("tile_scan", launch_exclusive_scan(ctx->bin_table.u(), nullptr, ctx->bin_table.u(), 64LL * ns_max + 1,
                                            nullptr, ctx->sort_ws.scan, ctx->stream));
-  TIMED("tile_write", launch_tile_write(ctx->views_dev(), n_views, svals, ctx->sranges.u(), ctx->nseg.u(),
-                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->recs(),
-                                        ctx->bin_table.u(), ctx->pair_vals.u(), ctx->stream));
+  TIMED("tile_write", launch_tile_write(skeys, svals, ctx->sranges.u(), ctx->nseg.u(),
+                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->bin_table.u(),
+                                        ctx->pair_vals.u(), ctx->stream));
   TIMED("tile_ranges", launch_tile_ranges_table(ctx->views_dev(), n_views, tiles, ctx->bin_table.u(), ctx->nseg.u(),
                                                 ctx->seg_base.u(), ctx->ranges.u(), ctx->stream));
   ctx->sorted_pair_vals = ctx->pair_vals.u();
@@ -303,26 +308,29 @@ splat_status loss_impl(splat_context* ctx, float w, float* losses) {
   const long long px = ctx->total_px;
   const int V = (int)ctx->batch_ids.size();
   CK(ctx->dL.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc dL");
-  CK(ctx->block_sums.reserve(sizeof(float) * 2 * std::max(1, ctx->total_tiles)), "alloc loss");
-  CK(ctx->loss_sums.reserve(sizeof(float) * 2 * V), "alloc loss");
+  CK(ctx->block_sums.reserve(sizeof(float) * 2 * std::max(1, ctx->loss_tiles)), "alloc loss");
   if (w > 0.f) CK(ctx->terms.reserve(sizeof(float) * 15 * std::max(1LL, px)), "alloc ssim terms");
-  // Gather the batch's targets into batch pixel order.
-  CK(ctx->batch_targets.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc targets");
+  CK(ctx->loss_sums.reserve(sizeof(float) * 2 * V), "alloc loss");
+  // targets uploaded after the forward: refresh the views' target offsets
+  bool stale = false;
   for (int k = 0; k < V; ++k) {
-    const ViewDesc& d = ctx->batch_desc[k];
-    CK(hipMemcpyAsync(ctx->batch_targets.f() + 3 * d.pixel_base, ctx->targets.f() + ctx->target_offset[ctx->batch_ids[k]],
-                      sizeof(float) * 3 * (size_t)d.width * d.height, hipMemcpyDeviceToDevice, ctx->stream),
-       "gather targets");
+    const long long want = ctx->target_offset[ctx->batch_ids[k]];
+    if (ctx->batch_desc[k].target_base != want) {
+      ctx->batch_desc[k].target_base = want;
+      stale = true;
+    }
   }
+  if (stale)
+    CK(hipMemcpyAsync(ctx->d_views.ptr, ctx->batch_desc.data(), sizeof(ViewDesc) * V, hipMemcpyHostToDevice,
+                      ctx->stream),
+       "upload views");
   LossParams lp{};
   for (int i = 0; i < 11; ++i) lp.kernel[i] = ctx->ssim_kernel[i];
   lp.ssim_weight = w;
   lp.n_views = V;
-  lp.total_tiles = ctx->total_tiles;
-  TIMED("loss_forward", launch_loss_forward(lp, ctx->views_dev(), ctx->rgb.f(), ctx->batch_targets.f(), ctx->dL.f(),
-                                            ctx->terms.f(), px, ctx->block_sums.f(), ctx->stream));
-  TIMED("loss_backward", launch_loss_backward(lp, ctx->views_dev(), ctx->rgb.f(), ctx->batch_targets.f(),
-                                              ctx->terms.f(), px, ctx->dL.f(), ctx->stream));
+  lp.total_tiles = ctx->loss_tiles;
+  TIMED("loss", launch_loss(lp, ctx->views_dev(), ctx->rgb.f(), ctx->targets.f(), ctx->dL.f(), ctx->terms.f(), px,
+                            ctx->block_sums.f(), ctx->stream));
   CK(launch_loss_reduce(ctx->views_dev(), V, ctx->block_sums.f(), ctx->loss_sums.f(), ctx->stream), "loss reduce");
   ctx->loss_done = true;
   ctx->last_ssim_weight = w;

@@ -1,23 +1,27 @@
-// Fused L1 + D-SSIM photometric loss and its gradient.
+// L1 + D-SSIM photometric loss and its gradient.
 //
 // The engine's photometricLoss / meanSsim / blur (engine/native/cpp/src/splat/
 // SplatRaster.cpp:580-738): L = (1-w) mean|x-y| + w (1 - mean SSIM) with an
-// 11-tap sigma 1.5 Gaussian window applied as a zero-padded separable blur,
-// C1 = 0.01^2, C2 = 0.03^2, means over 3 * pixels. The gradient of the SSIM
-// term is itself a blur of per-pixel partials (the blur is self-adjoint on a
-// zero-padded domain).
+// 11-tap sigma 1.5 Gaussian window applied as a zero-padded separable blur
+// (rows first, then columns), C1 = 0.01^2, C2 = 0.03^2, means over 3 * pixels.
+// The gradient of the SSIM term is itself a blur of five per-pixel partial
+// planes (the blur is self-adjoint on a zero-padded domain).
 //
-// Two passes over 16x16 output tiles of every view of the batch, one 256-thread
-// workgroup per tile, each staging a 26x26 halo tile per channel in LDS:
-//   pass 1: the 5 statistics planes (x, y, xx, yy, xy) blurred horizontally
-//           then vertically in LDS, the SSIM map, the L1 gradient, and the five
-//           per-pixel partial planes written to HBM; per-tile L1 and SSIM sums.
-//   pass 2: the five partial planes blurred the same way and combined into
-//           dL/drgb.
-// Tap order, zero padding and every expression follow the engine's, so
-// dL/drgb matches the CPU checker to float rounding of identical operations.
-// HBM-bound: pass 1 reads 24 B and writes 72 B per pixel, pass 2 reads
-// 84 B and writes 12 B per pixel (halo re-reads hit in L2).
+// Two passes over 64x16-pixel loss tiles of every view of the batch, one
+// 256-thread workgroup each, one channel at a time:
+//   stats: the image/target window (74x26, zero outside the image) goes to
+//          LDS; per statistics plane (x, y, xx, yy, xy) a row pass over the
+//          26 rows and a column pass down to the tile's 16 rows, every thread
+//          sliding an 11-tap window over a run of 4 outputs in registers; then
+//          per pixel the SSIM value, the five partial planes (to HBM), and the
+//          L1 terms; per-tile L1 and SSIM sums.
+//   grad:  per partial plane, its 74x26 window to LDS, the same two passes,
+//          and the blurred planes combined into dL/drgb.
+// Each blur tap is one fused multiply-add in tap order -5..5 and every other
+// expression is the engine's, exactly as the CPU checker evaluates them, so
+// dL/drgb matches the checker to float rounding of identical operations.
+// Compulsory HBM traffic per pixel: 24 B read and 60 B written by the first
+// pass, 84 B read and 12 B written by the second (halos hit in cache).
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -25,93 +29,140 @@ namespace splat {
 namespace {
 
 constexpr int kR = 5;
-constexpr int kHalo = kTile + 2 * kR;  // 26
+constexpr int kLW = 64, kLH = 16;         // loss tile, pixels
+constexpr int kWinW = kLW + 2 * kR;       // 74: window columns
+constexpr int kWinH = kLH + 2 * kR;       // 26: window rows
+constexpr int kRun = 4;                   // outputs per thread along a blur
+constexpr int kTaps = kRun + 2 * kR;      // 14 window values per run
+constexpr int kPitch = kWinW + 1;         // 75: odd, conflict-free row walks
+constexpr int kHPitch = kLW + 1;          // 65
+constexpr int kRowTasks = kWinH * (kLW / kRun);  // 416 row-pass runs
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 
-__device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int tile) {
+__device__ __forceinline__ int find_view_lt(const ViewDesc* views, int n_views, int lt) {
   int v = 0;
   for (int k = 1; k < n_views; ++k)
-    if (views[k].tile_base <= tile) v = k;
+    if (views[k].lt_base <= lt) v = k;
   return v;
 }
 
-// Blur `planes` LDS planes of kHalo x kHalo (zero outside the image) into
-// one value per plane at this thread's pixel: horizontal pass over all halo
-// rows into `hbuf`, then vertical. Tap order -5..5 as the engine's blur.
-template <int kPlanes>
-__device__ __forceinline__ void blur_tile(const float (*src)[kHalo][kHalo], float (*hbuf)[kHalo][kTile],
-                                          const float* kernel, float* out) {
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < kPlanes * kHalo * kTile; idx += blockDim.x) {
-    const int pl = idx / (kHalo * kTile);
-    const int rem = idx % (kHalo * kTile);
-    const int row = rem / kTile, col = rem % kTile;
-    float sum = 0.f;
+// out[k] = sum_i kernel[i] * win[k + i] for k < kRun, one fused multiply-add
+// per tap in tap order.
+__device__ __forceinline__ void blur_run(const float (&win)[kTaps], const float* kernel, float (&out)[kRun]) {
 #pragma unroll
-    for (int i = 0; i < 2 * kR + 1; ++i) sum += kernel[i] * src[pl][row][col + i];
-    hbuf[pl][row][col] = sum;
+  for (int k = 0; k < kRun; ++k) {
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 2 * kR + 1; ++i) s = __builtin_fmaf(kernel[i], win[k + i], s);
+    out[k] = s;
   }
-  __syncthreads();
-  const int ly = tid >> 4, lx = tid & 15;
-#pragma unroll
-  for (int pl = 0; pl < kPlanes; ++pl) {
-    float sum = 0.f;
-#pragma unroll
-    for (int i = 0; i < 2 * kR + 1; ++i) sum += kernel[i] * hbuf[pl][ly + i][lx];
-    out[pl] = sum;
-  }
-  __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_loss_forward(LossParams p, const ViewDesc* __restrict__ views,
-                                                      const float* __restrict__ rgb, const float* __restrict__ target,
-                                                      float* __restrict__ dL, float* __restrict__ terms,
-                                                      long long total_px, float* __restrict__ block_sums) {
-  __shared__ float s_src[5][kHalo][kHalo];
-  __shared__ float s_h[5][kHalo][kTile];
-  __shared__ float s_red[2][4];
-  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
-  const int v = find_view(views, p.n_views, tile);
+struct LossTile {
+  int ox, oy;  // first pixel
+  const ViewDesc* V;
+};
+
+__device__ __forceinline__ LossTile loss_tile_of(const LossParams& p, const ViewDesc* views) {
+  const int lt = xcd_remap(blockIdx.x, p.total_tiles);
+  const int v = find_view_lt(views, p.n_views, lt);
   const ViewDesc& V = views[v];
-  const int local = tile - V.tile_base;
-  const int tx = local % V.cols, ty = local / V.cols;
+  const int local = lt - V.lt_base;
+  const int ltcols = (V.width + kLW - 1) / kLW;
+  LossTile t;
+  t.ox = (local % ltcols) * kLW;
+  t.oy = (local / ltcols) * kLH;
+  t.V = &V;
+  return t;
+}
+
+// Row pass of one plane over the window's 26 rows into hbuf[26][65]: plane 0
+// blurs a, 1 blurs b, 2 a*a, 3 b*b, 4 a*b (a, b: LDS windows).
+__device__ __forceinline__ void row_pass(int plane, const float (*a)[kPitch], const float (*b)[kPitch],
+                                         const float* kernel, float (*hbuf)[kHPitch]) {
+  for (int task = threadIdx.x; task < kRowTasks; task += blockDim.x) {
+    const int r = task % kWinH, j0 = (task / kWinH) * kRun;
+    float win[kTaps], o[kRun];
+#pragma unroll
+    for (int i = 0; i < kTaps; ++i) {
+      const float x = a[r][j0 + i];
+      if (plane == 0) {
+        win[i] = x;
+      } else {
+        const float y = b[r][j0 + i];
+        win[i] = plane == 1 ? y : (plane == 2 ? x * x : (plane == 3 ? y * y : x * y));
+      }
+    }
+    blur_run(win, kernel, o);
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) hbuf[r][j0 + k] = o[k];
+  }
+}
+
+// Column pass: this thread's 4 outputs (column j, rows r0..r0+3 of the tile).
+__device__ __forceinline__ void col_pass(const float (*hbuf)[kHPitch], const float* kernel, int j, int r0,
+                                         float (&out)[kRun]) {
+  float win[kTaps];
+#pragma unroll
+  for (int i = 0; i < kTaps; ++i) win[i] = hbuf[r0 + i][j];
+  blur_run(win, kernel, out);
+}
+
+__global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc* __restrict__ views,
+                                                    const float* __restrict__ rgb,
+                                                    const float* __restrict__ targets, float* __restrict__ dL,
+                                                    float* __restrict__ terms, long long total_px,
+                                                    float* __restrict__ block_sums) {
+  __shared__ float s_x[kWinH][kPitch];
+  __shared__ float s_y[kWinH][kPitch];
+  __shared__ float s_h[kWinH][kHPitch];
+  __shared__ float s_red[2][4];
+  const LossTile L = loss_tile_of(p, views);
+  const ViewDesc& V = *L.V;
   const int tid = threadIdx.x;
-  const int px = tx * kTile + (tid & 15), py = ty * kTile + (tid >> 4);
-  const bool inside = px < V.width && py < V.height;
+  const int j = tid & (kLW - 1), r0 = (tid / kLW) * kRun;  // this thread's column and rows
   const long long pixels = (long long)V.width * V.height;
   const float inverse = 1.f / (float)(pixels * 3);
-  const float scale = inverse;
   const float w = p.ssim_weight;
+  const float* img = rgb + 3 * V.pixel_base;
+  const float* tgt = targets + V.target_base;
+  const int gx = L.ox + j;
   float l1 = 0.f, ssim = 0.f;
-  const long long pixel = V.pixel_base + (long long)py * V.width + px;
   for (int c = 0; c < 3; ++c) {
-    for (int idx = tid; idx < kHalo * kHalo; idx += blockDim.x) {
-      const int hy = idx / kHalo, hx = idx % kHalo;
-      const int gx = tx * kTile + hx - kR, gy = ty * kTile + hy - kR;
+    for (int idx = tid; idx < kWinH * kWinW; idx += blockDim.x) {
+      const int hy = idx / kWinW, hx = idx % kWinW;
+      const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
       float x = 0.f, y = 0.f;
-      if (gx >= 0 && gx < V.width && gy >= 0 && gy < V.height) {
-        const long long q = V.pixel_base + (long long)gy * V.width + gx;
-        x = rgb[3 * q + c];
-        y = target[3 * q + c];
+      if (qx >= 0 && qx < V.width && qy >= 0 && qy < V.height) {
+        const long long q = (long long)qy * V.width + qx;
+        x = img[3 * q + c];
+        y = tgt[3 * q + c];
       }
-      s_src[0][hy][hx] = x;
-      s_src[1][hy][hx] = y;
-      s_src[2][hy][hx] = x * x;
-      s_src[3][hy][hx] = y * y;
-      s_src[4][hy][hx] = x * y;
+      s_x[hy][hx] = x;
+      s_y[hy][hx] = y;
     }
     __syncthreads();
-    float st[5];
-    blur_tile<5>(s_src, s_h, p.kernel, st);
-    if (inside) {
-      const float x = rgb[3 * pixel + c], y = target[3 * pixel + c];
+    float st[5][kRun];
+    if (w > 0.f) {
+#pragma unroll
+      for (int pl = 0; pl < 5; ++pl) {
+        row_pass(pl, s_x, s_y, p.kernel, s_h);
+        __syncthreads();
+        col_pass(s_h, p.kernel, j, r0, st[pl]);
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) {
+      const int gy = L.oy + r0 + k;
+      if (gx >= V.width || gy >= V.height) continue;
+      const long long pixel = V.pixel_base + (long long)gy * V.width + gx;
+      const float x = s_x[r0 + k + kR][j + kR], y = s_y[r0 + k + kR][j + kR];
       const float difference = x - y;
       l1 += fabsf(difference);
-      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
-      dL[3 * pixel + c] = (1.f - w) * sign * inverse;
       if (w > 0.f) {
-        const float mu_x = st[0], mu_y = st[1], exx = st[2], eyy = st[3], exy = st[4];
+        const float mu_x = st[0][k], mu_y = st[1][k], exx = st[2][k], eyy = st[3][k], exy = st[4][k];
         const float var_x = exx - mu_x * mu_x;
         const float var_y = eyy - mu_y * mu_y;
         const float cov = exy - mu_x * mu_y;
@@ -120,17 +171,21 @@ __global__ __launch_bounds__(256) void k_loss_forward(LossParams p, const ViewDe
         const float cc = mu_x * mu_x + mu_y * mu_y + kC1;
         const float d = var_x + var_y + kC2;
         ssim += (a * b) / (cc * d);
-        const float term_mu = scale * (2.f * mu_y * b / (cc * d) - 2.f * mu_x * a * b / (cc * cc * d));
-        const float term_var = scale * (-(a * b) / (cc * d * d));
-        const float term_cov = scale * (2.f * a / (cc * d));
+        const float term_mu = inverse * (2.f * mu_y * b / (cc * d) - 2.f * mu_x * a * b / (cc * cc * d));
+        const float term_var = inverse * (-(a * b) / (cc * d * d));
+        const float term_cov = inverse * (2.f * a / (cc * d));
         float* tp = terms + (long long)(c * 5) * total_px + pixel;
         tp[0] = term_mu;
         tp[total_px] = term_var;
         tp[2 * total_px] = term_var * mu_x;
         tp[3 * total_px] = term_cov;
         tp[4 * total_px] = term_cov * mu_y;
+      } else {
+        const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+        dL[3 * pixel + c] = sign * inverse;
       }
     }
+    __syncthreads();
   }
   // fixed-order workgroup sums
   float a = l1, b = ssim;
@@ -145,48 +200,75 @@ __global__ __launch_bounds__(256) void k_loss_forward(LossParams p, const ViewDe
   }
   __syncthreads();
   if (tid == 0) {
-    block_sums[2 * tile] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
-    block_sums[2 * tile + 1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
+    const int lt = xcd_remap(blockIdx.x, p.total_tiles);
+    block_sums[2 * lt] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
+    block_sums[2 * lt + 1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
   }
 }
 
-__global__ __launch_bounds__(256) void k_loss_backward(LossParams p, const ViewDesc* __restrict__ views,
-                                                       const float* __restrict__ rgb,
-                                                       const float* __restrict__ target,
-                                                       const float* __restrict__ terms, long long total_px,
-                                                       float* __restrict__ dL) {
-  __shared__ float s_src[5][kHalo][kHalo];
-  __shared__ float s_h[5][kHalo][kTile];
-  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
-  const int v = find_view(views, p.n_views, tile);
-  const ViewDesc& V = views[v];
-  const int local = tile - V.tile_base;
-  const int tx = local % V.cols, ty = local / V.cols;
+__global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc* __restrict__ views,
+                                                   const float* __restrict__ rgb, const float* __restrict__ targets,
+                                                   const float* __restrict__ terms, long long total_px,
+                                                   float* __restrict__ dL) {
+  __shared__ float s_t[kWinH][kPitch];
+  __shared__ float s_h[kWinH][kHPitch];
+  const LossTile L = loss_tile_of(p, views);
+  const ViewDesc& V = *L.V;
   const int tid = threadIdx.x;
-  const int px = tx * kTile + (tid & 15), py = ty * kTile + (tid >> 4);
-  const bool inside = px < V.width && py < V.height;
-  const long long pixel = V.pixel_base + (long long)py * V.width + px;
+  const int j = tid & (kLW - 1), r0 = (tid / kLW) * kRun;
+  const long long pixels = (long long)V.width * V.height;
+  const float inverse = 1.f / (float)(pixels * 3);
+  const float w = p.ssim_weight;
+  const float* img = rgb + 3 * V.pixel_base;
+  const float* tgt = targets + V.target_base;
+  const int gx = L.ox + j;
   for (int c = 0; c < 3; ++c) {
-    for (int idx = tid; idx < kHalo * kHalo; idx += blockDim.x) {
-      const int hy = idx / kHalo, hx = idx % kHalo;
-      const int gx = tx * kTile + hx - kR, gy = ty * kTile + hy - kR;
-      const bool in = gx >= 0 && gx < V.width && gy >= 0 && gy < V.height;
-      const long long q = V.pixel_base + (long long)gy * V.width + gx;
+    float d[kRun] = {0.f, 0.f, 0.f, 0.f};
+    float xs[kRun], ys[kRun];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) s_src[k][hy][hx] = in ? terms[(long long)(c * 5 + k) * total_px + q] : 0.f;
+    for (int k = 0; k < kRun; ++k) {
+      const int gy = L.oy + r0 + k;
+      xs[k] = ys[k] = 0.f;
+      if (gx < V.width && gy < V.height) {
+        const long long q = (long long)gy * V.width + gx;
+        xs[k] = img[3 * q + c];
+        ys[k] = tgt[3 * q + c];
+      }
     }
-    __syncthreads();
-    float bl[5];
-    blur_tile<5>(s_src, s_h, p.kernel, bl);
-    if (inside) {
-      const float x = rgb[3 * pixel + c], y = target[3 * pixel + c];
-      float d = 0.f;
-      d += bl[0];
-      d += 2.f * x * bl[1];
-      d -= 2.f * bl[2];
-      d += y * bl[3];
-      d -= bl[4];
-      dL[3 * pixel + c] -= p.ssim_weight * d;
+#pragma unroll
+    for (int pl = 0; pl < 5; ++pl) {
+      const float* plane = terms + (long long)(c * 5 + pl) * total_px + V.pixel_base;
+      for (int idx = tid; idx < kWinH * kWinW; idx += blockDim.x) {
+        const int hy = idx / kWinW, hx = idx % kWinW;
+        const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
+        s_t[hy][hx] = (qx >= 0 && qx < V.width && qy >= 0 && qy < V.height)
+                          ? plane[(long long)qy * V.width + qx]
+                          : 0.f;
+      }
+      __syncthreads();
+      row_pass(0, s_t, s_t, p.kernel, s_h);
+      __syncthreads();
+      float bl[kRun];
+      col_pass(s_h, p.kernel, j, r0, bl);
+#pragma unroll
+      for (int k = 0; k < kRun; ++k) {
+        if (pl == 0) d[k] += bl[k];
+        else if (pl == 1) d[k] += 2.f * xs[k] * bl[k];
+        else if (pl == 2) d[k] -= 2.f * bl[k];
+        else if (pl == 3) d[k] += ys[k] * bl[k];
+        else d[k] -= bl[k];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) {
+      const int gy = L.oy + r0 + k;
+      if (gx >= V.width || gy >= V.height) continue;
+      const float difference = xs[k] - ys[k];
+      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+      float g = (1.f - w) * sign * inverse;
+      g -= w * d[k];
+      dL[3 * (V.pixel_base + (long long)gy * V.width + gx) + c] = g;
     }
   }
 }
@@ -199,11 +281,11 @@ __global__ __launch_bounds__(256) void k_loss_reduce(const ViewDesc* __restrict_
   __shared__ float s_l1[256], s_ss[256];
   const int v = blockIdx.x;
   const ViewDesc& V = views[v];
-  const int tiles = V.cols * V.rows;
+  const int tiles = ((V.width + kLW - 1) / kLW) * ((V.height + kLH - 1) / kLH);
   float l1 = 0.f, ssim = 0.f;
   for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
-    l1 += block_sums[2 * (V.tile_base + t)];
-    ssim += block_sums[2 * (V.tile_base + t) + 1];
+    l1 += block_sums[2 * (V.lt_base + t)];
+    ssim += block_sums[2 * (V.lt_base + t) + 1];
   }
   s_l1[threadIdx.x] = l1;
   s_ss[threadIdx.x] = ssim;
@@ -223,19 +305,17 @@ __global__ __launch_bounds__(256) void k_loss_reduce(const ViewDesc* __restrict_
 
 }  // namespace
 
-hipError_t launch_loss_forward(const LossParams& p, const ViewDesc* views, const float* rgb, const float* target,
-                               float* dL, float* terms, long long total_px, float* block_sums, hipStream_t stream) {
-  if (p.total_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_loss_forward, dim3(p.total_tiles), dim3(256), 0, stream, p, views, rgb, target, dL, terms,
-                     total_px, block_sums);
-  return hipGetLastError();
-}
+int loss_tiles(int width, int height) { return ((width + kLW - 1) / kLW) * ((height + kLH - 1) / kLH); }
 
-hipError_t launch_loss_backward(const LossParams& p, const ViewDesc* views, const float* rgb, const float* target,
-                                const float* terms, long long total_px, float* dL, hipStream_t stream) {
-  if (p.total_tiles <= 0 || !(p.ssim_weight > 0.f)) return hipSuccess;
-  hipLaunchKernelGGL(k_loss_backward, dim3(p.total_tiles), dim3(256), 0, stream, p, views, rgb, target, terms,
-                     total_px, dL);
+hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* rgb, const float* targets,
+                       float* dL, float* terms, long long total_px, float* block_sums, hipStream_t stream) {
+  if (p.total_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_loss_stats, dim3(p.total_tiles), dim3(256), 0, stream, p, views, rgb, targets, dL, terms,
+                     total_px, block_sums);
+  SPLAT_HIP_CHECK(hipGetLastError());
+  if (!(p.ssim_weight > 0.f)) return hipSuccess;
+  hipLaunchKernelGGL(k_loss_grad, dim3(p.total_tiles), dim3(256), 0, stream, p, views, rgb, targets, terms, total_px,
+                     dL);
   return hipGetLastError();
 }
 

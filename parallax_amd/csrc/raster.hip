@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
             const float my = dL_dweight * (-gy * b.x - gx * a.w);
             val[4] = -mx;
             val[5] = -my;
-            val[6] = __builtin_sqrtf(mx * mx + my * my);
+            val[6] = __builtin_amdgcn_sqrtf(mx * mx + my * my);  // densify statistic only: hardware sqrt
             val[7] = -0.5f * dL_dweight * gx * dx;
             val[8] = -dL_dweight * gx * dy;
             val[9] = -0.5f * dL_dweight * gy * dy;

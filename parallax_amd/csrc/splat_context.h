@@ -49,7 +49,6 @@ struct splat_context {
   std::vector<long long> target_offset;
   std::vector<bool> has_target;
   DeviceBuffer targets;
-  DeviceBuffer batch_targets;
 
   // the current batch
   splat_raster_options opts{};
@@ -57,6 +56,7 @@ struct splat_context {
   std::vector<splat::ViewDesc> batch_desc;
   DeviceBuffer d_views;
   int total_tiles = 0;
+  int loss_tiles = 0;
   long long total_px = 0;
   int sh_use = 0;
   long long M = 0;

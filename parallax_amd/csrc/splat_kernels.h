@@ -47,11 +47,15 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
 
 // ---- binning (binning.hip) ----
 constexpr int kBinSegment = 1024;  // splats per segment of a super-tile list
+constexpr int kStBits = 20;        // super-tile id bits of a super-tile key
+constexpr uint32_t kStMask = (1u << kStBits) - 1u;
 
-// For each depth-sorted slot s: writes one (global super-tile id, slot) pair
-// per super-tile of its rect at soffsets[s], and pair_base[slot] = offsets[s].
+// For each depth-sorted slot s: writes one (super-tile key, slot) pair per
+// super-tile of its rect at soffsets[s] (*s_total: all pairs); a key holds the
+// global super-tile id in its low kStBits bits and the splat's rect inside
+// that super-tile above them. With pair_base: pair_base[slot] = offsets[s].
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* scounts, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
                              long long m, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base, hipStream_t stream);
 // sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted pairs.
 hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, uint32_t* sranges, int n_st,
@@ -64,15 +68,14 @@ hipError_t launch_segment_list(const uint32_t* nseg, const uint32_t* seg_base, i
 // Per (super-tile, tile, segment) counts into table[0 .. 64*ns_max] (the last
 // entry and those of missing segments zeroed); *ns_total = real segments
 // (device), ns_max >= *ns_total.
-hipError_t launch_tile_count(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
+hipError_t launch_tile_count(const uint32_t* skeys, const uint32_t* sranges,
                              const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
-                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, uint32_t* table,
-                             hipStream_t stream);
+                             const uint32_t* ns_total, uint32_t ns_max, uint32_t* table, hipStream_t stream);
 // Every tile's list, in depth order, into pair_vals at the runs `scan` (the
 // exclusive scan of the count table) assigns.
-hipError_t launch_tile_write(const ViewDesc* views, int n_views, const uint32_t* svals, const uint32_t* sranges,
-                             const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
-                             const uint32_t* ns_total, uint32_t ns_max, const SplatRec* rec, const uint32_t* scan,
+hipError_t launch_tile_write(const uint32_t* skeys, const uint32_t* svals,
+                             const uint32_t* sranges, const uint32_t* nseg, const uint32_t* seg_base,
+                             const uint32_t* seg_st, const uint32_t* ns_total, uint32_t ns_max, const uint32_t* scan,
                              uint32_t* pair_vals, hipStream_t stream);
 // ranges[2*t] / ranges[2*t+1] = [begin, end) of global tile t in pair_vals.
 hipError_t launch_tile_ranges_table(const ViewDesc* views, int n_views, int total_tiles, const uint32_t* scan,
@@ -106,15 +109,15 @@ struct LossParams {
   float kernel[11];
   float ssim_weight;
   int n_views;
-  int total_tiles;  // 16x16 tiles over all views
+  int total_tiles;  // loss tiles (64x16 pixels) over all views
 };
-// First pass: SSIM statistics, L1 gradient and the five term planes; per-block
-// partial sums of L1 and SSIM into block_sums (2 floats per block).
-hipError_t launch_loss_forward(const LossParams& p, const ViewDesc* views, const float* rgb, const float* target,
-                               float* dL, float* terms, long long total_px, float* block_sums, hipStream_t stream);
-// Second pass: blurs the term planes and finishes dL/drgb.
-hipError_t launch_loss_backward(const LossParams& p, const ViewDesc* views, const float* rgb, const float* target,
-                                const float* terms, long long total_px, float* dL, hipStream_t stream);
+// L1 + SSIM loss terms and dL/drgb of every view of the batch (two passes
+// over 64x16-pixel loss tiles; `terms` holds 15 floats per pixel between
+// them); per-loss-tile partial sums of L1 and SSIM into block_sums (2 floats
+// per loss tile). Each view's target starts at targets + views[v].target_base.
+int loss_tiles(int width, int height);
+hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* rgb, const float* targets,
+                       float* dL, float* terms, long long total_px, float* block_sums, hipStream_t stream);
 // Fixed-order reduction of the block sums into one (l1, ssim) pair per view.
 hipError_t launch_loss_reduce(const ViewDesc* views, int n_views, const float* block_sums, float* out,
                               hipStream_t stream);
