@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--deterministic", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="batched leg as a plain launch sequence")
     ap.add_argument("--profile", action="store_true",
                     help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
     return ap.parse_args()
@@ -103,7 +104,8 @@ def main():
         it += 1
         return out
 
-    # ---- batched ----
+    # ---- batched: the whole step replayed from a captured HIP graph ----
+    ctx.set_graph_mode(not args.no_graph)
     for w in range(args.warmup):
         run(batches[w % len(batches)])
     ctx.synchronize()
@@ -118,7 +120,8 @@ def main():
         print(json.dumps({"profile": True, "batched_ms_per_step": round(batched_s * 1e3, 3), "batch_views": B}))
         return
 
-    # ---- plain per-view ----
+    # ---- plain per-view launch sequence ----
+    ctx.set_graph_mode(False)
     plain_steps = max(args.steps, 2 * B)
     for w in range(args.warmup):
         run([w % len(views)])
@@ -203,7 +206,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{n} SH3 gaussians, 1920x1080, batch of {B} views from a 64-camera orbit",
                    "gaussians": n, "width": 1920, "height": 1080, "batch_views": B, "sh_degree": 3,
-                   "deterministic": bool(args.deterministic), "parallelism": "single GPU"},
+                   "deterministic": bool(args.deterministic), "graph_captured_step": not args.no_graph,
+                   "parallelism": "single GPU"},
         "plain_steps_per_s": round(1.0 / plain_s, 3),
         "batched_vs_plain_per_view": round(plain_s / (batched_s / B), 3),
         "gaussian_pixels_per_s": float(f"{n * px / batched_s:.4g}"),

@@ -22,7 +22,7 @@
  *   rasterizeSplatsBackward  SplatRaster.h:138-142       splat_render_backward (+ splat_set_image_gradient,
  *                                                        splat_download_gradients, splat_zero_gradients)
  *   SplatTrainer applyGradients  SplatTrainer.cpp:218-256 splat_adam_step
- *   SplatBackend::renderAndAccumulate + Adam              splat_train_step
+ *   SplatBackend::renderAndAccumulate + Adam              splat_train_step (+ splat_set_graph_mode)
  *     SplatTrainer.cpp:337-349
  *   densifyAndPrune + remapState                          splat_densify
  *     SplatDensify.h:55-57, SplatTrainer.cpp:267-299
@@ -188,6 +188,15 @@ splat_status splat_adam_step(splat_context* ctx, const splat_adam_params* params
 splat_status splat_train_step(splat_context* ctx, int32_t n_views, const int32_t* view_ids,
                               const splat_raster_options* options, float ssim_weight,
                               const splat_adam_params* adam, float* losses);
+/* enabled != 0: splat_train_step replays the whole step (forward, loss,
+ * adjoint, Adam) as a captured HIP graph, one per batch shape and options,
+ * with no host synchronisation inside the step. Pair buffers then run at
+ * capacities learnt from earlier steps (the first step runs plain); a step
+ * that outgrows them is detected on the device and redone. Results equal the
+ * plain launch sequence's. No counterpart in the engine, whose Vulkan loop
+ * records its command buffers per iteration (VulkanSplatTrainer.cpp:748-861).
+ * Default: off. */
+splat_status splat_set_graph_mode(splat_context* ctx, int32_t enabled);
 
 /* ---- adaptive density ---- */
 /* densifyAndPrune + remapState on the device. rng_state is advanced exactly

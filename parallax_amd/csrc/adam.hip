@@ -5,7 +5,8 @@
 // shared betas and bias corrections; the quaternion is renormalised after its
 // update (identity if its norm collapses). Also folds in accumulateStatistics
 // (:258-263) for the densify signal. Elementwise over the capacity-laid
-// parameter buffer; grid.y selects the group. HBM-bound: 28 B/param of
+// parameter buffer, four elements per thread (16-byte accesses where the
+// group is aligned). HBM-bound: 28 B/param of
 // compulsory traffic (read p, g, m, v; write p, m, v) plus the 4 B gradient
 // clear. Arithmetic is IEEE float without contraction, so with equal
 // gradients the update matches the CPU trainer bit for bit.
@@ -15,7 +16,7 @@
 namespace splat {
 namespace {
 
-__device__ __forceinline__ float adam_delta(float& m, float& v, float g, float lr, const AdamParams& p) {
+__device__ __forceinline__ float adam_delta(float& m, float& v, float g, float lr, const AdamStep& p) {
   m = p.beta1 * m + (1.f - p.beta1) * g;
   v = p.beta2 * v + (1.f - p.beta2) * g * g;
   const float m_hat = m / p.c1;
@@ -23,23 +24,53 @@ __device__ __forceinline__ float adam_delta(float& m, float& v, float g, float l
   return -lr * m_hat / (__builtin_sqrtf(v_hat) + p.eps);
 }
 
-// groups: 0 pos (3), 1 log_scale (3), 2 rot (4, per gaussian), 3 opacity (1),
-// 4 dc (3), 5 rest (rs)
-__global__ __launch_bounds__(256) void k_adam(AdamParams p, float* __restrict__ params, float* __restrict__ grads,
-                                              float* __restrict__ m, float* __restrict__ v, float* __restrict__ sg,
-                                              int* __restrict__ vc, float* __restrict__ stat_sg,
-                                              int* __restrict__ stat_vc) {
-  const int group = blockIdx.y;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long cap = p.cap, n = p.n;
-  if (group == 2) {
-    if (i >= n) return;
-    const long long o = 6 * cap + 4 * i;
+// Groups: 0 pos (3n), 1 log_scale (3n), 2 rot (4n, one quaternion per
+// thread), 3 opacity (n), 4 dc (3n), 5 rest (rs n). Each group's elements are
+// padded to a multiple of 4 in a virtual index space that one flat grid
+// covers with 4 elements per thread, so no workgroup is launched idle.
+struct Groups {
+  long long off[6], len[6], vbase[7];
+};
+
+__device__ __forceinline__ Groups groups_of(const AdamShape& s) {
+  Groups G;
+  const long long cap = s.cap, n = s.n;
+  const long long off[6] = {0, 3 * cap, 6 * cap, 10 * cap, 11 * cap, 14 * cap};
+  const long long len[6] = {3 * n, 3 * n, 4 * n, n, 3 * n, (long long)s.rest_stride * n};
+  G.vbase[0] = 0;
+#pragma unroll
+  for (int g = 0; g < 6; ++g) {
+    G.off[g] = off[g];
+    G.len[g] = len[g];
+    G.vbase[g + 1] = G.vbase[g] + ((len[g] + 3) & ~3LL);
+  }
+  return G;
+}
+
+__global__ __launch_bounds__(256) void k_adam(AdamShape shape, const AdamStep* __restrict__ step,
+                                              const uint32_t* __restrict__ skip, float* __restrict__ params,
+                                              float* __restrict__ grads, float* __restrict__ m,
+                                              float* __restrict__ v, float* __restrict__ sg, int* __restrict__ vc,
+                                              float* __restrict__ stat_sg, int* __restrict__ stat_vc) {
+  if (skip && *skip) return;
+  const AdamStep P = *step;
+  const Groups G = groups_of(shape);
+  const long long v0 = 4 * ((long long)blockIdx.x * blockDim.x + threadIdx.x);
+  if (v0 >= G.vbase[6]) return;
+  int g = 0;
+#pragma unroll
+  for (int k = 1; k < 6; ++k)
+    if (v0 >= G.vbase[k]) g = k;
+  const long long local = v0 - G.vbase[g];
+  const float lr = P.lr[g];
+  const long long o = G.off[g] + local;
+  if (g == 2) {
+    const long long i = local / 4;  // gaussian
     float q[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       float mm = m[o + a], vv = v[o + a];
-      q[a] = params[o + a] + adam_delta(mm, vv, grads[o + a], p.lr[2], p);
+      q[a] = params[o + a] + adam_delta(mm, vv, grads[o + a], lr, P);
       m[o + a] = mm;
       v[o + a] = vv;
       grads[o + a] = 0.f;
@@ -62,32 +93,44 @@ __global__ __launch_bounds__(256) void k_adam(AdamParams p, float* __restrict__ 
     vc[i] = 0;
     return;
   }
-  long long off, count;
-  switch (group) {
-    case 0: off = 0; count = 3 * n; break;
-    case 1: off = 3 * cap; count = 3 * n; break;
-    case 3: off = 10 * cap; count = n; break;
-    case 4: off = 11 * cap; count = 3 * n; break;
-    default: off = 14 * cap; count = (long long)p.rest_stride * n; break;
+  const long long count = G.len[g] - local;  // elements of this group left from here
+  if (count >= 4 && (o & 3) == 0) {
+    float4 pp = *reinterpret_cast<const float4*>(params + o);
+    const float4 gg = *reinterpret_cast<const float4*>(grads + o);
+    float4 mm = *reinterpret_cast<const float4*>(m + o);
+    float4 vv = *reinterpret_cast<const float4*>(v + o);
+    pp.x += adam_delta(mm.x, vv.x, gg.x, lr, P);
+    pp.y += adam_delta(mm.y, vv.y, gg.y, lr, P);
+    pp.z += adam_delta(mm.z, vv.z, gg.z, lr, P);
+    pp.w += adam_delta(mm.w, vv.w, gg.w, lr, P);
+    *reinterpret_cast<float4*>(params + o) = pp;
+    *reinterpret_cast<float4*>(m + o) = mm;
+    *reinterpret_cast<float4*>(v + o) = vv;
+    *reinterpret_cast<float4*>(grads + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
   }
-  if (i >= count) return;
-  const long long k = off + i;
-  float mm = m[k], vv = v[k];
-  params[k] += adam_delta(mm, vv, grads[k], p.lr[group], p);
-  m[k] = mm;
-  v[k] = vv;
-  grads[k] = 0.f;
+  const int todo = count < 4 ? (int)count : 4;
+  for (int a = 0; a < todo; ++a) {
+    float mm = m[o + a], vv = v[o + a];
+    params[o + a] += adam_delta(mm, vv, grads[o + a], lr, P);
+    m[o + a] = mm;
+    v[o + a] = vv;
+    grads[o + a] = 0.f;
+  }
 }
 
 }  // namespace
 
-hipError_t launch_adam(const AdamParams& p, float* params, float* grads, float* m, float* v, float* sg, int* vc,
-                       float* stat_sg, int* stat_vc, hipStream_t stream) {
-  if (p.n <= 0) return hipSuccess;
-  long long widest = 4LL * p.n;
-  if ((long long)p.rest_stride * p.n > widest) widest = (long long)p.rest_stride * p.n;
-  const unsigned blocks = (unsigned)((widest + 255) / 256);
-  hipLaunchKernelGGL(k_adam, dim3(blocks, 6), dim3(256), 0, stream, p, params, grads, m, v, sg, vc, stat_sg, stat_vc);
+hipError_t launch_adam(const AdamShape& s, const AdamStep* step, const uint32_t* skip_flag, float* params,
+                       float* grads, float* m, float* v, float* sg, int* vc, float* stat_sg, int* stat_vc,
+                       hipStream_t stream) {
+  if (s.n <= 0) return hipSuccess;
+  const long long n = s.n;
+  const long long virt = ((3 * n + 3) & ~3LL) * 3 + 4 * n + ((n + 3) & ~3LL) + (((long long)s.rest_stride * n + 3) & ~3LL);
+  const long long threads = virt / 4;
+  const unsigned blocks = (unsigned)((threads + 255) / 256);
+  hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(256), 0, stream, s, step, skip_flag, params, grads, m, v, sg, vc,
+                     stat_sg, stat_vc);
   return hipGetLastError();
 }
 

@@ -84,7 +84,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
                                                  const float* __restrict__ screen,
                                                  const uint32_t* __restrict__ pair_base,
                                                  const uint32_t* __restrict__ counts,
-                                                 const float* __restrict__ partial, float* __restrict__ grads,
+                                                 const float* __restrict__ partial, uint32_t p_cap,
+                                                 float* __restrict__ grads,
                                                  float* __restrict__ sg, int* __restrict__ vc) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.n) return;
@@ -149,7 +150,9 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
 #pragma unroll
       for (int k = 0; k < 11; ++k) acc[k] = 0.f;
       const uint32_t base = pair_base[slot], cnt = counts[slot];
-      for (uint32_t q = 0; q < cnt; ++q) {
+      // past p_cap only when the step's capacity check has flagged it
+      const uint32_t stop = base < p_cap ? (cnt < p_cap - base ? cnt : p_cap - base) : 0u;
+      for (uint32_t q = 0; q < stop; ++q) {
         const float* src = partial + (long long)(base + q) * kScreenStride;
 #pragma unroll
         for (int k = 0; k < 11; ++k) acc[k] += src[k];
@@ -291,7 +294,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
 
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
                           const float* screen, const uint32_t* pair_base, const uint32_t* counts,
-                          const float* partial, float* grads, float* sg, int* vc, hipStream_t stream) {
+                          const float* partial, uint32_t p_cap, float* grads, float* sg, int* vc,
+                          hipStream_t stream) {
   if (p.n <= 0) return hipSuccess;
   if (3 * (p.sh_use == 0 ? 0 : (p.sh_use == 1 ? 3 : (p.sh_use == 2 ? 8 : 15))) > p.rest_stride)
     return hipErrorInvalidValue;
@@ -299,19 +303,19 @@ hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const f
   switch (p.sh_use) {
     case 0:
       hipLaunchKernelGGL(k_adjoint<0>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, grads, sg, vc);
+                         partial, p_cap, grads, sg, vc);
       break;
     case 1:
       hipLaunchKernelGGL(k_adjoint<1>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, grads, sg, vc);
+                         partial, p_cap, grads, sg, vc);
       break;
     case 2:
       hipLaunchKernelGGL(k_adjoint<2>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, grads, sg, vc);
+                         partial, p_cap, grads, sg, vc);
       break;
     default:
       hipLaunchKernelGGL(k_adjoint<3>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, grads, sg, vc);
+                         partial, p_cap, grads, sg, vc);
       break;
   }
   return hipGetLastError();

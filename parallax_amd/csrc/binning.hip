@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ s_total,
                                                     const SplatRec* __restrict__ rec,
                                                     const ViewDesc* __restrict__ views, long long per_view,
-                                                    long long m, uint32_t* __restrict__ skeys,
+                                                    long long m, uint32_t s_cap, uint32_t* __restrict__ skeys,
                                                     uint32_t* __restrict__ svals, uint32_t* __restrict__ pair_base) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
     const int obase = __shfl(st_base, owner, 64);
     const int ocols = __shfl(scols, owner, 64);
     const uint32_t oslot = __shfl(slot, owner, 64);
-    if (i < stop) {
+    if (i < stop && i < s_cap) {  // past s_cap: the capacity check has flagged the step
       const uint32_t local = i - o_off;
       const int sy = oy0 / kSuper + (int)(local / (uint32_t)ow);
       const int sx = ox0 / kSuper + (int)(local % (uint32_t)ow);
@@ -112,9 +112,11 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
   }
 }
 
-__global__ __launch_bounds__(256) void k_super_ranges(const uint32_t* __restrict__ keys, long long n,
+__global__ __launch_bounds__(256) void k_super_ranges(const uint32_t* __restrict__ keys, long long n_cap,
+                                                      const uint32_t* __restrict__ n_dev,
                                                       uint32_t* __restrict__ ranges, int n_st) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
   if (i >= n) return;
   const uint32_t k = keys[i] & kStMask;
   if (k >= (uint32_t)n_st) return;  // cannot happen for keys the emitter wrote
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ seg_base,
                                                     const uint32_t* __restrict__ seg_st,
                                                     const uint32_t* __restrict__ ns_total,
-                                                    const uint32_t* __restrict__ scan,
+                                                    const uint32_t* __restrict__ scan, uint32_t p_cap,
                                                     uint32_t* __restrict__ pair_vals) {
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
@@ -247,9 +249,9 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
         const bool mine = (mask >> t) & 1ull;
         const uint64_t b = __ballot(mine);
         const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
-        if (mine) pair_vals[at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] =
-            slot;
+        const uint32_t to =
+            at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+        if (mine && to < p_cap) pair_vals[to] = slot;  // past p_cap: the step is flagged
         if (lane == t) base += (uint32_t)__popcll(b);
       }
     }
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
 __global__ __launch_bounds__(256) void k_tile_ranges_table(const ViewDesc* __restrict__ views, int n_views,
                                                            int total_tiles, const uint32_t* __restrict__ scan,
                                                            const uint32_t* __restrict__ nseg,
-                                                           const uint32_t* __restrict__ seg_base,
+                                                           const uint32_t* __restrict__ seg_base, uint32_t p_cap,
                                                            uint32_t* __restrict__ ranges) {
   const int tile = blockIdx.x * blockDim.x + threadIdx.x;
   if (tile >= total_tiles) return;
@@ -273,26 +275,38 @@ __global__ __launch_bounds__(256) void k_tile_ranges_table(const ViewDesc* __res
   const int st = V.st_base + (ty / kSuper) * V.scols + tx / kSuper;
   const int t = (ty % kSuper) * kSuper + tx % kSuper;
   const uint64_t i0 = 64ull * seg_base[st] + (uint64_t)t * nseg[st];
-  ranges[2 * (long long)tile] = scan[i0];
-  ranges[2 * (long long)tile + 1] = scan[i0 + nseg[st]];
+  ranges[2 * (long long)tile] = min(scan[i0], p_cap);
+  ranges[2 * (long long)tile + 1] = min(scan[i0 + nseg[st]], p_cap);
+}
+
+// totals[0] = tile pairs, totals[1] = super-tile pairs; totals[3] = 1 when
+// either exceeds the buffers the step was sized for.
+__global__ void k_check_caps(uint32_t* totals, uint32_t p_cap, uint32_t s_cap) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) totals[3] = (totals[0] > p_cap || totals[1] > s_cap) ? 1u : 0u;
 }
 
 }  // namespace
 
-hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
-                             long long m, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base, hipStream_t stream) {
-  if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
-                     soffsets, s_total, rec, views, per_view, m, skeys, svals, pair_base);
+hipError_t launch_check_caps(uint32_t* totals, uint32_t p_cap, uint32_t s_cap, hipStream_t stream) {
+  hipLaunchKernelGGL(k_check_caps, dim3(1), dim3(64), 0, stream, totals, p_cap, s_cap);
   return hipGetLastError();
 }
 
-hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, uint32_t* sranges, int n_st,
-                               hipStream_t stream) {
+hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
+                             const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             long long m, uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
+                             hipStream_t stream) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
+                     soffsets, s_total, rec, views, per_view, m, s_cap, skeys, svals, pair_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, const uint32_t* n_dev, uint32_t* sranges,
+                               int n_st, hipStream_t stream) {
   SPLAT_HIP_CHECK(launch_fill_u32(sranges, 0u, 2LL * n_st, stream));
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_super_ranges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sorted_keys, n,
+  hipLaunchKernelGGL(k_super_ranges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sorted_keys, n, n_dev,
                      sranges, n_st);
   return hipGetLastError();
 }
@@ -335,19 +349,19 @@ hipError_t launch_tile_count(const uint32_t* skeys, const uint32_t* sranges,
 hipError_t launch_tile_write(const uint32_t* skeys, const uint32_t* svals,
                              const uint32_t* sranges, const uint32_t* nseg, const uint32_t* seg_base,
                              const uint32_t* seg_st, const uint32_t* ns_total, uint32_t ns_max, const uint32_t* scan,
-                             uint32_t* pair_vals, hipStream_t stream) {
+                             uint32_t p_cap, uint32_t* pair_vals, hipStream_t stream) {
   if (ns_max == 0) return hipSuccess;
   hipLaunchKernelGGL(k_tile_write, dim3(segment_grid(ns_max)), dim3(256), 0, stream, skeys, svals,
-                     sranges, nseg, seg_base, seg_st, ns_total, scan, pair_vals);
+                     sranges, nseg, seg_base, seg_st, ns_total, scan, p_cap, pair_vals);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_ranges_table(const ViewDesc* views, int n_views, int total_tiles, const uint32_t* scan,
-                                    const uint32_t* nseg, const uint32_t* seg_base, uint32_t* ranges,
+                                    const uint32_t* nseg, const uint32_t* seg_base, uint32_t p_cap, uint32_t* ranges,
                                     hipStream_t stream) {
   if (total_tiles <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_tile_ranges_table, dim3((unsigned)((total_tiles + 255) / 256)), dim3(256), 0, stream, views,
-                     n_views, total_tiles, scan, nseg, seg_base, ranges);
+                     n_views, total_tiles, scan, nseg, seg_base, p_cap, ranges);
   return hipGetLastError();
 }
 

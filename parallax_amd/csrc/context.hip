@@ -3,6 +3,7 @@
 // ranges -> forward -> loss -> adjoint -> per-gaussian adjoint -> Adam).
 // Host code only; the kernels live in project/sort/raster/loss/adjoint/adam.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -74,8 +75,14 @@ int bit_width(unsigned long long x) {
 }  // namespace
 
 // ------------------------------------------------------------- buffers --
+namespace {
+std::atomic<unsigned long long> g_alloc_generation{1};
+}
+unsigned long long alloc_generation() { return g_alloc_generation.load(); }
+
 hipError_t DeviceBuffer::reserve(size_t want) {
   if (want <= bytes) return hipSuccess;
+  g_alloc_generation.fetch_add(1);
   if (ptr) {
     hipError_t e = hipFree(ptr);
     ptr = nullptr;
@@ -93,12 +100,21 @@ hipError_t DeviceBuffer::reserve(size_t want) {
 }
 
 void DeviceBuffer::release() {
+  if (ptr) g_alloc_generation.fetch_add(1);
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
 }
 
+void splat_context::drop_graphs() {
+  for (Graph& g : graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  graphs.clear();
+}
+
 splat_context::~splat_context() {
+  drop_graphs();
+  if (readback) (void)hipHostFree(readback);
   for (DeviceBuffer* b : all_buffers()) b->release();
   for (auto& kv : timers) {
     for (auto& pr : kv.second.pending) {
@@ -114,19 +130,19 @@ std::vector<DeviceBuffer*> splat_context::all_buffers() {
           &vals_alt, &counts, &offsets, &pair_base, &scounts, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
           &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &scan_ws, &hist_ws, &scalars,
-          &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src};
+          &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam};
 }
 
 // ------------------------------------------------------------- timing --
 hipError_t splat_context::timed_begin(const char* name, hipEvent_t* start) {
-  if (!timing) return hipSuccess;
+  if (!timing || capturing) return hipSuccess;
   SPLAT_HIP_CHECK(hipEventCreate(start));
   (void)name;
   return hipEventRecord(*start, stream);
 }
 
 hipError_t splat_context::timed_end(const char* name, hipEvent_t start) {
-  if (!timing) return hipSuccess;
+  if (!timing || capturing) return hipSuccess;
   hipEvent_t stop;
   SPLAT_HIP_CHECK(hipEventCreate(&stop));
   SPLAT_HIP_CHECK(hipEventRecord(stop, stream));
@@ -167,7 +183,9 @@ splat_status check_ready(splat_context* ctx) {
   return SPLAT_OK;
 }
 
-splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_ids, const splat_raster_options* o) {
+// Host side of a batch: descriptors, totals, and every buffer whose size does
+// not depend on the pair counts. No device work but allocation.
+splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_ids, const splat_raster_options* o) {
   if (n_views <= 0 || n_views > kMaxBatchViews || !view_ids || !o)
     return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: 1..64 views and options required");
   for (int k = 0; k < n_views; ++k)
@@ -191,6 +209,7 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
     px += (long long)d.width * d.height;
     ctx->batch_desc.push_back(d);
   }
+  if (supers > (int)kStMask) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: batch too large");
   ctx->total_tiles = tiles;
   ctx->loss_tiles = ltiles;
   ctx->n_st = supers;
@@ -201,10 +220,15 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   const long long n = ctx->n;
   const long long M = n * n_views;
   ctx->M = M;
+  RasterParams rp{};
+  for (int c = 0; c < 3; ++c) rp.bg[c] = o->background[c];
+  rp.min_alpha = o->min_alpha;
+  rp.cutoff = o->transmittance_cutoff;
+  rp.n_views = n_views;
+  rp.total_tiles = tiles;
+  rp.per_view = n;
+  ctx->raster = rp;
   CK(ctx->d_views.reserve(sizeof(ViewDesc) * n_views), "alloc views");
-  CK(hipMemcpyAsync(ctx->d_views.ptr, ctx->batch_desc.data(), sizeof(ViewDesc) * n_views, hipMemcpyHostToDevice,
-                    ctx->stream),
-     "upload views");
   CK(ctx->rgb.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc image");
   CK(ctx->T.reserve(sizeof(float) * std::max(1LL, px)), "alloc image");
   CK(ctx->last.reserve(sizeof(int) * std::max(1LL, px)), "alloc image");
@@ -225,144 +249,205 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   CK(ctx->nseg.reserve(sizeof(uint32_t) * std::max(1, supers)), "alloc segments");
   CK(ctx->seg_base.reserve(sizeof(uint32_t) * std::max(1, supers)), "alloc segments");
   CK(ctx->ensure_sort_workspace(std::max(M, 256LL * 4096)), "alloc sort workspace");
+  return SPLAT_OK;
+}
 
+splat_status upload_batch(splat_context* ctx) {
+  CK(hipMemcpyAsync(ctx->d_views.ptr, ctx->batch_desc.data(), sizeof(ViewDesc) * ctx->batch_desc.size(),
+                    hipMemcpyHostToDevice, ctx->stream),
+     "upload views");
+  return SPLAT_OK;
+}
+
+// Buffers sized by the pair counts: tile pairs p_cap, super-tile pairs s_cap.
+splat_status reserve_pairs(splat_context* ctx, unsigned long long p_cap, unsigned long long s_cap) {
+  p_cap = std::max(1ULL, p_cap);
+  s_cap = std::max(1ULL, s_cap);
+  if (p_cap > 0xFFFFFFF0ULL || s_cap > 0xFFFFFFF0ULL)
+    return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: more than 2^32 pairs in one batch");
+  ctx->P_cap = p_cap;
+  ctx->S_cap = s_cap;
+  // segments: at most ceil(len / kBinSegment) per super-tile
+  const unsigned long long ns_max = s_cap / kBinSegment + (unsigned long long)ctx->n_st;
+  CK(ctx->pair_vals.reserve(4 * p_cap), "alloc pairs");
+  CK(ctx->skeys.reserve(4 * s_cap), "alloc super pairs");
+  CK(ctx->svals.reserve(4 * s_cap), "alloc super pairs");
+  CK(ctx->skeys_alt.reserve(4 * s_cap), "alloc super pairs");
+  CK(ctx->svals_alt.reserve(4 * s_cap), "alloc super pairs");
+  CK(ctx->seg_st.reserve(4 * ns_max), "alloc segments");
+  CK(ctx->bin_table.reserve(4 * (64ull * ns_max + 1)), "alloc binning table");
+  CK(ctx->ensure_sort_workspace(std::max<long long>(std::max<long long>((long long)s_cap, ctx->M),
+                                                    (long long)(64ull * ns_max + 1))),
+     "alloc sort workspace");
+  if (ctx->opts.deterministic)
+    CK(ctx->partial.reserve(sizeof(float) * kScreenStride * p_cap), "alloc partials");
+  else
+    CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, ctx->M)), "alloc screen");
+  return SPLAT_OK;
+}
+
+// Projection, depth sort and the pair counts: scalars[0] = tile pairs P,
+// scalars[1] = super-tile pairs S; scalars[3] (overflow) cleared.
+splat_status enqueue_front(splat_context* ctx) {
+  const long long n = ctx->n, M = ctx->M;
+  const int n_views = (int)ctx->batch_desc.size();
+  const splat_raster_options* o = &ctx->opts;
+  uint32_t* d_total = ctx->scalars.u();
+  CK(launch_fill_u32(d_total, 0u, 4, ctx->stream), "clear counters");
   TIMED("project", launch_project(ctx->params.f(), ctx->cap, (int)n, ctx->rs, ctx->sh_use, ctx->views_dev(), n_views,
                                   o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(), ctx->vals.u(),
                                   ctx->counts.u(), ctx->scounts.u(), ctx->stream));
   bool alt = false;
   TIMED("depth_sort", launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
                                         ctx->sort_ws, &alt, ctx->stream));
-  uint32_t* sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
-  // scalars: [0] tile pairs P, [1] super-tile pairs S, [2] segments
+  ctx->sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
+  TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), ctx->sorted_slots, ctx->offsets.u(), M, d_total,
+                                           ctx->sort_ws.scan, ctx->stream));
+  TIMED("pair_scan", launch_exclusive_scan(ctx->scounts.u(), ctx->sorted_slots, ctx->soffsets.u(), M, d_total + 1,
+                                           ctx->sort_ws.scan, ctx->stream));
+  return SPLAT_OK;
+}
+
+// Binning within the reserved capacities (the pair counts are read on the
+// device), then forward compositing.
+splat_status enqueue_back(splat_context* ctx) {
+  const long long n = ctx->n, M = ctx->M;
+  const int n_views = (int)ctx->batch_desc.size();
+  const int supers = ctx->n_st, tiles = ctx->total_tiles;
+  const uint32_t p_cap = (uint32_t)ctx->P_cap, s_cap = (uint32_t)ctx->S_cap;
+  const uint32_t ns_max = (uint32_t)(ctx->S_cap / kBinSegment + (unsigned long long)supers);
   uint32_t* d_total = ctx->scalars.u();
-  TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), sorted_slots, ctx->offsets.u(), M, d_total,
-                                           ctx->sort_ws.scan, ctx->stream));
-  TIMED("pair_scan", launch_exclusive_scan(ctx->scounts.u(), sorted_slots, ctx->soffsets.u(), M, d_total + 1,
-                                           ctx->sort_ws.scan, ctx->stream));
-  uint32_t totals[2] = {0, 0};
-  if (M > 0) {
-    CK(hipMemcpyAsync(totals, d_total, sizeof(totals), hipMemcpyDeviceToHost, ctx->stream), "read pair count");
-    CK(hipStreamSynchronize(ctx->stream), "sync");
-  }
-  const uint32_t P = totals[0];
-  const uint32_t S = totals[1];
-  ctx->P = P;
-  ctx->S = S;
-  const long long Pc = std::max(1LL, (long long)P), Sc = std::max(1LL, (long long)S);
-  // segments: at most ceil(len / kBinSegment) per super-tile
-  const uint32_t ns_max = (uint32_t)(S / kBinSegment + (unsigned long long)supers);
-  CK(ctx->pair_vals.reserve(4 * Pc), "alloc pairs");
-  CK(ctx->skeys.reserve(4 * Sc), "alloc super pairs");
-  CK(ctx->svals.reserve(4 * Sc), "alloc super pairs");
-  CK(ctx->skeys_alt.reserve(4 * Sc), "alloc super pairs");
-  CK(ctx->svals_alt.reserve(4 * Sc), "alloc super pairs");
-  CK(ctx->seg_st.reserve(4 * std::max(1u, ns_max)), "alloc segments");
-  CK(ctx->bin_table.reserve(4 * (64ull * ns_max + 1)), "alloc binning table");
-  CK(ctx->ensure_sort_workspace(std::max<long long>(std::max<long long>(S, M), 64LL * ns_max + 1)),
-     "alloc sort workspace");
-  if (supers > (int)kStMask) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: batch too large");
-  TIMED("emit_super", launch_emit_super(sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), d_total + 1, ctx->recs(),
-                                        ctx->views_dev(), n, M, ctx->skeys.u(), ctx->svals.u(),
-                                        o->deterministic ? ctx->pair_base.u() : nullptr, ctx->stream));
+  CK(launch_check_caps(d_total, p_cap, s_cap, ctx->stream), "capacity check");
+  TIMED("emit_super", launch_emit_super(ctx->sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), d_total + 1,
+                                        ctx->recs(), ctx->views_dev(), n, M, s_cap, ctx->skeys.u(), ctx->svals.u(),
+                                        ctx->opts.deterministic ? ctx->pair_base.u() : nullptr, ctx->stream));
   const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
   bool salt = false;
-  TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(), S, 0,
-                                        st_bits, ctx->sort_ws, &salt, ctx->stream));
+  TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(), s_cap,
+                                        0, st_bits, ctx->sort_ws, &salt, ctx->stream, d_total + 1));
   const uint32_t* skeys = salt ? ctx->skeys_alt.u() : ctx->skeys.u();
   const uint32_t* svals = salt ? ctx->svals_alt.u() : ctx->svals.u();
-  TIMED("segments", launch_super_ranges(skeys, S, ctx->sranges.u(), supers, ctx->stream));
+  TIMED("segments", launch_super_ranges(skeys, s_cap, d_total + 1, ctx->sranges.u(), supers, ctx->stream));
   TIMED("segments", launch_segment_counts(ctx->sranges.u(), supers, ctx->nseg.u(), ctx->stream));
   TIMED("segments", launch_exclusive_scan(ctx->nseg.u(), nullptr, ctx->seg_base.u(), supers, d_total + 2,
                                           ctx->sort_ws.scan, ctx->stream));
   TIMED("segments", launch_segment_list(ctx->nseg.u(), ctx->seg_base.u(), supers, ctx->seg_st.u(), ctx->stream));
-  TIMED("tile_count", launch_tile_count(skeys, ctx->sranges.u(), ctx->nseg.u(),
-                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->bin_table.u(),
-                                        ctx->stream));
+  TIMED("tile_count", launch_tile_count(skeys, ctx->sranges.u(), ctx->nseg.u(), ctx->seg_base.u(), ctx->seg_st.u(),
+                                        d_total + 2, ns_max, ctx->bin_table.u(), ctx->stream));
   TIMED("tile_scan", launch_exclusive_scan(ctx->bin_table.u(), nullptr, ctx->bin_table.u(), 64LL * ns_max + 1,
                                            nullptr, ctx->sort_ws.scan, ctx->stream));
-  TIMED("tile_write", launch_tile_write(skeys, svals, ctx->sranges.u(), ctx->nseg.u(),
-                                        ctx->seg_base.u(), ctx->seg_st.u(), d_total + 2, ns_max, ctx->bin_table.u(),
+  TIMED("tile_write", launch_tile_write(skeys, svals, ctx->sranges.u(), ctx->nseg.u(), ctx->seg_base.u(),
+                                        ctx->seg_st.u(), d_total + 2, ns_max, ctx->bin_table.u(), p_cap,
                                         ctx->pair_vals.u(), ctx->stream));
   TIMED("tile_ranges", launch_tile_ranges_table(ctx->views_dev(), n_views, tiles, ctx->bin_table.u(), ctx->nseg.u(),
-                                                ctx->seg_base.u(), ctx->ranges.u(), ctx->stream));
+                                                ctx->seg_base.u(), p_cap, ctx->ranges.u(), ctx->stream));
   ctx->sorted_pair_vals = ctx->pair_vals.u();
-  ctx->sorted_slots = sorted_slots;
-  RasterParams rp{};
-  for (int c = 0; c < 3; ++c) rp.bg[c] = o->background[c];
-  rp.min_alpha = o->min_alpha;
-  rp.cutoff = o->transmittance_cutoff;
-  rp.n_views = n_views;
-  rp.total_tiles = tiles;
-  rp.per_view = n;
-  ctx->raster = rp;
-  TIMED("forward", launch_forward(rp, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
+  TIMED("forward", launch_forward(ctx->raster, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
                                   ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->stream));
+  return SPLAT_OK;
+}
+
+// The forward of one batch, its pair buffers sized exactly (one host read of
+// the pair counts between projection and binning).
+splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_ids, const splat_raster_options* o) {
+  if (splat_status s = batch_setup(ctx, n_views, view_ids, o)) return s;
+  if (splat_status s = upload_batch(ctx)) return s;
+  if (splat_status s = enqueue_front(ctx)) return s;
+  uint32_t totals[2] = {0, 0};
+  if (ctx->M > 0) {
+    CK(hipMemcpyAsync(totals, ctx->scalars.u(), sizeof(totals), hipMemcpyDeviceToHost, ctx->stream),
+       "read pair count");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+  }
+  ctx->P = totals[0];
+  ctx->S = totals[1];
+  if (splat_status s = reserve_pairs(ctx, totals[0], totals[1])) return s;
+  if (splat_status s = enqueue_back(ctx)) return s;
   ctx->forward_done = true;
   return SPLAT_OK;
 }
 
-splat_status loss_impl(splat_context* ctx, float w, float* losses) {
-  if (!ctx->forward_done) return fail(ctx, SPLAT_ERROR_STATE, "loss: no forward batch");
-  for (int id : ctx->batch_ids)
-    if (!ctx->has_target[id]) return fail(ctx, SPLAT_ERROR_STATE, "loss: a batch view has no target image");
+splat_status reserve_loss(splat_context* ctx, float w) {
   const long long px = ctx->total_px;
   const int V = (int)ctx->batch_ids.size();
   CK(ctx->dL.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc dL");
   CK(ctx->block_sums.reserve(sizeof(float) * 2 * std::max(1, ctx->loss_tiles)), "alloc loss");
-  if (w > 0.f) CK(ctx->terms.reserve(sizeof(float) * 15 * std::max(1LL, px)), "alloc ssim terms");
   CK(ctx->loss_sums.reserve(sizeof(float) * 2 * V), "alloc loss");
-  // targets uploaded after the forward: refresh the views' target offsets
+  if (w > 0.f) CK(ctx->terms.reserve(sizeof(float) * 15 * std::max(1LL, px)), "alloc ssim terms");
+  return SPLAT_OK;
+}
+
+// Targets uploaded after the batch was set up: refresh the views' target
+// offsets. Returns whether the descriptors changed.
+bool refresh_targets(splat_context* ctx) {
   bool stale = false;
-  for (int k = 0; k < V; ++k) {
+  for (size_t k = 0; k < ctx->batch_desc.size(); ++k) {
     const long long want = ctx->target_offset[ctx->batch_ids[k]];
     if (ctx->batch_desc[k].target_base != want) {
       ctx->batch_desc[k].target_base = want;
       stale = true;
     }
   }
-  if (stale)
-    CK(hipMemcpyAsync(ctx->d_views.ptr, ctx->batch_desc.data(), sizeof(ViewDesc) * V, hipMemcpyHostToDevice,
-                      ctx->stream),
-       "upload views");
+  return stale;
+}
+
+splat_status enqueue_loss(splat_context* ctx, float w) {
   LossParams lp{};
   for (int i = 0; i < 11; ++i) lp.kernel[i] = ctx->ssim_kernel[i];
   lp.ssim_weight = w;
-  lp.n_views = V;
+  lp.n_views = (int)ctx->batch_ids.size();
   lp.total_tiles = ctx->loss_tiles;
-  TIMED("loss", launch_loss(lp, ctx->views_dev(), ctx->rgb.f(), ctx->targets.f(), ctx->dL.f(), ctx->terms.f(), px,
-                            ctx->block_sums.f(), ctx->stream));
-  CK(launch_loss_reduce(ctx->views_dev(), V, ctx->block_sums.f(), ctx->loss_sums.f(), ctx->stream), "loss reduce");
+  TIMED("loss", launch_loss(lp, ctx->views_dev(), ctx->rgb.f(), ctx->targets.f(), ctx->dL.f(), ctx->terms.f(),
+                            ctx->total_px, ctx->block_sums.f(), ctx->stream));
+  CK(launch_loss_reduce(ctx->views_dev(), lp.n_views, ctx->block_sums.f(), ctx->loss_sums.f(), ctx->stream),
+     "loss reduce");
+  return SPLAT_OK;
+}
+
+void losses_from_sums(const splat_context* ctx, const float* sums, float w, float* losses) {
+  for (size_t k = 0; k < ctx->batch_desc.size(); ++k) {
+    const ViewDesc& d = ctx->batch_desc[k];
+    const float inverse = 1.f / static_cast<float>((long long)d.width * d.height * 3);
+    const float l1 = sums[2 * k] * inverse;
+    losses[k] = w <= 0.f ? l1 : (1.f - w) * l1 + w * (1.f - sums[2 * k + 1] * inverse);
+  }
+}
+
+splat_status loss_impl(splat_context* ctx, float w, float* losses) {
+  if (!ctx->forward_done) return fail(ctx, SPLAT_ERROR_STATE, "loss: no forward batch");
+  for (int id : ctx->batch_ids)
+    if (!ctx->has_target[id]) return fail(ctx, SPLAT_ERROR_STATE, "loss: a batch view has no target image");
+  if (splat_status s = reserve_loss(ctx, w)) return s;
+  if (refresh_targets(ctx))
+    if (splat_status s = upload_batch(ctx)) return s;
+  if (splat_status s = enqueue_loss(ctx, w)) return s;
   ctx->loss_done = true;
   ctx->last_ssim_weight = w;
   if (losses) {
+    const int V = (int)ctx->batch_ids.size();
     std::vector<float> sums(2 * V);
     CK(hipMemcpyAsync(sums.data(), ctx->loss_sums.f(), sizeof(float) * 2 * V, hipMemcpyDeviceToHost, ctx->stream),
        "read loss");
     CK(hipStreamSynchronize(ctx->stream), "sync");
-    for (int k = 0; k < V; ++k) {
-      const ViewDesc& d = ctx->batch_desc[k];
-      const float inverse = 1.f / static_cast<float>((long long)d.width * d.height * 3);
-      const float l1 = sums[2 * k] * inverse;
-      losses[k] = w <= 0.f ? l1 : (1.f - w) * l1 + w * (1.f - sums[2 * k + 1] * inverse);
-    }
+    losses_from_sums(ctx, sums.data(), w, losses);
   }
   return SPLAT_OK;
 }
 
-splat_status backward_impl(splat_context* ctx) {
-  if (!ctx->forward_done) return fail(ctx, SPLAT_ERROR_STATE, "render_backward: no forward batch");
-  if (!ctx->loss_done) return fail(ctx, SPLAT_ERROR_STATE, "render_backward: no image gradient (run splat_loss)");
+splat_status enqueue_backward(splat_context* ctx) {
   const long long M = ctx->M;
   const bool det = ctx->opts.deterministic != 0;
   if (det) {
-    CK(ctx->partial.reserve(sizeof(float) * kScreenStride * std::max<long long>(1, ctx->P)), "alloc partials");
+    CK(ctx->partial.reserve(sizeof(float) * kScreenStride * std::max<unsigned long long>(1, ctx->P_cap)),
+       "alloc partials");
   } else {
     CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, M)), "alloc screen");
     TIMED("screen_clear", launch_fill_f32(ctx->screen.f(), 0.f, kScreenStride * M, ctx->stream));
   }
+  const uint32_t p_cap = (uint32_t)ctx->P_cap;
   TIMED("backward", launch_backward(ctx->raster, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
                                     ctx->dL.f(), ctx->T.f(), ctx->last.i(), ctx->screen.f(), ctx->pair_base.u(),
-                                    ctx->partial.f(), det, ctx->stream));
+                                    ctx->partial.f(), p_cap, det, ctx->stream));
   AdjointParams ap{};
   ap.cap = ctx->cap;
   ap.n = ctx->n;
@@ -371,13 +456,20 @@ splat_status backward_impl(splat_context* ctx) {
   ap.n_views = (int)ctx->batch_ids.size();
   ap.deterministic = det ? 1 : 0;
   TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
-                                  ctx->pair_base.u(), ctx->counts.u(), ctx->partial.f(), ctx->grads.f(), ctx->sg.f(),
-                                  ctx->vc.i(), ctx->stream));
+                                  ctx->pair_base.u(), ctx->counts.u(), ctx->partial.f(), p_cap, ctx->grads.f(),
+                                  ctx->sg.f(), ctx->vc.i(), ctx->stream));
   return SPLAT_OK;
 }
 
-splat_status adam_impl(splat_context* ctx, const splat_adam_params* a) {
-  AdamParams p{};
+splat_status backward_impl(splat_context* ctx) {
+  if (!ctx->forward_done) return fail(ctx, SPLAT_ERROR_STATE, "render_backward: no forward batch");
+  if (!ctx->loss_done) return fail(ctx, SPLAT_ERROR_STATE, "render_backward: no image gradient (run splat_loss)");
+  return enqueue_backward(ctx);
+}
+
+// The step's Adam scalars to device memory (ahead of the update on the stream).
+splat_status stage_adam(splat_context* ctx, const splat_adam_params* a) {
+  AdamStep& p = ctx->adam_host;
   p.lr[0] = a->lr_position;
   p.lr[1] = a->lr_log_scale;
   p.lr[2] = a->lr_rotation;
@@ -389,11 +481,165 @@ splat_status adam_impl(splat_context* ctx, const splat_adam_params* a) {
   p.eps = a->epsilon;
   p.c1 = a->correction1;
   p.c2 = a->correction2;
-  p.cap = ctx->cap;
-  p.n = ctx->n;
-  p.rest_stride = ctx->rs;
-  TIMED("adam", launch_adam(p, ctx->params.f(), ctx->grads.f(), ctx->m.f(), ctx->v.f(), ctx->sg.f(), ctx->vc.i(),
-                            ctx->stat_sg.f(), ctx->stat_vc.i(), ctx->stream));
+  CK(ctx->d_adam.reserve(sizeof(AdamStep)), "alloc adam");
+  CK(hipMemcpyAsync(ctx->d_adam.ptr, &ctx->adam_host, sizeof(AdamStep), hipMemcpyHostToDevice, ctx->stream),
+     "upload adam");
+  return SPLAT_OK;
+}
+
+splat_status enqueue_adam(splat_context* ctx, const uint32_t* skip) {
+  AdamShape sh{};
+  sh.cap = ctx->cap;
+  sh.n = ctx->n;
+  sh.rest_stride = ctx->rs;
+  TIMED("adam", launch_adam(sh, static_cast<const AdamStep*>(ctx->d_adam.ptr), skip, ctx->params.f(), ctx->grads.f(),
+                            ctx->m.f(), ctx->v.f(), ctx->sg.f(), ctx->vc.i(), ctx->stat_sg.f(), ctx->stat_vc.i(),
+                            ctx->stream));
+  return SPLAT_OK;
+}
+
+splat_status adam_impl(splat_context* ctx, const splat_adam_params* a) {
+  if (splat_status s = stage_adam(ctx, a)) return s;
+  return enqueue_adam(ctx, nullptr);
+}
+
+// One training step as a plain launch sequence (one host read of the pair
+// counts inside the forward).
+splat_status train_step_plain(splat_context* ctx, int n_views, const int32_t* view_ids,
+                              const splat_raster_options* o, float w, const splat_adam_params* adam, float* losses) {
+  if (splat_status s = forward_impl(ctx, n_views, view_ids, o)) return s;
+  if (splat_status s = loss_impl(ctx, w, nullptr)) return s;
+  if (splat_status s = backward_impl(ctx)) return s;
+  if (splat_status s = adam_impl(ctx, adam)) return s;
+  if (losses) {
+    std::vector<float> sums(2 * n_views);
+    CK(hipMemcpyAsync(sums.data(), ctx->loss_sums.f(), sizeof(float) * 2 * n_views, hipMemcpyDeviceToHost,
+                      ctx->stream),
+       "read loss");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+    losses_from_sums(ctx, sums.data(), w, losses);
+  }
+  return SPLAT_OK;
+}
+
+// Everything a captured step bakes in: shapes, sizes, options, capacities.
+std::string graph_key(const splat_context* ctx, float w) {
+  std::string k;
+  auto put = [&k](const void* p, size_t n) { k.append(static_cast<const char*>(p), n); };
+  const long long head[] = {ctx->cap, ctx->n, ctx->rs, ctx->sh_use, (long long)ctx->P_cap, (long long)ctx->S_cap};
+  put(head, sizeof head);
+  put(&ctx->opts, sizeof ctx->opts);
+  put(&w, sizeof w);
+  for (const ViewDesc& d : ctx->batch_desc) {
+    const int wh[2] = {d.width, d.height};
+    put(wh, sizeof wh);
+  }
+  return k;
+}
+
+constexpr size_t kMaxGraphs = 16;
+
+// One training step replayed from a captured graph: no host synchronisation
+// inside the step, one read of the losses and counters after it. Pair buffers
+// run at capacities learnt from earlier steps; a step whose pairs outgrow them
+// skips its update on the device, and is redone as a plain step after the
+// capacities grow.
+splat_status train_step_graph(splat_context* ctx, int n_views, const int32_t* view_ids,
+                              const splat_raster_options* o, float w, const splat_adam_params* adam, float* losses) {
+  if (ctx->graph_p_cap == 0) {
+    if (splat_status s = train_step_plain(ctx, n_views, view_ids, o, w, adam, losses)) return s;
+    ctx->graph_p_cap = ctx->P + ctx->P / 4 + 65536;
+    ctx->graph_s_cap = ctx->S + ctx->S / 4 + 4096;
+    return SPLAT_OK;
+  }
+  if (splat_status s = batch_setup(ctx, n_views, view_ids, o)) return s;
+  for (int id : ctx->batch_ids)
+    if (!ctx->has_target[id]) return fail(ctx, SPLAT_ERROR_STATE, "train_step: a batch view has no target image");
+  if (splat_status s = reserve_pairs(ctx, ctx->graph_p_cap, ctx->graph_s_cap)) return s;
+  if (splat_status s = reserve_loss(ctx, w)) return s;
+  CK(ctx->d_adam.reserve(sizeof(AdamStep)), "alloc adam");
+  const size_t rb_bytes = sizeof(float) * 2 * n_views + 16;
+  if (ctx->readback_bytes < rb_bytes) {
+    if (ctx->readback) (void)hipHostFree(ctx->readback);
+    ctx->readback = nullptr;
+    ctx->readback_bytes = 0;
+    CK(hipHostMalloc(reinterpret_cast<void**>(&ctx->readback), sizeof(float) * 2 * kMaxBatchViews + 16),
+       "alloc pinned");
+    ctx->readback_bytes = sizeof(float) * 2 * kMaxBatchViews + 16;
+  }
+  if (splat_status s = upload_batch(ctx)) return s;
+  if (splat_status s = stage_adam(ctx, adam)) return s;
+  const std::string key = graph_key(ctx, w);
+  const unsigned long long gen = alloc_generation();
+  splat_context::Graph* g = nullptr;
+  for (auto& e : ctx->graphs)
+    if (e.key == key && e.generation == gen) g = &e;
+  if (!g) {
+    for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();)
+      if (it->generation != gen) {
+        (void)hipGraphExecDestroy(it->exec);
+        it = ctx->graphs.erase(it);
+      } else {
+        ++it;
+      }
+    if (ctx->graphs.size() >= kMaxGraphs) ctx->drop_graphs();
+    splat_context::Graph fresh;
+    fresh.key = key;
+    fresh.generation = gen;
+    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "begin capture");
+    ctx->capturing = true;
+    splat_status s = enqueue_front(ctx);
+    if (!s) s = enqueue_back(ctx);
+    if (!s) s = enqueue_loss(ctx, w);
+    if (!s) s = enqueue_backward(ctx);
+    if (!s) s = enqueue_adam(ctx, ctx->scalars.u() + 3);
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (s) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return s;
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "end capture");
+    if (alloc_generation() != gen) {
+      (void)hipGraphDestroy(graph);
+      return fail(ctx, SPLAT_ERROR_DEVICE, "train_step: a buffer moved during capture");
+    }
+    const hipError_t ie = hipGraphInstantiate(&fresh.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return hip_fail(ctx, ie, "instantiate graph");
+    fresh.sorted_slots = ctx->sorted_slots;
+    fresh.raster = ctx->raster;
+    ctx->graphs.push_back(fresh);
+    g = &ctx->graphs.back();
+  }
+  CK(hipGraphLaunch(g->exec, ctx->stream), "launch graph");
+  ctx->sorted_slots = g->sorted_slots;
+  ctx->sorted_pair_vals = ctx->pair_vals.u();
+  ctx->raster = g->raster;
+  float* sums = reinterpret_cast<float*>(ctx->readback);
+  uint32_t* counters = ctx->readback + 2 * n_views;
+  CK(hipMemcpyAsync(sums, ctx->loss_sums.f(), sizeof(float) * 2 * n_views, hipMemcpyDeviceToHost, ctx->stream),
+     "read loss");
+  CK(hipMemcpyAsync(counters, ctx->scalars.u(), 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+     "read counters");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->P = counters[0];
+  ctx->S = counters[1];
+  if (counters[3]) {
+    // outgrew the capacities: the update was skipped on the device; clear this
+    // step's partial gradients and statistics and redo it with room to spare
+    ctx->graph_p_cap = ctx->P + ctx->P / 4 + 65536;
+    ctx->graph_s_cap = ctx->S + ctx->S / 4 + 4096;
+    CK(launch_fill_f32(ctx->grads.f(), 0.f, ctx->cap * (14 + ctx->rs), ctx->stream), "clear");
+    CK(launch_fill_f32(ctx->sg.f(), 0.f, ctx->cap, ctx->stream), "clear");
+    CK(launch_fill_u32(ctx->vc.u(), 0u, ctx->cap, ctx->stream), "clear");
+    return train_step_plain(ctx, n_views, view_ids, o, w, adam, losses);
+  }
+  ctx->forward_done = true;
+  ctx->loss_done = true;
+  ctx->last_ssim_weight = w;
+  if (losses) losses_from_sums(ctx, sums, w, losses);
   return SPLAT_OK;
 }
 
@@ -806,23 +1052,16 @@ splat_status splat_train_step(splat_context* ctx, int32_t n_views, const int32_t
                               float* losses) {
   if (splat_status s = check_ready(ctx)) return s;
   if (!adam) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "train_step: null adam params");
+  if (!options) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "train_step: null options");
   CK(hipSetDevice(ctx->device), "set device");
-  if (splat_status s = forward_impl(ctx, n_views, view_ids, options)) return s;
-  if (splat_status s = loss_impl(ctx, ssim_weight, nullptr)) return s;
-  if (splat_status s = backward_impl(ctx)) return s;
-  if (splat_status s = adam_impl(ctx, adam)) return s;
-  if (losses) {
-    std::vector<float> sums(2 * n_views);
-    CK(hipMemcpyAsync(sums.data(), ctx->loss_sums.f(), sizeof(float) * 2 * n_views, hipMemcpyDeviceToHost, ctx->stream),
-       "read loss");
-    CK(hipStreamSynchronize(ctx->stream), "sync");
-    for (int k = 0; k < n_views; ++k) {
-      const ViewDesc& d = ctx->batch_desc[k];
-      const float inverse = 1.f / static_cast<float>((long long)d.width * d.height * 3);
-      const float l1 = sums[2 * k] * inverse;
-      losses[k] = ssim_weight <= 0.f ? l1 : (1.f - ssim_weight) * l1 + ssim_weight * (1.f - sums[2 * k + 1] * inverse);
-    }
-  }
+  if (ctx->graph_mode) return train_step_graph(ctx, n_views, view_ids, options, ssim_weight, adam, losses);
+  return train_step_plain(ctx, n_views, view_ids, options, ssim_weight, adam, losses);
+}
+
+splat_status splat_set_graph_mode(splat_context* ctx, int32_t enabled) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  ctx->graph_mode = enabled != 0;
+  if (!ctx->graph_mode) ctx->drop_graphs();
   return SPLAT_OK;
 }
 

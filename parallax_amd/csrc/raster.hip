@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
                                                   const float* __restrict__ dL_drgb, const float* __restrict__ T_in,
                                                   const int* __restrict__ last_in, float* __restrict__ screen,
                                                   const uint32_t* __restrict__ pair_base, float* __restrict__ partial,
-                                                  int deterministic) {
+                                                  uint32_t p_cap, int deterministic) {
   __shared__ float4 s_a[kBatchB];
   __shared__ float4 s_b[kBatchB];
   __shared__ float s_c[kBatchB];
@@ -268,6 +268,7 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
       const uint32_t slot = pair_vals[e];
       const SplatRec& r = rec[slot];
       const uint32_t pid = pair_base[slot] + (uint32_t)((ty - r.y0) * (r.x1 - r.x0 + 1) + (tx - r.x0));
+      if (pid >= p_cap) continue;  // only when the step's capacity check has flagged it
       float* dst = partial + (long long)pid * kScreenStride;
 #pragma unroll
       for (int k = 0; k < kScreenStride; ++k) dst[k] = 0.f;
@@ -388,10 +389,12 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
           for (int k = 0; k < kVals; ++k) sum[k] += s_part[w][tid][k];
         }
       if (deterministic) {
-        float* dst = partial + (long long)s_dst[tid] * kScreenStride;
+        if (s_dst[tid] < p_cap) {
+          float* dst = partial + (long long)s_dst[tid] * kScreenStride;
 #pragma unroll
-        for (int k = 0; k < kVals; ++k) dst[k] = sum[k];
-        dst[kVals] = 0.f;
+          for (int k = 0; k < kVals; ++k) dst[k] = sum[k];
+          dst[kVals] = 0.f;
+        }
       } else if (m) {
         float* dst = screen + (long long)s_dst[tid] * kScreenStride;
 #pragma unroll
@@ -417,10 +420,10 @@ hipError_t launch_forward(const RasterParams& p, const ViewDesc* views, const ui
 hipError_t launch_backward(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
                            const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
                            const int* last, float* screen, const uint32_t* pair_base, float* partial,
-                           bool deterministic, hipStream_t stream) {
+                           uint32_t p_cap, bool deterministic, hipStream_t stream) {
   if (p.total_tiles <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_backward, dim3(p.total_tiles), dim3(256), 0, stream, p, views, ranges, pair_vals, rec, dL_drgb,
-                     T, last, screen, pair_base, partial, deterministic ? 1 : 0);
+                     T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -86,9 +86,19 @@ hipError_t scan_level(const uint32_t* in, const uint32_t* idx, uint32_t* out, lo
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys, long long n, int shift,
-                                                    uint32_t mask, uint32_t* __restrict__ hist, int nblocks) {
+// n_dev (nullable): the item count, read on the device and clamped to n
+// (the capacity the grid was sized for).
+__device__ __forceinline__ long long item_count(long long n, const uint32_t* n_dev) {
+  if (!n_dev) return n;
+  const long long d = (long long)*n_dev;
+  return d < n ? d : n;
+}
+
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys, long long n_cap,
+                                                    const uint32_t* __restrict__ n_dev, int shift, uint32_t mask,
+                                                    uint32_t* __restrict__ hist, int nblocks) {
   __shared__ uint32_t h[4][256];
+  const long long n = item_count(n_cap, n_dev);
   const int tid = threadIdx.x, wave = tid >> 6;
   for (int w = 0; w < 4; ++w) h[w][tid] = 0;
   __syncthreads();
@@ -114,8 +124,11 @@ constexpr int kRsRounds = kRsWaveItems / 64;  // 16
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ vals,
                                                        uint32_t* __restrict__ keys_out,
-                                                       uint32_t* __restrict__ vals_out, long long n, int shift,
-                                                       int bits, const uint32_t* __restrict__ hist, int nblocks) {
+                                                       uint32_t* __restrict__ vals_out, long long n_cap,
+                                                       const uint32_t* __restrict__ n_dev, int shift, int bits,
+                                                       const uint32_t* __restrict__ hist, int nblocks) {
+  const long long n = item_count(n_cap, n_dev);
+  if ((long long)blockIdx.x * kScanTile >= n) return;  // whole block past the items (uniform)
   __shared__ uint32_t s_cnt[4][256];   // per-wave digit counts, then per-wave digit offsets
   __shared__ uint32_t s_start[256];    // block-local start of each digit
   __shared__ uint32_t s_global[256];   // global start of each digit for this block
@@ -233,21 +246,21 @@ hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx,
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
-                             hipStream_t stream) {
+                             hipStream_t stream, const uint32_t* n_dev) {
   *result_in_alt = false;
-  if (n <= 1 || end_bit <= begin_bit) return hipSuccess;
+  if (end_bit <= begin_bit || n <= (n_dev ? 0 : 1)) return hipSuccess;
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (256 * nb > ws.hist_capacity) return hipErrorInvalidValue;
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     const int bits = (end_bit - shift) < 8 ? (end_bit - shift) : 8;
     const uint32_t mask = (1u << bits) - 1u;
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, shift, mask, ws.hist,
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, n_dev, shift, mask, ws.hist,
                        (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     SPLAT_HIP_CHECK(launch_exclusive_scan(ws.hist, nullptr, ws.hist, 256 * nb, nullptr, ws.scan, stream));
-    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, ko, vo, n, shift, bits,
-                       ws.hist, (int)nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, ko, vo, n, n_dev, shift,
+                       bits, ws.hist, (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     uint32_t* t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;

@@ -23,6 +23,10 @@ struct DeviceBuffer {
   int* i() const { return static_cast<int*>(ptr); }
 };
 
+// Bumped whenever any DeviceBuffer (re)allocates or frees: a captured graph
+// recorded under an older value may hold stale pointers.
+unsigned long long alloc_generation();
+
 struct KernelTimer {
   double total_ms = 0.0;
   long long launches = 0;
@@ -76,9 +80,29 @@ struct splat_context {
   int n_st = 0;                                     // super-tiles of the batch
   uint32_t* sorted_slots = nullptr;
   uint32_t* sorted_pair_vals = nullptr;
+  // capacities the pair-count-sized buffers were reserved for (kernel bounds)
+  unsigned long long P_cap = 0, S_cap = 0;
+
+  // captured training steps
+  struct Graph {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long generation = 0;
+    uint32_t* sorted_slots = nullptr;
+    splat::RasterParams raster{};
+  };
+  bool graph_mode = false;
+  bool capturing = false;
+  unsigned long long graph_p_cap = 0, graph_s_cap = 0;  // capacities graph steps run with
+  std::vector<Graph> graphs;
+  splat::AdamStep adam_host{};
+  DeviceBuffer d_adam;
+  uint32_t* readback = nullptr;  // pinned: loss sums then the step counters
+  size_t readback_bytes = 0;
+  void drop_graphs();
   DeviceBuffer rgb, T, last, dL, terms, block_sums, loss_sums;
   DeviceBuffer screen, partial;
-  DeviceBuffer scan_ws, hist_ws, scalars;
+  DeviceBuffer scan_ws, hist_ws, scalars;  // scalars: [0] tile pairs, [1] super-tile pairs, [2] segments, [3] overflow
   splat::SortWorkspace sort_ws;
 
   // densify scratch

@@ -39,11 +39,14 @@ struct SortWorkspace {
 long long sort_hist_elems(long long max_n);
 
 // Stable LSD radix sort of (key, value) pairs over key bits [begin_bit,
-// end_bit), 8-bit digits. Ping-pongs between (keys, vals) and (keys_alt,
-// vals_alt); *result_in_alt tells which pair holds the sorted output.
+// end_bit), 8-bit digits; key bits above end_bit ride along. Ping-pongs
+// between (keys, vals) and (keys_alt, vals_alt); *result_in_alt tells which
+// pair holds the sorted output. With n_dev, the item count is read on the
+// device (clamped to n, which then only sizes the launches), so the sort can
+// sit in a captured graph whose item count changes from replay to replay.
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
-                             hipStream_t stream);
+                             hipStream_t stream, const uint32_t* n_dev = nullptr);
 
 // ---- binning (binning.hip) ----
 constexpr int kBinSegment = 1024;  // splats per segment of a super-tile list
@@ -54,12 +57,18 @@ constexpr uint32_t kStMask = (1u << kStBits) - 1u;
 // super-tile of its rect at soffsets[s] (*s_total: all pairs); a key holds the
 // global super-tile id in its low kStBits bits and the splat's rect inside
 // that super-tile above them. With pair_base: pair_base[slot] = offsets[s].
+// Pairs at or past s_cap are dropped (the capacity check flags the step).
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
                              const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
-                             long long m, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base, hipStream_t stream);
-// sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted pairs.
-hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, uint32_t* sranges, int n_st,
-                               hipStream_t stream);
+                             long long m, uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
+                             hipStream_t stream);
+// sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted
+// pairs; the pair count is min(*n_dev, n) with n_dev, else n.
+hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, const uint32_t* n_dev, uint32_t* sranges,
+                               int n_st, hipStream_t stream);
+// totals[3] = 1 if totals[0] (tile pairs) > p_cap or totals[1] (super-tile
+// pairs) > s_cap, else 0.
+hipError_t launch_check_caps(uint32_t* totals, uint32_t p_cap, uint32_t s_cap, hipStream_t stream);
 // nseg[st] = segments of kBinSegment splats in super-tile st's list.
 hipError_t launch_segment_counts(const uint32_t* sranges, int n_st, uint32_t* nseg, hipStream_t stream);
 // seg_st[seg_base[st] + s] = st for every segment s of st.
@@ -72,14 +81,16 @@ hipError_t launch_tile_count(const uint32_t* skeys, const uint32_t* sranges,
                              const uint32_t* nseg, const uint32_t* seg_base, const uint32_t* seg_st,
                              const uint32_t* ns_total, uint32_t ns_max, uint32_t* table, hipStream_t stream);
 // Every tile's list, in depth order, into pair_vals at the runs `scan` (the
-// exclusive scan of the count table) assigns.
+// exclusive scan of the count table) assigns; entries at or past p_cap are
+// dropped.
 hipError_t launch_tile_write(const uint32_t* skeys, const uint32_t* svals,
                              const uint32_t* sranges, const uint32_t* nseg, const uint32_t* seg_base,
                              const uint32_t* seg_st, const uint32_t* ns_total, uint32_t ns_max, const uint32_t* scan,
-                             uint32_t* pair_vals, hipStream_t stream);
-// ranges[2*t] / ranges[2*t+1] = [begin, end) of global tile t in pair_vals.
+                             uint32_t p_cap, uint32_t* pair_vals, hipStream_t stream);
+// ranges[2*t] / ranges[2*t+1] = [begin, end) of global tile t in pair_vals,
+// clamped to p_cap.
 hipError_t launch_tile_ranges_table(const ViewDesc* views, int n_views, int total_tiles, const uint32_t* scan,
-                                    const uint32_t* nseg, const uint32_t* seg_base, uint32_t* ranges,
+                                    const uint32_t* nseg, const uint32_t* seg_base, uint32_t p_cap, uint32_t* ranges,
                                     hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* ptr, uint32_t value, long long n, hipStream_t stream);
 hipError_t launch_fill_f32(float* ptr, float value, long long n, hipStream_t stream);
@@ -102,7 +113,7 @@ hipError_t launch_forward(const RasterParams& p, const ViewDesc* views, const ui
 hipError_t launch_backward(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
                            const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
                            const int* last, float* screen, const uint32_t* pair_base, float* partial,
-                           bool deterministic, hipStream_t stream);
+                           uint32_t p_cap, bool deterministic, hipStream_t stream);
 
 // ---- loss ----
 struct LossParams {
@@ -133,17 +144,25 @@ struct AdjointParams {
 };
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
                           const float* screen, const uint32_t* pair_base, const uint32_t* counts,
-                          const float* partial, float* grads, float* sg, int* vc, hipStream_t stream);
+                          const float* partial, uint32_t p_cap, float* grads, float* sg, int* vc,
+                          hipStream_t stream);
 
 // ---- optimiser ----
-struct AdamParams {
-  float lr[6];
+// The per-step scalars, read from device memory so a captured step can take
+// new ones each replay.
+struct AdamStep {
+  float lr[6];  // position, log_scale, rotation, opacity, dc, rest
   float beta1, beta2, eps, c1, c2;
+};
+struct AdamShape {
   long long cap;
   int n;
   int rest_stride;
 };
-hipError_t launch_adam(const AdamParams& p, float* params, float* grads, float* m, float* v, float* sg, int* vc,
-                       float* stat_sg, int* stat_vc, hipStream_t stream);
+// Skips the update (and leaves gradients and statistics in place) when
+// skip_flag is non-null and *skip_flag != 0.
+hipError_t launch_adam(const AdamShape& s, const AdamStep* step, const uint32_t* skip_flag, float* params,
+                       float* grads, float* m, float* v, float* sg, int* vc, float* stat_sg, int* stat_vc,
+                       hipStream_t stream);
 
 }  // namespace splat

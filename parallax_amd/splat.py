@@ -93,6 +93,7 @@ SIGNATURES = {
                                    f32p]),
     "splat_densify": (C.c_int, [C.c_void_p, C.POINTER(CDensify), C.c_float, C.c_int32, C.POINTER(C.c_uint32), i32p,
                                 i32p]),
+    "splat_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "splat_reset_opacity": (C.c_int, [C.c_void_p, C.c_float]),
     "splat_download_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
     "splat_prune_and_sort": (C.c_int, [C.c_void_p, C.c_float]),
@@ -371,6 +372,10 @@ class Context:
     # ---- measurement ----
     def set_timing(self, on: bool):
         self._ck(lib().splat_set_timing(self.h, 1 if on else 0))
+
+    def set_graph_mode(self, on: bool):
+        """train_step replays a captured HIP graph per batch shape."""
+        self._ck(lib().splat_set_graph_mode(self.h, 1 if on else 0))
 
     def timings(self) -> dict:
         out = {}
