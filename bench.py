@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="batched leg as a plain launch sequence")
+    ap.add_argument("--no-autotune", action="store_true", help="default compositing launch shape")
     ap.add_argument("--profile", action="store_true",
                     help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
     return ap.parse_args()
@@ -75,6 +76,7 @@ def main():
     opts = S.options(sh_degree=3, deterministic=args.deterministic)
     ctx = S.Context(0)
     ctx.set_capacity(n, 3)
+    ctx.set_autotune(not args.no_autotune)
     if args.profile:
         # no extra forward launches in the profile: fixed-seed noise targets
         rng = np.random.default_rng(5)
@@ -114,7 +116,8 @@ def main():
         run(batches[s % len(batches)])
     ctx.synchronize()
     batched_s = (time.perf_counter() - t0) / args.steps
-    log(f"batched: {batched_s * 1e3:.2f} ms per step of {B} views")
+    cfg_batched = ctx.raster_config()
+    log(f"batched: {batched_s * 1e3:.2f} ms per step of {B} views ({cfg_batched})")
     if args.profile:
         ctx.close()
         print(json.dumps({"profile": True, "batched_ms_per_step": round(batched_s * 1e3, 3), "batch_views": B}))
@@ -131,7 +134,8 @@ def main():
         run([s % len(views)])
     ctx.synchronize()
     plain_s = (time.perf_counter() - t0) / plain_steps
-    log(f"plain: {plain_s * 1e3:.2f} ms per single-view step")
+    cfg_plain = ctx.raster_config()
+    log(f"plain: {plain_s * 1e3:.2f} ms per single-view step ({cfg_plain})")
 
     # ---- per-kernel times (HIP events on the context's stream) ----
     ctx.set_timing(True)
@@ -209,6 +213,7 @@ def main():
                    "deterministic": bool(args.deterministic), "graph_captured_step": not args.no_graph,
                    "parallelism": "single GPU"},
         "plain_steps_per_s": round(1.0 / plain_s, 3),
+        "raster_config": {"batched": cfg_batched, "plain": cfg_plain},
         "batched_vs_plain_per_view": round(plain_s / (batched_s / B), 3),
         "gaussian_pixels_per_s": float(f"{n * px / batched_s:.4g}"),
         "pairs_per_view": int(pairs // B),

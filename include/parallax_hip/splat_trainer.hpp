@@ -141,6 +141,11 @@ struct SplatTrainingConfig {
   int batch_views = 1;
   bool deterministic = false;
   int device = 0;
+  /// Replay each step from a captured HIP graph (same results as plain launches).
+  bool graph_step = true;
+  /// Time the compositing launch shapes on the first batch of each resolution
+  /// mix and keep the fastest (same results whichever is kept).
+  bool autotune = true;
 
   std::string validate() const;
 };

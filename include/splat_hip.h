@@ -198,6 +198,27 @@ splat_status splat_train_step(splat_context* ctx, int32_t n_views, const int32_t
  * Default: off. */
 splat_status splat_set_graph_mode(splat_context* ctx, int32_t enabled);
 
+/* ---- compositing launch shape (performance only) ---- */
+/* How the forward and adjoint compositing kernels are launched. Every choice
+ * gives the same images, and the same gradients in deterministic mode. */
+typedef struct splat_raster_config {
+  int32_t forward_shape;   /* workgroup of the forward: 0 16x16 pixels (one tile,
+                              four 8x8 waves), 1 8x8 (one wave), 2 32x8 (the top
+                              or bottom halves of two adjacent tiles) */
+  int32_t forward_queue;   /* 1: persistent grid pulling tiles, longest list
+                              first, from a work queue; 0: one workgroup per unit */
+  int32_t backward_batch;  /* splats staged per adjoint batch: 64 or 128 */
+  int32_t backward_queue;  /* as forward_queue, for the adjoint */
+} splat_raster_config;
+/* enabled != 0: the first batch of each new shape (the resolutions of its
+ * views) times every launch shape on that batch and keeps the fastest for the
+ * shape. Default: off (16x16, 128, no queue). No counterpart in the engine. */
+splat_status splat_set_autotune(splat_context* ctx, int32_t enabled);
+/* Use `config` for every batch (NULL: back to default / tuned). */
+splat_status splat_set_raster_config(splat_context* ctx, const splat_raster_config* config);
+/* The launch shape of the current batch. */
+splat_status splat_get_raster_config(const splat_context* ctx, splat_raster_config* out);
+
 /* ---- adaptive density ---- */
 /* densifyAndPrune + remapState on the device. rng_state is advanced exactly
  * as the CPU trainer advances it. source_index (nullable, sized for the new

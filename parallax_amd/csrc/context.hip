@@ -130,7 +130,8 @@ std::vector<DeviceBuffer*> splat_context::all_buffers() {
           &vals_alt, &counts, &offsets, &pair_base, &scounts, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
           &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &scan_ws, &hist_ws, &scalars,
-          &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam};
+          &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam,
+          &order_keys, &order_vals, &order_keys_alt, &order_vals_alt};
 }
 
 // ------------------------------------------------------------- timing --
@@ -210,6 +211,16 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
     ctx->batch_desc.push_back(d);
   }
   if (supers > (int)kStMask) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: batch too large");
+  ctx->shape_key.clear();
+  for (const ViewDesc& d : ctx->batch_desc) {
+    ctx->shape_key += std::to_string(d.width) + "x" + std::to_string(d.height) + ",";
+  }
+  {
+    const auto it = ctx->tuned.find(ctx->shape_key);
+    const int cus = ctx->rcfg.cus;
+    ctx->rcfg = ctx->forced ? ctx->forced_cfg : (it != ctx->tuned.end() ? it->second : RasterConfig{});
+    ctx->rcfg.cus = cus;
+  }
   ctx->total_tiles = tiles;
   ctx->loss_tiles = ltiles;
   ctx->n_st = supers;
@@ -233,6 +244,10 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
   CK(ctx->T.reserve(sizeof(float) * std::max(1LL, px)), "alloc image");
   CK(ctx->last.reserve(sizeof(int) * std::max(1LL, px)), "alloc image");
   CK(ctx->ranges.reserve(sizeof(uint32_t) * 2 * std::max(1, tiles)), "alloc ranges");
+  CK(ctx->order_keys.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
+  CK(ctx->order_vals.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
+  CK(ctx->order_keys_alt.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
+  CK(ctx->order_vals_alt.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
   CK(ctx->scalars.reserve(64), "alloc scalars");
   const long long Mc = std::max(1LL, M);
   CK(ctx->rec.reserve(sizeof(SplatRec) * Mc), "alloc records");
@@ -308,6 +323,8 @@ splat_status enqueue_front(splat_context* ctx) {
   return SPLAT_OK;
 }
 
+splat_status enqueue_tile_order(splat_context* ctx);
+
 // Binning within the reserved capacities (the pair counts are read on the
 // device), then forward compositing.
 splat_status enqueue_back(splat_context* ctx) {
@@ -342,8 +359,76 @@ splat_status enqueue_back(splat_context* ctx) {
   TIMED("tile_ranges", launch_tile_ranges_table(ctx->views_dev(), n_views, tiles, ctx->bin_table.u(), ctx->nseg.u(),
                                                 ctx->seg_base.u(), p_cap, ctx->ranges.u(), ctx->stream));
   ctx->sorted_pair_vals = ctx->pair_vals.u();
-  TIMED("forward", launch_forward(ctx->raster, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
-                                  ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->stream));
+  if (ctx->rcfg.fwd_queue || ctx->rcfg.bwd_queue || (ctx->autotune && !ctx->capturing))
+    if (splat_status s = enqueue_tile_order(ctx)) return s;
+  TIMED("forward", launch_forward(ctx->raster, ctx->rcfg, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals,
+                                  ctx->recs(), ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->tile_order,
+                                  ctx->scalars.u() + 4, ctx->stream));
+  return SPLAT_OK;
+}
+
+// Tiles by list length, longest first: the work order of the queue launches.
+splat_status enqueue_tile_order(splat_context* ctx) {
+  const int tiles = ctx->total_tiles;
+  CK(launch_tile_order_keys(ctx->ranges.u(), tiles, ctx->order_keys.u(), ctx->order_vals.u(), ctx->stream),
+     "tile order");
+  bool alt = false;
+  CK(launch_radix_sort(ctx->order_keys.u(), ctx->order_vals.u(), ctx->order_keys_alt.u(), ctx->order_vals_alt.u(),
+                       tiles, 0, 16, ctx->sort_ws, &alt, ctx->stream),
+     "tile order");
+  ctx->tile_order = alt ? ctx->order_vals_alt.u() : ctx->order_vals.u();
+  return SPLAT_OK;
+}
+
+// Mean time of `reps` launches of `launch` on the context's stream, in ms.
+template <typename F>
+splat_status time_launches(splat_context* ctx, int reps, F launch, float* ms) {
+  hipEvent_t a, b;
+  CK(hipEventCreate(&a), "event");
+  CK(hipEventCreate(&b), "event");
+  hipError_t e = launch();  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  CK(e, "autotune");
+  *ms = t / reps;
+  return SPLAT_OK;
+}
+
+// The forward launch shape for this batch shape: every variant timed on the
+// batch just binned, the fastest kept (all give the same image; the last one
+// timed is re-run with the choice so the outputs are the chosen kernel's).
+splat_status tune_forward(splat_context* ctx) {
+  RasterConfig best = ctx->rcfg;
+  float best_ms = 1e30f;
+  for (int shape = 0; shape < 3; ++shape)
+    for (int queue = 0; queue < 2; ++queue) {
+      RasterConfig c = ctx->rcfg;
+      c.fwd_shape = shape;
+      c.fwd_queue = queue != 0;
+      float ms = 0.f;
+      if (splat_status s = time_launches(ctx, 2, [&] {
+            return launch_forward(ctx->raster, c, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals,
+                                  ctx->recs(), ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->tile_order,
+                                  ctx->scalars.u() + 4, ctx->stream);
+          }, &ms))
+        return s;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  ctx->rcfg.fwd_shape = best.fwd_shape;
+  ctx->rcfg.fwd_queue = best.fwd_queue;
+  CK(launch_forward(ctx->raster, ctx->rcfg, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
+                    ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->tile_order, ctx->scalars.u() + 4, ctx->stream),
+     "forward");
+  ctx->tuning_backward = true;
   return SPLAT_OK;
 }
 
@@ -363,6 +448,9 @@ splat_status forward_impl(splat_context* ctx, int n_views, const int32_t* view_i
   ctx->S = totals[1];
   if (splat_status s = reserve_pairs(ctx, totals[0], totals[1])) return s;
   if (splat_status s = enqueue_back(ctx)) return s;
+  ctx->tuning_backward = false;
+  if (ctx->autotune && !ctx->forced && !ctx->tuned.count(ctx->shape_key))
+    if (splat_status s = tune_forward(ctx)) return s;
   ctx->forward_done = true;
   return SPLAT_OK;
 }
@@ -445,9 +533,35 @@ splat_status enqueue_backward(splat_context* ctx) {
     TIMED("screen_clear", launch_fill_f32(ctx->screen.f(), 0.f, kScreenStride * M, ctx->stream));
   }
   const uint32_t p_cap = (uint32_t)ctx->P_cap;
-  TIMED("backward", launch_backward(ctx->raster, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
-                                    ctx->dL.f(), ctx->T.f(), ctx->last.i(), ctx->screen.f(), ctx->pair_base.u(),
-                                    ctx->partial.f(), p_cap, det, ctx->stream));
+  auto launch = [&](const RasterConfig& c) {
+    return launch_backward(ctx->raster, c, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
+                           ctx->dL.f(), ctx->T.f(), ctx->last.i(), ctx->screen.f(), ctx->pair_base.u(),
+                           ctx->partial.f(), p_cap, det, ctx->tile_order, ctx->scalars.u() + 4, ctx->stream);
+  };
+  if (ctx->tuning_backward && !ctx->capturing) {
+    // the adjoint's launch shape for this batch shape: each variant timed on
+    // this batch (the screen accumulators cleared again before the real run)
+    RasterConfig best = ctx->rcfg;
+    float best_ms = 1e30f;
+    for (int batch = 64; batch <= 128; batch *= 2)
+      for (int queue = 0; queue < 2; ++queue) {
+        RasterConfig c = ctx->rcfg;
+        c.bwd_batch = batch;
+        c.bwd_queue = queue != 0;
+        float ms = 0.f;
+        if (splat_status s = time_launches(ctx, 2, [&] { return launch(c); }, &ms)) return s;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+    ctx->rcfg.bwd_batch = best.bwd_batch;
+    ctx->rcfg.bwd_queue = best.bwd_queue;
+    ctx->tuned[ctx->shape_key] = ctx->rcfg;
+    ctx->tuning_backward = false;
+    if (!det) CK(launch_fill_f32(ctx->screen.f(), 0.f, kScreenStride * M, ctx->stream), "clear");
+  }
+  TIMED("backward", launch(ctx->rcfg));
   AdjointParams ap{};
   ap.cap = ctx->cap;
   ap.n = ctx->n;
@@ -530,6 +644,8 @@ std::string graph_key(const splat_context* ctx, float w) {
   put(head, sizeof head);
   put(&ctx->opts, sizeof ctx->opts);
   put(&w, sizeof w);
+  const int cfg[4] = {ctx->rcfg.fwd_shape, ctx->rcfg.fwd_queue ? 1 : 0, ctx->rcfg.bwd_batch, ctx->rcfg.bwd_queue ? 1 : 0};
+  put(cfg, sizeof cfg);
   for (const ViewDesc& d : ctx->batch_desc) {
     const int wh[2] = {d.width, d.height};
     put(wh, sizeof wh);
@@ -555,6 +671,8 @@ splat_status train_step_graph(splat_context* ctx, int n_views, const int32_t* vi
   if (splat_status s = batch_setup(ctx, n_views, view_ids, o)) return s;
   for (int id : ctx->batch_ids)
     if (!ctx->has_target[id]) return fail(ctx, SPLAT_ERROR_STATE, "train_step: a batch view has no target image");
+  if (ctx->autotune && !ctx->forced && !ctx->tuned.count(ctx->shape_key))
+    return train_step_plain(ctx, n_views, view_ids, o, w, adam, losses);  // tunes this batch shape
   if (splat_status s = reserve_pairs(ctx, ctx->graph_p_cap, ctx->graph_s_cap)) return s;
   if (splat_status s = reserve_loss(ctx, w)) return s;
   CK(ctx->d_adam.reserve(sizeof(AdamStep)), "alloc adam");
@@ -712,6 +830,11 @@ splat_status splat_create(int32_t device, splat_context** out) {
   ctx->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    ctx->rcfg.cus = cus > 0 ? cus : 256;
+  }
   if (e != hipSuccess) {
     std::string msg = std::string("stream creation failed: ") + hipGetErrorString(e);
     delete ctx;
@@ -1062,6 +1185,38 @@ splat_status splat_set_graph_mode(splat_context* ctx, int32_t enabled) {
   if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
   ctx->graph_mode = enabled != 0;
   if (!ctx->graph_mode) ctx->drop_graphs();
+  return SPLAT_OK;
+}
+
+splat_status splat_set_autotune(splat_context* ctx, int32_t enabled) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  ctx->autotune = enabled != 0;
+  return SPLAT_OK;
+}
+
+splat_status splat_set_raster_config(splat_context* ctx, const splat_raster_config* config) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  if (!config) {
+    ctx->forced = false;
+    return SPLAT_OK;
+  }
+  if (config->forward_shape < 0 || config->forward_shape > 2 ||
+      (config->backward_batch != 64 && config->backward_batch != 128))
+    return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "set_raster_config: forward_shape 0..2, backward_batch 64 or 128");
+  ctx->forced = true;
+  ctx->forced_cfg.fwd_shape = config->forward_shape;
+  ctx->forced_cfg.fwd_queue = config->forward_queue != 0;
+  ctx->forced_cfg.bwd_batch = config->backward_batch;
+  ctx->forced_cfg.bwd_queue = config->backward_queue != 0;
+  return SPLAT_OK;
+}
+
+splat_status splat_get_raster_config(const splat_context* ctx, splat_raster_config* out) {
+  if (!ctx || !out) return SPLAT_ERROR_INVALID_ARGUMENT;
+  out->forward_shape = ctx->rcfg.fwd_shape;
+  out->forward_queue = ctx->rcfg.fwd_queue ? 1 : 0;
+  out->backward_batch = ctx->rcfg.bwd_batch;
+  out->backward_queue = ctx->rcfg.bwd_queue ? 1 : 0;
   return SPLAT_OK;
 }
 

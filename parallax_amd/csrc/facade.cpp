@@ -339,6 +339,8 @@ HipSplatTrainer::HipSplatTrainer(GaussianCloud initial, std::vector<SplatTrainin
     m.fail("reserving device memory");
     return;
   }
+  splat_set_graph_mode(m.ctx, config.graph_step ? 1 : 0);
+  splat_set_autotune(m.ctx, config.autotune ? 1 : 0);
   splat_cloud c = soa.abi();
   if (splat_upload_cloud(m.ctx, &c) != SPLAT_OK) {
     m.fail("uploading the cloud");

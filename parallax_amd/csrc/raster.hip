@@ -36,6 +36,8 @@
 // store into that pair's own slot for a later fixed-order per-gaussian sum
 // (deterministic mode). VALU-bound; the figure of merit is (pixel, splat)
 // evaluations per second.
+#include <algorithm>
+
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -43,7 +45,6 @@ namespace splat {
 namespace {
 
 constexpr int kChunk = 64;    // forward: splats fetched per wave step
-constexpr int kBatchB = 128;  // adjoint: splats staged per workgroup batch
 constexpr int kVals = 11;  // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3, touched
 
 __device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int tile) {
@@ -82,22 +83,21 @@ __device__ __forceinline__ bool box_hits(float mx, float my, float ex, float ey,
   return mx + ex >= q.bx0 && mx - ex <= q.bx1 && my + ey >= q.by0 && my - ey <= q.by1;
 }
 
-__global__ __launch_bounds__(256) void k_forward(RasterParams p, const ViewDesc* __restrict__ views,
-                                                 const uint32_t* __restrict__ ranges,
-                                                 const uint32_t* __restrict__ pair_vals,
-                                                 const SplatRec* __restrict__ rec, float* __restrict__ rgb,
-                                                 float* __restrict__ T_out, int* __restrict__ last_out) {
-  __shared__ float4 s_a[4][kChunk];  // mx, my, ca, cb
-  __shared__ float4 s_b[4][kChunk];  // cc, opacity, r, g
-  __shared__ float2 s_c[4][kChunk];  // b, list index (int bits)
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
+// One wave's forward over one 8x8 quadrant (0..3, row-major) of one tile. The
+// LDS slices are the wave's own.
+__device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDesc* __restrict__ views,
+                                             const uint32_t* __restrict__ ranges,
+                                             const uint32_t* __restrict__ pair_vals,
+                                             const SplatRec* __restrict__ rec, float* __restrict__ rgb,
+                                             float* __restrict__ T_out, int* __restrict__ last_out, int tile,
+                                             int quadrant, float4* s_a, float4* s_b, float2* s_c) {
+  const int lane = threadIdx.x & 63;
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
   const int local = tile - V.tile_base;
   const int tx = local % V.cols, ty = local / V.cols;
   Quad q;
-  if (!quad_of(V, tx, ty, wave, &q)) return;  // no barrier in this kernel: waves are independent
+  if (!quad_of(V, tx, ty, quadrant, &q)) return;
   const int px = q.x0 + (lane & 7), py = q.y0 + (lane >> 3);
   const bool inside = px < V.width && py < V.height;
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
@@ -120,23 +120,23 @@ __global__ __launch_bounds__(256) void k_forward(RasterParams p, const ViewDesc*
     const uint64_t m = __ballot(hit);
     if (hit) {
       const int pos = lane_rank(m);
-      s_a[wave][pos] = a;
-      s_b[wave][pos] = b;
-      s_c[wave][pos] = make_float2(c.x, __int_as_float((int)(e - begin)));
+      s_a[pos] = a;
+      s_b[pos] = b;
+      s_c[pos] = make_float2(c.x, __int_as_float((int)(e - begin)));
     }
     wave_sync_lds();
     const int cnt = __popcll(m);
     if (!done) {
       for (int j = 0; j < cnt; ++j) {
-        const float4 sa = s_a[wave][j];
-        const float4 sb = s_b[wave][j];
+        const float4 sa = s_a[j];
+        const float4 sb = s_b[j];
         const float dx = fx - sa.x, dy = fy - sa.y;
         const float power = -0.5f * (sa.z * dx * dx + sb.x * dy * dy) - sa.w * dx * dy;
         if (power > 0.f) continue;
         const float raw = sb.y * det_exp_nonpos(power);
         const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
         if (alpha < p.min_alpha) continue;
-        const float2 sc = s_c[wave][j];
+        const float2 sc = s_c[j];
         const float w = T * alpha;
         acc0 += w * sb.z;
         acc1 += w * sb.w;
@@ -158,6 +158,100 @@ __global__ __launch_bounds__(256) void k_forward(RasterParams p, const ViewDesc*
     rgb[3 * pixel + 2] = acc2 + T * p.bg[2];
     T_out[pixel] = T;
     last_out[pixel] = last;
+  }
+}
+
+// Work units of the forward launch shapes. kShape 0: a 256-thread workgroup
+// per 16x16 tile, its waves on the four quadrants. kShape 1: a one-wave
+// workgroup per quadrant (8x8). kShape 2: a 256-thread workgroup per 32x8
+// strip -- the top or bottom halves of two side-by-side tiles (units of tiles
+// in odd columns are empty). The waves never wait for each other, so every
+// shape gives the same image; they differ in scheduling and cache sharing.
+template <int kShape>
+struct FwdShape;
+template <>
+struct FwdShape<0> {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ int units(int tiles) { return tiles; }
+  static __device__ __forceinline__ bool wave_work(const ViewDesc* views, int n_views, int unit, int wave, int* tile,
+                                                   int* quadrant) {
+    (void)views;
+    (void)n_views;
+    *tile = unit;
+    *quadrant = wave;
+    return true;
+  }
+};
+template <>
+struct FwdShape<1> {
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ int units(int tiles) { return 4 * tiles; }
+  static __device__ __forceinline__ bool wave_work(const ViewDesc* views, int n_views, int unit, int wave, int* tile,
+                                                   int* quadrant) {
+    (void)views;
+    (void)n_views;
+    (void)wave;
+    *tile = unit >> 2;
+    *quadrant = unit & 3;
+    return true;
+  }
+};
+template <>
+struct FwdShape<2> {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ int units(int tiles) { return 2 * tiles; }
+  static __device__ __forceinline__ bool wave_work(const ViewDesc* views, int n_views, int unit, int wave, int* tile,
+                                                   int* quadrant) {
+    const int left = unit >> 1, half = unit & 1;
+    const ViewDesc& V = views[find_view(views, n_views, left)];
+    const int tx = (left - V.tile_base) % V.cols;
+    if (tx & 1) return false;                          // units start on even columns
+    if ((wave >> 1) && tx + 1 >= V.cols) return false;  // no right neighbour
+    *tile = left + (wave >> 1);
+    *quadrant = 2 * half + (wave & 1);
+    return true;
+  }
+};
+
+// kQueue false: one workgroup per unit, units spread over the XCDs in
+// contiguous runs. kQueue true: a persistent grid; each workgroup takes the
+// next unit from a counter over `order` (tiles, longest list first), so the
+// longest tiles start first and the launch ends on short ones.
+template <int kShape, bool kQueue>
+__global__ __launch_bounds__(FwdShape<kShape>::kThreads) void k_forward(
+    RasterParams p, const ViewDesc* __restrict__ views, const uint32_t* __restrict__ ranges,
+    const uint32_t* __restrict__ pair_vals, const SplatRec* __restrict__ rec, float* __restrict__ rgb,
+    float* __restrict__ T_out, int* __restrict__ last_out, const uint32_t* __restrict__ order,
+    uint32_t* __restrict__ counter) {
+  using Sh = FwdShape<kShape>;
+  constexpr int kWaves = Sh::kThreads / 64;
+  __shared__ float4 s_a[kWaves][kChunk];  // mx, my, ca, cb
+  __shared__ float4 s_b[kWaves][kChunk];  // cc, opacity, r, g
+  __shared__ float2 s_c[kWaves][kChunk];  // b, list index (int bits)
+  __shared__ int s_unit;
+  const int wave = threadIdx.x >> 6;
+  const int n_units = Sh::units(p.total_tiles);
+  if (!kQueue) {
+    const int unit = xcd_remap(blockIdx.x, n_units);
+    int tile, quadrant;
+    if (Sh::wave_work(views, p.n_views, unit, wave, &tile, &quadrant))
+      forward_quad(p, views, ranges, pair_vals, rec, rgb, T_out, last_out, tile, quadrant, s_a[wave], s_b[wave],
+                   s_c[wave]);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int idx = s_unit;
+    __syncthreads();
+    if (idx >= n_units) break;
+    // the idx-th unit: tiles longest first, a tile's units together
+    const int per = n_units / p.total_tiles;
+    const int unit = (int)order[idx / per] * per + idx % per;
+    int tile, quadrant;
+    if (Sh::wave_work(views, p.n_views, unit, wave, &tile, &quadrant))
+      forward_quad(p, views, ranges, pair_vals, rec, rgb, T_out, last_out, tile, quadrant, s_a[wave], s_b[wave],
+                   s_c[wave]);
   }
 }
 
@@ -214,25 +308,40 @@ __device__ __forceinline__ int butterfly_slot(int lane) {
   return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
 }
 
-__global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
-                                                  const uint32_t* __restrict__ ranges,
-                                                  const uint32_t* __restrict__ pair_vals,
-                                                  const SplatRec* __restrict__ rec,
-                                                  const float* __restrict__ dL_drgb, const float* __restrict__ T_in,
-                                                  const int* __restrict__ last_in, float* __restrict__ screen,
-                                                  const uint32_t* __restrict__ pair_base, float* __restrict__ partial,
-                                                  uint32_t p_cap, int deterministic) {
-  __shared__ float4 s_a[kBatchB];
-  __shared__ float4 s_b[kBatchB];
-  __shared__ float s_c[kBatchB];
-  __shared__ uint32_t s_dst[kBatchB];          // slot (atomic) or pair id (deterministic)
-  __shared__ uint32_t s_mask[kBatchB];         // quadrants that visit the splat
-  __shared__ float s_part[4][kBatchB][kVals];  // per-wave sums (valid where s_mask says)
-  __shared__ int s_stop[4];                    // per-quadrant last contributor
+// The workgroup's shared storage for one batch of kBB splats.
+template <int kBB>
+struct BwdShared {
+  float4 a[kBB];
+  float4 b[kBB];
+  float c[kBB];
+  uint32_t dst[kBB];          // slot (atomic) or pair id (deterministic)
+  uint32_t mask[kBB];         // quadrants that visit the splat
+  float part[4][kBB][kVals];  // per-wave sums (valid where mask says)
+  int stop[4];                // per-quadrant last contributor
+};
+
+// The adjoint of one 16x16 tile by one 256-thread workgroup (four quadrant
+// waves); block-synchronous, so every thread of the workgroup calls it.
+template <int kBB>
+__device__ __forceinline__ void backward_tile(const RasterParams& p, const ViewDesc* __restrict__ views,
+                                              const uint32_t* __restrict__ ranges,
+                                              const uint32_t* __restrict__ pair_vals,
+                                              const SplatRec* __restrict__ rec, const float* __restrict__ dL_drgb,
+                                              const float* __restrict__ T_in, const int* __restrict__ last_in,
+                                              float* __restrict__ screen, const uint32_t* __restrict__ pair_base,
+                                              float* __restrict__ partial, uint32_t p_cap, int deterministic,
+                                              int tile, BwdShared<kBB>& sh) {
+  constexpr int kBatchB = kBB;
+  float4* s_a = sh.a;
+  float4* s_b = sh.b;
+  float* s_c = sh.c;
+  uint32_t* s_dst = sh.dst;
+  uint32_t* s_mask = sh.mask;
+  auto& s_part = sh.part;
+  int* s_stop = sh.stop;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int my_slot = butterfly_slot(lane);
   const bool writer = lane < 16 && my_slot < kVals;
-  const int tile = xcd_remap(blockIdx.x, p.total_tiles);
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
   const int local = tile - V.tile_base;
@@ -256,6 +365,7 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
     T_final = T_in[pixel];
     lastp = last_in[pixel];
   }
+  __syncthreads();  // the previous tile's last reads of the shared storage are done
   if (tid < 4) s_stop[tid] = 0;
   __syncthreads();
   if (lastp > 0) atomicMax(&s_stop[wave], lastp);
@@ -305,11 +415,13 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
     }
     __syncthreads();
     // this wave's splats of the batch, visited from the back
-    uint64_t mine[2];
-    mine[0] = __ballot(lane < cnt && ((s_mask[lane] >> wave) & 1u));
-    mine[1] = __ballot(lane + 64 < cnt && ((s_mask[lane + 64] >> wave) & 1u));
+    constexpr int kHalves = kBatchB / 64;
+    uint64_t mine[kHalves];
 #pragma unroll
-    for (int h = 1; h >= 0; --h) {
+    for (int h = 0; h < kHalves; ++h)
+      mine[h] = __ballot(lane + 64 * h < cnt && ((s_mask[lane + 64 * h] >> wave) & 1u));
+#pragma unroll
+    for (int h = kHalves - 1; h >= 0; --h) {
       uint64_t bits = mine[h];
       while (bits) {
         const int bit = 63 - __builtin_clzll(bits);
@@ -406,25 +518,144 @@ __global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc
   }
 }
 
+
+// kQueue false: one workgroup per tile, tiles spread over the XCDs in
+// contiguous runs. kQueue true: a persistent grid taking tiles, longest list
+// first, from a counter over `order`.
+template <int kBB, bool kQueue>
+__global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
+                                                  const uint32_t* __restrict__ ranges,
+                                                  const uint32_t* __restrict__ pair_vals,
+                                                  const SplatRec* __restrict__ rec,
+                                                  const float* __restrict__ dL_drgb, const float* __restrict__ T_in,
+                                                  const int* __restrict__ last_in, float* __restrict__ screen,
+                                                  const uint32_t* __restrict__ pair_base, float* __restrict__ partial,
+                                                  uint32_t p_cap, int deterministic, const uint32_t* __restrict__ order,
+                                                  uint32_t* __restrict__ counter) {
+  __shared__ BwdShared<kBB> sh;
+  __shared__ int s_unit;
+  if (!kQueue) {
+    backward_tile<kBB>(p, views, ranges, pair_vals, rec, dL_drgb, T_in, last_in, screen, pair_base, partial, p_cap,
+                       deterministic, xcd_remap(blockIdx.x, p.total_tiles), sh);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int idx = s_unit;
+    __syncthreads();
+    if (idx >= p.total_tiles) break;
+    backward_tile<kBB>(p, views, ranges, pair_vals, rec, dL_drgb, T_in, last_in, screen, pair_base, partial, p_cap,
+                       deterministic, (int)order[idx], sh);
+  }
+}
+
+// Tiles by list length, longest first (key: 0xFFFF - min(length, 0xFFFF)).
+__global__ __launch_bounds__(256) void k_tile_order_keys(const uint32_t* __restrict__ ranges, int total_tiles,
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total_tiles) return;
+  const uint32_t len = ranges[2 * t + 1] - ranges[2 * t];
+  keys[t] = 0xFFFFu - (len < 0xFFFFu ? len : 0xFFFFu);
+  vals[t] = (uint32_t)t;
+}
+
 }  // namespace
 
-hipError_t launch_forward(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
-                          const uint32_t* pair_vals, const SplatRec* rec, float* rgb, float* T, int* last,
-                          hipStream_t stream) {
-  if (p.total_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_forward, dim3(p.total_tiles), dim3(256), 0, stream, p, views, ranges, pair_vals, rec, rgb, T,
-                     last);
+namespace {
+
+template <int kShape>
+int forward_units(int tiles) {
+  return kShape == 0 ? tiles : (kShape == 1 ? 4 * tiles : 2 * tiles);
+}
+
+// Resident workgroups per CU of a kernel (for persistent grids).
+template <typename K>
+int resident_blocks(K kernel, int threads) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+  return n;
+}
+
+template <int kShape, bool kQueue>
+hipError_t forward_variant(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
+                           const uint32_t* pair_vals, const SplatRec* rec, float* rgb, float* T, int* last,
+                           const uint32_t* order, uint32_t* counter, int cus, hipStream_t stream) {
+  constexpr int kThreads = kShape == 1 ? 64 : 256;
+  const int units = forward_units<kShape>(p.total_tiles);
+  static const int per_cu = resident_blocks(k_forward<kShape, kQueue>, kThreads);
+  const int blocks = kQueue ? std::min(units, per_cu * cus) : units;
+  hipLaunchKernelGGL((k_forward<kShape, kQueue>), dim3(blocks), dim3(kThreads), 0, stream, p, views, ranges,
+                     pair_vals, rec, rgb, T, last, order, counter);
   return hipGetLastError();
 }
 
-hipError_t launch_backward(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
-                           const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
-                           const int* last, float* screen, const uint32_t* pair_base, float* partial,
-                           uint32_t p_cap, bool deterministic, hipStream_t stream) {
-  if (p.total_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_backward, dim3(p.total_tiles), dim3(256), 0, stream, p, views, ranges, pair_vals, rec, dL_drgb,
-                     T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0);
+template <int kBB, bool kQueue>
+hipError_t backward_variant(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
+                            const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
+                            const int* last, float* screen, const uint32_t* pair_base, float* partial,
+                            uint32_t p_cap, bool deterministic, const uint32_t* order, uint32_t* counter,
+                            int cus, hipStream_t stream) {
+  static const int per_cu = resident_blocks(k_backward<kBB, kQueue>, 256);
+  const int blocks = kQueue ? std::min(p.total_tiles, per_cu * cus) : p.total_tiles;
+  hipLaunchKernelGGL((k_backward<kBB, kQueue>), dim3(blocks), dim3(256), 0, stream, p, views, ranges, pair_vals, rec,
+                     dL_drgb, T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0, order, counter);
   return hipGetLastError();
+}
+
+}  // namespace
+
+const char* raster_shape_name(int shape) {
+  return shape == 0 ? "16x16" : (shape == 1 ? "8x8" : "32x8");
+}
+
+hipError_t launch_tile_order_keys(const uint32_t* ranges, int total_tiles, uint32_t* keys, uint32_t* vals,
+                                  hipStream_t stream) {
+  if (total_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_order_keys, dim3((unsigned)((total_tiles + 255) / 256)), dim3(256), 0, stream, ranges,
+                     total_tiles, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
+                          const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec, float* rgb, float* T,
+                          int* last, const uint32_t* order, uint32_t* counter, hipStream_t stream) {
+  if (p.total_tiles <= 0) return hipSuccess;
+  if (cfg.fwd_queue) SPLAT_HIP_CHECK(launch_fill_u32(counter, 0u, 1, stream));
+  const int pb = cfg.cus;
+  switch (cfg.fwd_shape * 2 + (cfg.fwd_queue ? 1 : 0)) {
+    case 0: return forward_variant<0, false>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+    case 1: return forward_variant<0, true>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+    case 2: return forward_variant<1, false>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+    case 3: return forward_variant<1, true>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+    case 4: return forward_variant<2, false>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+    case 5: return forward_variant<2, true>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
+                           const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec,
+                           const float* dL_drgb, const float* T, const int* last, float* screen,
+                           const uint32_t* pair_base, float* partial, uint32_t p_cap, bool deterministic,
+                           const uint32_t* order, uint32_t* counter, hipStream_t stream) {
+  if (p.total_tiles <= 0) return hipSuccess;
+  if (cfg.bwd_queue) SPLAT_HIP_CHECK(launch_fill_u32(counter, 0u, 1, stream));
+  const int pb = cfg.cus;
+  switch ((cfg.bwd_batch == 64 ? 0 : 2) + (cfg.bwd_queue ? 1 : 0)) {
+    case 0:
+      return backward_variant<64, false>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
+                                         partial, p_cap, deterministic, order, counter, pb, stream);
+    case 1:
+      return backward_variant<64, true>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
+                                        partial, p_cap, deterministic, order, counter, pb, stream);
+    case 2:
+      return backward_variant<128, false>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
+                                          partial, p_cap, deterministic, order, counter, pb, stream);
+    default:
+      return backward_variant<128, true>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
+                                         partial, p_cap, deterministic, order, counter, pb, stream);
+  }
 }
 
 }  // namespace splat

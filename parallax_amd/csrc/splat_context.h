@@ -83,6 +83,17 @@ struct splat_context {
   // capacities the pair-count-sized buffers were reserved for (kernel bounds)
   unsigned long long P_cap = 0, S_cap = 0;
 
+  // compositing launch shape: the current one, and the tuned one per batch shape
+  splat::RasterConfig rcfg{};
+  bool autotune = false;
+  bool forced = false;  // forced_cfg for every batch (splat_set_raster_config)
+  splat::RasterConfig forced_cfg{};
+  bool tuning_backward = false;  // the forward of this batch was just tuned
+  std::map<std::string, splat::RasterConfig> tuned;
+  std::string shape_key;  // resolutions of the current batch
+  DeviceBuffer order_keys, order_vals, order_keys_alt, order_vals_alt;  // tiles, longest list first
+  uint32_t* tile_order = nullptr;
+
   // captured training steps
   struct Graph {
     std::string key;

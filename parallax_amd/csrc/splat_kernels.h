@@ -104,16 +104,32 @@ struct RasterParams {
   int total_tiles;
   long long per_view;  // slots per view (= gaussian count)
 };
-hipError_t launch_forward(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
-                          const uint32_t* pair_vals, const SplatRec* rec, float* rgb, float* T, int* last,
-                          hipStream_t stream);
+// Launch shape of the compositing kernels (performance only: every choice
+// gives the same image, and the same gradients in deterministic mode).
+struct RasterConfig {
+  int fwd_shape = 0;     // forward workgroup: 0 16x16 (4 waves), 1 8x8 (1 wave), 2 32x8 (4 waves)
+  bool fwd_queue = false;  // persistent work queue, longest tiles first
+  int bwd_batch = 128;   // adjoint: splats staged per batch (64 or 128)
+  bool bwd_queue = false;
+  int cus = 256;         // compute units: persistent grids are sized from it
+};
+const char* raster_shape_name(int shape);
+// keys / vals for a sort of the tiles by list length, longest first.
+hipError_t launch_tile_order_keys(const uint32_t* ranges, int total_tiles, uint32_t* keys, uint32_t* vals,
+                                  hipStream_t stream);
+// `order` (tiles, longest first) and `counter` (one u32, cleared here) are
+// read only by the queue variants.
+hipError_t launch_forward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
+                          const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec, float* rgb, float* T,
+                          int* last, const uint32_t* order, uint32_t* counter, hipStream_t stream);
 // Atomic mode: screen accumulators (kScreenStride floats per slot) via
 // wave-reduced atomics. Deterministic mode: per-pair partials written to
-// partial[pair_base[slot] + local pair] (10 floats each).
-hipError_t launch_backward(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
-                           const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
-                           const int* last, float* screen, const uint32_t* pair_base, float* partial,
-                           uint32_t p_cap, bool deterministic, hipStream_t stream);
+// partial[pair_base[slot] + local pair] (kScreenStride floats each).
+hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
+                           const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec,
+                           const float* dL_drgb, const float* T, const int* last, float* screen,
+                           const uint32_t* pair_base, float* partial, uint32_t p_cap, bool deterministic,
+                           const uint32_t* order, uint32_t* counter, hipStream_t stream);
 
 // ---- loss ----
 struct LossParams {

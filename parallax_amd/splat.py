@@ -50,6 +50,11 @@ class CGrads(C.Structure):
                 ("rest", f32p), ("screen_gradient", f32p), ("visible_count", i32p)]
 
 
+class CRasterConfig(C.Structure):
+    _fields_ = [("forward_shape", C.c_int32), ("forward_queue", C.c_int32), ("backward_batch", C.c_int32),
+                ("backward_queue", C.c_int32)]
+
+
 class CAdam(C.Structure):
     _fields_ = [("lr_position", C.c_float), ("lr_log_scale", C.c_float), ("lr_rotation", C.c_float),
                 ("lr_opacity", C.c_float), ("lr_dc", C.c_float), ("lr_rest", C.c_float), ("beta1", C.c_float),
@@ -94,6 +99,9 @@ SIGNATURES = {
     "splat_densify": (C.c_int, [C.c_void_p, C.POINTER(CDensify), C.c_float, C.c_int32, C.POINTER(C.c_uint32), i32p,
                                 i32p]),
     "splat_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int32]),
+    "splat_set_autotune": (C.c_int, [C.c_void_p, C.c_int32]),
+    "splat_set_raster_config": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "splat_get_raster_config": (C.c_int, [C.c_void_p, C.c_void_p]),
     "splat_reset_opacity": (C.c_int, [C.c_void_p, C.c_float]),
     "splat_download_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
     "splat_prune_and_sort": (C.c_int, [C.c_void_p, C.c_float]),
@@ -372,6 +380,24 @@ class Context:
     # ---- measurement ----
     def set_timing(self, on: bool):
         self._ck(lib().splat_set_timing(self.h, 1 if on else 0))
+
+    def set_autotune(self, on: bool):
+        self._ck(lib().splat_set_autotune(self.h, 1 if on else 0))
+
+    def set_raster_config(self, forward_shape=None, forward_queue=False, backward_batch=128, backward_queue=False):
+        """Force a compositing launch shape; forward_shape None: back to default/tuned."""
+        if forward_shape is None:
+            self._ck(lib().splat_set_raster_config(self.h, None))
+            return
+        c = CRasterConfig(int(forward_shape), 1 if forward_queue else 0, int(backward_batch),
+                          1 if backward_queue else 0)
+        self._ck(lib().splat_set_raster_config(self.h, C.byref(c)))
+
+    def raster_config(self) -> dict:
+        c = CRasterConfig()
+        self._ck(lib().splat_get_raster_config(self.h, C.byref(c)))
+        return {"forward_shape": ("16x16", "8x8", "32x8")[c.forward_shape], "forward_queue": bool(c.forward_queue),
+                "backward_batch": int(c.backward_batch), "backward_queue": bool(c.backward_queue)}
 
     def set_graph_mode(self, on: bool):
         """train_step replays a captured HIP graph per batch shape."""
