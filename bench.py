@@ -48,6 +48,13 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="default compositing launch shape")
     ap.add_argument("--profile", action="store_true",
                     help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
+    ap.add_argument("--workload", default="headline", choices=["headline", "c1", "c3", "c4", "c5"],
+                    help="headline: the benchmark line; c1/c3/c4/c5: the other configurations' measurements")
+    ap.add_argument("--train-steps", type=int, default=2000, help="c4: steps of the densifying run")
+    ap.add_argument("--no-check", action="store_true", help="c4: skip the CPU densify comparison")
+    ap.add_argument("--grad-quantile", type=float, default=None,
+                    help="c4: densify threshold = this quantile of the first call's mean view-space gradients "
+                         "(default: the configuration's 2e-4)")
     return ap.parse_args()
 
 
@@ -68,6 +75,10 @@ def main():
     if not ok:
         raise SystemExit(f"no usable device: {reason}")
     log("device", name)
+    if args.workload != "headline":
+        out = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args, S, syn)
+        print(json.dumps(out), flush=True)
+        return
     n = args.gaussians
     B = args.batch
     t0 = time.time()
@@ -271,6 +282,206 @@ def cpu_baseline(cloud, view):
             "sample": f"1 training step (fwd + L1/D-SSIM + bwd + Adam) on one 1920x1080 view of the same "
                       f"{cloud.n}-gaussian SH3 scene, single-threaded; {secs:.2f} s",
             "host_cpu": cpu_name, "host_threads": os.cpu_count()}
+
+
+
+
+# --------------------------------------------------- the other workloads --
+
+def device_targets(ctx, S, truth, views, deg):
+    ctx.set_capacity(truth.n, truth.sh_degree)
+    ctx.upload_cloud(truth)
+    ctx.upload_views(views)
+    out = []
+    for k in range(len(views)):
+        ctx.render_forward([k], S.options(sh_degree=deg))
+        out.append(ctx.image(0)[0])
+    return out
+
+
+def timed_steps(ctx, S, batches, opts, steps, warmup, it0=0):
+    for w in range(warmup):
+        ctx.train_step(batches[w % len(batches)], opts, 0.2, S.adam_params(it0 + w, 1e-4), want_loss=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.train_step(batches[k % len(batches)], opts, 0.2, S.adam_params(it0 + warmup + k, 1e-4), want_loss=False)
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_c1(args, S, syn):
+    """The reference's CPU-runnable case: 10k SH0, 256x256, one view; GPU
+    step (plain and graph-captured) against the reference trainer's step."""
+    cloud, truth, views = syn.config1()
+    ctx = S.Context(0)
+    tgt = device_targets(ctx, S, truth, views, 0)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views, tgt)
+    opts = S.options(sh_degree=0)
+    plain = timed_steps(ctx, S, [[0]], opts, 200, 10)
+    ctx.set_graph_mode(True)
+    graph = timed_steps(ctx, S, [[0]], opts, 200, 10)
+    ctx.close()
+    cpu = None if args.no_cpu else cpu_step(cloud, views[0], tgt[0], 0, samples=20)
+    return {"workload": "c1: 10k SH0 gaussians, 256x256, one view per step", "plain_ms": round(plain * 1e3, 4),
+            "graph_ms": round(graph * 1e3, 4), "cpu_reference_ms": cpu,
+            "graph_vs_cpu": round(cpu / (graph * 1e3), 1) if cpu else None}
+
+
+def bench_c3(args, S, syn):
+    """500k SH3, 32 orbit views at four resolutions: all 32 in one
+    graph-captured step against the per-view launch loop."""
+    cloud, truth, views = syn.config3()
+    ctx = S.Context(0)
+    ctx.set_autotune(not args.no_autotune)
+    tgt = device_targets(ctx, S, truth, views, 3)
+    ctx.set_capacity(cloud.n, 3)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views, tgt)
+    del tgt
+    opts = S.options(sh_degree=3)
+    everything = [list(range(len(views)))]
+    ctx.set_graph_mode(True)
+    batched = timed_steps(ctx, S, everything, opts, max(3, args.steps // 3), 2)
+    cfg = ctx.raster_config()
+    pairs = ctx.last_pair_count
+    ctx.set_graph_mode(False)
+    singles = [[k] for k in range(len(views))]
+    plain = timed_steps(ctx, S, singles, opts, len(views), 4)
+    ctx.close()
+    px = sum(v.pixels for v in views)
+    return {"workload": "c3: 500k SH3 gaussians, 32 orbit views of 640x480..1920x1080 in one batch",
+            "batched_ms_per_step": round(batched * 1e3, 3), "batched_ms_per_view": round(batched * 1e3 / len(views), 4),
+            "plain_ms_per_view": round(plain * 1e3, 4), "batched_vs_plain_per_view": round(plain * len(views) / batched, 3),
+            "pixels_per_step": px, "pairs_per_step": pairs, "raster_config": cfg,
+            "gaussian_pixels_per_s": float(f"{cloud.n * px / batched:.4g}")}
+
+
+def bench_c4(args, S, syn):
+    """The densifying training run: 300k -> 1M cap, 64-camera 1080p orbit,
+    densify every 100 steps (view-space gradient 2e-4, prune opacity 0.005),
+    each densify call compared with the CPU densify on the same state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cloud = syn.random_cloud(300_000, 0, 3, (-2, -2, -2), (2, 2, 2))
+    truth = syn.random_cloud(1_000_000, 1, 3, (-2, -2, -2), (2, 2, 2))
+    views = syn.orbit_views(64, 6.0, [(1920, 1080)], 2)
+    cap = 1_000_000
+    ctx = S.Context(0)
+    ctx.set_autotune(not args.no_autotune)
+    tgt = device_targets(ctx, S, truth, views, 3)
+    ctx.set_capacity(cap, 3)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views, tgt)
+    del tgt
+    ctx.set_graph_mode(True)
+    cfg = O.default_config(iterations=args.train_steps, max_gaussians=cap, densify_grad_threshold=2e-4,
+                           prune_opacity=0.005)
+    dp = S.CDensify()
+    for k in ("densify_grad_threshold", "percent_dense", "split_scale_divider", "opacity_reset_interval",
+              "prune_opacity", "prune_scale_fraction", "max_gaussians"):
+        setattr(dp, k, getattr(cfg, k))
+    extent = O.camera_extent(views)
+    order = np.random.default_rng(3).permutation(args.train_steps) % len(views)
+    rng_state, gpu_s, checks, sizes = 7, 0.0, [], []
+    ratio = np.float32(cfg.position_lr_final) / np.float32(cfg.position_lr_initial)
+    for it in range(args.train_steps):
+        t = np.float32(it) / np.float32(max(1, args.train_steps - 1))
+        lr = float(np.float32(cfg.position_lr_initial) * np.power(ratio, t, dtype=np.float32) * np.float32(extent))
+        degree = min(3, it // max(1, cfg.sh_increase_interval))
+        t0 = time.perf_counter()
+        ctx.train_step([int(order[it])], S.options(sh_degree=degree), 0.2, S.adam_params(it, lr), want_loss=False)
+        if (it + 1) % 100 == 0 and it + 1 < args.train_steps:
+            if not args.no_check or (args.grad_quantile is not None and it + 1 == 100):
+                ctx.synchronize()
+                gpu_s += time.perf_counter() - t0
+                before = ctx.download_cloud()
+                sg, vc = ctx.stats()
+                if args.grad_quantile is not None and it + 1 == 100:
+                    thr = float(np.quantile(sg / np.maximum(vc, 1), args.grad_quantile))
+                    cfg.densify_grad_threshold = thr
+                    dp.densify_grad_threshold = thr
+                    log(f"c4 densify threshold {thr:.3g} (quantile {args.grad_quantile})")
+                t0 = time.perf_counter()
+            src, counts, state = ctx.densify(dp, extent, it + 1, rng_state)
+            if not args.no_check:
+                gpu_s += time.perf_counter() - t0
+                _, osrc, ocounts, ostate = O.densify(before, sg, vc, cfg, extent, it + 1, rng_state)
+                checks.append(bool(np.array_equal(src, osrc) and np.array_equal(counts, ocounts) and state == ostate))
+                t0 = time.perf_counter()
+            rng_state = state
+            sizes.append(ctx.count)
+            log(f"c4 step {it + 1}: {ctx.count} gaussians (clone {counts[0]}, split {counts[1]}, prune {counts[2]})"
+                + (f", indices equal: {checks[-1]}" if checks else ""))
+        ctx.synchronize()
+        gpu_s += time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "c4: densifying run, 300k -> 1M SH3, 64-camera 1920x1080 orbit, densify every 100 steps",
+            "steps": args.train_steps, "gpu_seconds": round(gpu_s, 3), "steps_per_s": round(args.train_steps / gpu_s, 2),
+            "densify_grad_threshold": float(cfg.densify_grad_threshold),
+            "gaussians_after_each_densify": sizes, "densify_calls_checked": len(checks),
+            "densify_indices_equal_cpu": all(checks) if checks else None}
+
+
+def bench_c5(args, S, syn):
+    """3M SH3 room, 8 interior 1080p views per batch: deterministic and atomic
+    gradient accumulation -- step time, device memory, and run-to-run spread
+    of the gradients."""
+    cloud, truth, views = syn.config5()
+    ctx = S.Context(0)
+    ctx.set_autotune(not args.no_autotune)
+    tgt = device_targets(ctx, S, truth, views, 3)
+    ctx.set_capacity(cloud.n, 3)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views, tgt)
+    del tgt
+    out = {"workload": "c5: 3M SH3 gaussians in a 10x10x3 m room, batch of 8 interior 1920x1080 views"}
+    for det in (True, False):
+        opts = S.options(sh_degree=3, deterministic=det)
+        ctx.set_graph_mode(True)
+        ms = timed_steps(ctx, S, [list(range(len(views)))], opts, max(3, args.steps // 2), 2) * 1e3
+        ctx.set_graph_mode(False)
+        grads = []
+        for r in range(2):
+            ctx.zero_gradients()
+            ctx.render_forward(list(range(len(views))), opts)
+            ctx.loss(0.2)
+            ctx.render_backward()
+            g = ctx.gradients()
+            grads.append(np.concatenate([np.asarray(g[k], np.float64).ravel() for k in
+                                         ("pos", "log_scale", "rot", "opacity", "dc", "rest")]))
+            del g
+        spread = float(np.linalg.norm(grads[0] - grads[1]) / max(1e-30, np.linalg.norm(grads[0])))
+        mine, used = ctx.memory()
+        key = "deterministic" if det else "atomic"
+        out[key] = {"ms_per_step": round(ms, 3), "views_per_s": round(len(views) / ms * 1e3, 2),
+                    "pairs_per_step": ctx.last_pair_count, "context_hbm_gb": round(mine / 1e9, 3),
+                    "device_used_gb": round(used / 1e9, 3), "gradient_rel_l2_run_to_run": spread}
+        del grads
+        log(f"c5 {key}: {out[key]}")
+    ctx.close()
+    return out
+
+
+def cpu_step(cloud, view, tgt, sh, samples=1):
+    """Mean seconds of one reference CPU trainer step (ms)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import oracle as O
+    if not O.ref_available():
+        return None
+    cfg = O.default_config(iterations=1000, sh_degree=sh, sh_increase_interval=0, densify_from_iteration=10**6,
+                           densify_until_iteration=10**6, opacity_reset_interval=0)
+    R = O.ref()
+    k = O.ccloud(cloud)
+    varr = (O.CView * 1)(O.cview(view))
+    img = np.ascontiguousarray(tgt, np.float32)
+    ip = (O.f32p * 1)(O.ptr(img))
+    loss = C.c_float()
+    secs = R.ref_time_steps(C.byref(k.struct), 1, varr, ip, C.byref(cfg), samples, C.byref(loss))
+    return round(secs / samples * 1e3, 3)
 
 
 if __name__ == "__main__":

@@ -249,6 +249,9 @@ splat_status splat_timing_entry(splat_context* ctx, int32_t i, const char** name
 splat_status splat_reset_timing(splat_context* ctx);
 /* Number of (tile, splat) pairs of the last forward batch. */
 int64_t splat_last_pair_count(const splat_context* ctx);
+/* Device memory the context holds (its buffers only grow, so this is also
+ * its peak), and the device's used memory as the runtime reports it. */
+splat_status splat_memory(splat_context* ctx, uint64_t* context_bytes, uint64_t* device_used_bytes);
 splat_status splat_synchronize(splat_context* ctx);
 
 #ifdef __cplusplus

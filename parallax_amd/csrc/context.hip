@@ -1310,6 +1310,20 @@ splat_status splat_reset_timing(splat_context* ctx) {
 
 int64_t splat_last_pair_count(const splat_context* ctx) { return ctx ? (int64_t)ctx->P : 0; }
 
+splat_status splat_memory(splat_context* ctx, uint64_t* context_bytes, uint64_t* device_used_bytes) {
+  if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
+  CK(hipSetDevice(ctx->device), "set device");
+  uint64_t mine = 0;
+  for (DeviceBuffer* b : ctx->all_buffers()) mine += b->bytes;
+  if (context_bytes) *context_bytes = mine;
+  if (device_used_bytes) {
+    size_t free_b = 0, total_b = 0;
+    CK(hipMemGetInfo(&free_b, &total_b), "memory info");
+    *device_used_bytes = (uint64_t)(total_b - free_b);
+  }
+  return SPLAT_OK;
+}
+
 splat_status splat_synchronize(splat_context* ctx) {
   if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
   CK(hipStreamSynchronize(ctx->stream), "sync");

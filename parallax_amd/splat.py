@@ -114,6 +114,7 @@ SIGNATURES = {
                                      C.POINTER(C.c_int64)]),
     "splat_reset_timing": (C.c_int, [C.c_void_p]),
     "splat_last_pair_count": (C.c_int64, [C.c_void_p]),
+    "splat_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "splat_synchronize": (C.c_int, [C.c_void_p]),
 }
 
@@ -415,6 +416,12 @@ class Context:
 
     def reset_timing(self):
         self._ck(lib().splat_reset_timing(self.h))
+
+    def memory(self):
+        """(bytes the context holds, bytes used on the device)."""
+        mine, used = C.c_uint64(), C.c_uint64()
+        self._ck(lib().splat_memory(self.h, C.byref(mine), C.byref(used)))
+        return int(mine.value), int(used.value)
 
     @property
     def last_pair_count(self) -> int:
