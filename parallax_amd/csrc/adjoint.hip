@@ -8,11 +8,17 @@
 // Jacobian (with the halved z-derivative of a clamped entry), scale, and the
 // normalised quaternion. One thread per gaussian; it loops over the views of
 // the batch, so gradients from all views are reduced in registers and written
-// once -- no atomics on parameter gradients. In deterministic mode the
-// screen-space terms of each (view, gaussian) are summed from that splat's
-// per-tile partials in pair order. Gradients are added (the engine's
+// once -- no atomics on parameter gradients. Gradients are added (the engine's
 // accumulate semantics). HBM-bound: reads params (56 + 4 rs B) and 48 B of
 // screen terms per (view, gaussian), read-modify-writes gradients.
+//
+// In deterministic mode the screen terms of each (view, gaussian) are first
+// summed from the per-tile partials the compositing adjoint wrote, in a fixed
+// order that does not depend on scheduling: the partials lie in depth-sorted
+// slot order, each slot's run contiguous, so a wave takes 64 consecutive
+// records, sums each slot's records with a segmented scan across its lanes,
+// and the pieces of runs crossing a 64-record boundary are added in chunk
+// order by a second pass. Balanced by records, however uneven the runs.
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -81,11 +87,7 @@ __device__ __forceinline__ void sh_jacobian(int count, const float d[3], float j
 template <int kDeg>
 __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc* __restrict__ views,
                                                  const float* __restrict__ params, const SplatRec* __restrict__ rec,
-                                                 const float* __restrict__ screen,
-                                                 const uint32_t* __restrict__ pair_base,
-                                                 const uint32_t* __restrict__ counts,
-                                                 const float* __restrict__ partial, uint32_t p_cap,
-                                                 float* __restrict__ grads,
+                                                 const float* __restrict__ screen, float* __restrict__ grads,
                                                  float* __restrict__ sg, int* __restrict__ vc) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.n) return;
@@ -146,21 +148,20 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
     const ViewDesc& V = views[v];
     vc_sum += 1;
     float acc[11];
-    if (p.deterministic) {
-#pragma unroll
-      for (int k = 0; k < 11; ++k) acc[k] = 0.f;
-      const uint32_t base = pair_base[slot], cnt = counts[slot];
-      // past p_cap only when the step's capacity check has flagged it
-      const uint32_t stop = base < p_cap ? (cnt < p_cap - base ? cnt : p_cap - base) : 0u;
-      for (uint32_t q = 0; q < stop; ++q) {
-        const float* src = partial + (long long)(base + q) * kScreenStride;
-#pragma unroll
-        for (int k = 0; k < 11; ++k) acc[k] += src[k];
-      }
-    } else {
-      const float* src = screen + slot * kScreenStride;
-#pragma unroll
-      for (int k = 0; k < 11; ++k) acc[k] = src[k];
+    {
+      const float4* src = reinterpret_cast<const float4*>(screen + slot * kScreenStride);
+      const float4 a = src[0], b = src[1], c = src[2];
+      acc[0] = a.x;
+      acc[1] = a.y;
+      acc[2] = a.z;
+      acc[3] = a.w;
+      acc[4] = b.x;
+      acc[5] = b.y;
+      acc[6] = b.z;
+      acc[7] = b.w;
+      acc[8] = c.x;
+      acc[9] = c.y;
+      acc[10] = c.z;
     }
     sg_sum += acc[6];
     if (acc[10] == 0.f) continue;  // not touched
@@ -292,30 +293,169 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
 
 }  // namespace
 
+namespace {
+
+constexpr int kDrChunk = 64;  // partial records per wave (one per lane)
+
+struct Vals11 {
+  float v[11];
+};
+
+__device__ __forceinline__ void store_screen(float* dst_f, const float (&v)[11]) {
+  float4* dst = reinterpret_cast<float4*>(dst_f);
+  dst[0] = make_float4(v[0], v[1], v[2], v[3]);
+  dst[1] = make_float4(v[4], v[5], v[6], v[7]);
+  dst[2] = make_float4(v[8], v[9], v[10], 0.f);
+}
+
+// Level 1: a wave per 64 consecutive partial records. Records are ordered by
+// depth-sorted slot position, each slot's run contiguous, so the lanes' owners
+// are non-decreasing; a segmented inclusive scan across the wave (fixed shape)
+// sums each owner's records in the chunk. An owner whose run lies inside the
+// chunk is written to its screen slot; the pieces of runs that cross a chunk
+// edge go to carry_first (the chunk's first owner) / carry_last (its last).
+__global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__ sorted_slots,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ p_total, long long m, uint32_t p_cap,
+                                                    const float* __restrict__ partial, float* __restrict__ screen,
+                                                    float* __restrict__ carry_first, float* __restrict__ carry_last) {
+  const int lane = threadIdx.x & 63;
+  const long long c = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t total = min(*p_total, p_cap);  // records past p_cap exist only in a flagged step
+  const long long r0 = c * kDrChunk;
+  if (r0 >= total || m <= 0) return;  // wave-uniform
+  // owner of r0: the last sorted position whose run starts at or before it
+  long long lo = 0, hi = m;  // offsets[lo] <= r0 < offsets[hi] (offsets[m] = total)
+  while (hi - lo > 1) {
+    const long long mid = (lo + hi) >> 1;
+    if (offsets[mid] <= (uint32_t)r0) lo = mid;
+    else hi = mid;
+  }
+  const long long o0 = lo;  // owners of this chunk: o0 .. o0 + 63 (every run holds a record)
+  auto off_at = [&](long long k) -> uint32_t { return k < m ? min(offsets[k], total) : total; };
+  const uint32_t my_off = off_at(o0 + lane);      // run start of candidate owner o0 + lane
+  const uint32_t next_off = off_at(o0 + lane + 1);
+  const long long r = r0 + lane;
+  const bool valid = r < total;
+  // this lane's record's owner: the last candidate whose run starts at or before r
+  int own = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const int cand = own + step;
+    const uint32_t oc = __shfl(my_off, cand & 63, 64);
+    if (cand < 64 && oc <= (uint32_t)r) own = cand;
+  }
+  float v[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) v[k] = 0.f;
+  if (valid) {
+    const float4* src = reinterpret_cast<const float4*>(partial + r * kScreenStride);
+    const float4 a = src[0], b = src[1], cc = src[2];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    v[8] = cc.x; v[9] = cc.y; v[10] = cc.z;
+  }
+  const int key = valid ? own : 64 + lane;  // invalid lanes: segments of their own
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int other = __shfl_up(key, d, 64);
+    const bool join = lane >= d && other == key;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+      const float x = __shfl_up(v[k], d, 64);
+      if (join) v[k] += x;
+    }
+  }
+  const int next_key = __shfl_down(key, 1, 64);
+  // the owner's run bounds, fetched while every lane is still active
+  const uint32_t start = __shfl(my_off, own, 64), stop = __shfl(next_off, own, 64);
+  const bool seg_last = valid && (lane == 63 || next_key != key);
+  if (!seg_last) return;
+  const bool starts_inside = start >= (uint32_t)r0;
+  const bool ends_inside = stop <= (uint32_t)(r0 + kDrChunk);
+  if (starts_inside && ends_inside) {
+    store_screen(screen + (long long)sorted_slots[o0 + own] * kScreenStride, v);
+  } else {
+    if (!starts_inside) store_screen(carry_first + c * kScreenStride, v);
+    if (!ends_inside) store_screen(carry_last + c * kScreenStride, v);
+  }
+}
+
+// Level 2: one thread per depth-sorted position. Runs that crossed a chunk
+// edge are summed from the carries in chunk order; slots without records get
+// zeros.
+__global__ __launch_bounds__(256) void k_det_owners(const uint32_t* __restrict__ sorted_slots,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ p_total, long long m, uint32_t p_cap,
+                                                    const float* __restrict__ carry_first,
+                                                    const float* __restrict__ carry_last, float* __restrict__ screen) {
+  const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= m) return;
+  const uint32_t total = min(*p_total, p_cap);
+  const uint32_t a = min(offsets[o], total), b = o + 1 < m ? min(offsets[o + 1], total) : total;
+  float v[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) v[k] = 0.f;
+  float* dst = screen + (long long)sorted_slots[o] * kScreenStride;
+  if (a == b) {
+    store_screen(dst, v);
+    return;
+  }
+  const long long ca = a / kDrChunk, cb = (b - 1) / kDrChunk;
+  if (ca == cb) return;  // written by the chunk pass
+  auto add = [&](const float* src_f) {
+    const float4* src = reinterpret_cast<const float4*>(src_f);
+    const float4 x = src[0], y = src[1], z = src[2];
+    v[0] += x.x; v[1] += x.y; v[2] += x.z; v[3] += x.w;
+    v[4] += y.x; v[5] += y.y; v[6] += y.z; v[7] += y.w;
+    v[8] += z.x; v[9] += z.y; v[10] += z.z;
+  };
+  add(carry_last + ca * kScreenStride);
+  for (long long c = ca + 1; c <= cb; ++c) add(carry_first + c * kScreenStride);
+  store_screen(dst, v);
+}
+
+}  // namespace
+
+long long det_reduce_carry_floats(unsigned long long p_cap) {
+  return 2LL * kScreenStride * (long long)((p_cap + kDrChunk - 1) / kDrChunk + 1);
+}
+
+hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
+                             long long m, uint32_t p_cap, const float* partial, float* screen, float* carry,
+                             hipStream_t stream) {
+  if (m <= 0) return hipSuccess;
+  const long long chunks = ((long long)p_cap + kDrChunk - 1) / kDrChunk + 1;
+  float* carry_first = carry;
+  float* carry_last = carry + kScreenStride * chunks;
+  if (p_cap > 0) {
+    hipLaunchKernelGGL(k_det_chunks, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, stream, sorted_slots, offsets,
+                       p_total, m, p_cap, partial, screen, carry_first, carry_last);
+    SPLAT_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_det_owners, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
+                     p_total, m, p_cap, carry_first, carry_last, screen);
+  return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
-                          const float* screen, const uint32_t* pair_base, const uint32_t* counts,
-                          const float* partial, uint32_t p_cap, float* grads, float* sg, int* vc,
-                          hipStream_t stream) {
+                          const float* screen, float* grads, float* sg, int* vc, hipStream_t stream) {
   if (p.n <= 0) return hipSuccess;
   if (3 * (p.sh_use == 0 ? 0 : (p.sh_use == 1 ? 3 : (p.sh_use == 2 ? 8 : 15))) > p.rest_stride)
     return hipErrorInvalidValue;
   const dim3 grid((p.n + 255) / 256);
   switch (p.sh_use) {
     case 0:
-      hipLaunchKernelGGL(k_adjoint<0>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, p_cap, grads, sg, vc);
+      hipLaunchKernelGGL(k_adjoint<0>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
       break;
     case 1:
-      hipLaunchKernelGGL(k_adjoint<1>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, p_cap, grads, sg, vc);
+      hipLaunchKernelGGL(k_adjoint<1>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
       break;
     case 2:
-      hipLaunchKernelGGL(k_adjoint<2>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, p_cap, grads, sg, vc);
+      hipLaunchKernelGGL(k_adjoint<2>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
       break;
     default:
-      hipLaunchKernelGGL(k_adjoint<3>, grid, dim3(256), 0, stream, p, views, params, rec, screen, pair_base, counts,
-                         partial, p_cap, grads, sg, vc);
+      hipLaunchKernelGGL(k_adjoint<3>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
       break;
   }
   return hipGetLastError();

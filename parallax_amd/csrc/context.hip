@@ -129,7 +129,7 @@ std::vector<DeviceBuffer*> splat_context::all_buffers() {
   return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
           &vals_alt, &counts, &offsets, &pair_base, &scounts, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
-          &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &scan_ws, &hist_ws, &scalars,
+          &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &carry, &scan_ws, &hist_ws, &scalars,
           &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam,
           &order_keys, &order_vals, &order_keys_alt, &order_vals_alt};
 }
@@ -294,10 +294,11 @@ splat_status reserve_pairs(splat_context* ctx, unsigned long long p_cap, unsigne
   CK(ctx->ensure_sort_workspace(std::max<long long>(std::max<long long>((long long)s_cap, ctx->M),
                                                     (long long)(64ull * ns_max + 1))),
      "alloc sort workspace");
-  if (ctx->opts.deterministic)
+  if (ctx->opts.deterministic) {
     CK(ctx->partial.reserve(sizeof(float) * kScreenStride * p_cap), "alloc partials");
-  else
-    CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, ctx->M)), "alloc screen");
+    CK(ctx->carry.reserve(sizeof(float) * det_reduce_carry_floats(p_cap)), "alloc carries");
+  }
+  CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, ctx->M)), "alloc screen");
   return SPLAT_OK;
 }
 
@@ -525,11 +526,13 @@ splat_status loss_impl(splat_context* ctx, float w, float* losses) {
 splat_status enqueue_backward(splat_context* ctx) {
   const long long M = ctx->M;
   const bool det = ctx->opts.deterministic != 0;
+  CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, M)), "alloc screen");
   if (det) {
     CK(ctx->partial.reserve(sizeof(float) * kScreenStride * std::max<unsigned long long>(1, ctx->P_cap)),
        "alloc partials");
+    CK(ctx->carry.reserve(sizeof(float) * det_reduce_carry_floats(std::max<unsigned long long>(1, ctx->P_cap))),
+       "alloc carries");
   } else {
-    CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, M)), "alloc screen");
     TIMED("screen_clear", launch_fill_f32(ctx->screen.f(), 0.f, kScreenStride * M, ctx->stream));
   }
   const uint32_t p_cap = (uint32_t)ctx->P_cap;
@@ -569,9 +572,11 @@ splat_status enqueue_backward(splat_context* ctx) {
   ap.sh_use = ctx->sh_use;
   ap.n_views = (int)ctx->batch_ids.size();
   ap.deterministic = det ? 1 : 0;
+  if (det)
+    TIMED("det_reduce", launch_det_reduce(ctx->sorted_slots, ctx->offsets.u(), ctx->scalars.u(), M, p_cap,
+                                          ctx->partial.f(), ctx->screen.f(), ctx->carry.f(), ctx->stream));
   TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
-                                  ctx->pair_base.u(), ctx->counts.u(), ctx->partial.f(), p_cap, ctx->grads.f(),
-                                  ctx->sg.f(), ctx->vc.i(), ctx->stream));
+                                  ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream));
   return SPLAT_OK;
 }
 

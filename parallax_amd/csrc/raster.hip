@@ -379,9 +379,11 @@ __device__ __forceinline__ void backward_tile(const RasterParams& p, const ViewD
       const SplatRec& r = rec[slot];
       const uint32_t pid = pair_base[slot] + (uint32_t)((ty - r.y0) * (r.x1 - r.x0 + 1) + (tx - r.x0));
       if (pid >= p_cap) continue;  // only when the step's capacity check has flagged it
-      float* dst = partial + (long long)pid * kScreenStride;
-#pragma unroll
-      for (int k = 0; k < kScreenStride; ++k) dst[k] = 0.f;
+      float4* dst = reinterpret_cast<float4*>(partial + (long long)pid * kScreenStride);
+      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+      dst[0] = zero;
+      dst[1] = zero;
+      dst[2] = zero;
     }
   }
   const float bg_term = p.bg[0] * dL0 + p.bg[1] * dL1 + p.bg[2] * dL2;
@@ -501,11 +503,11 @@ __device__ __forceinline__ void backward_tile(const RasterParams& p, const ViewD
           for (int k = 0; k < kVals; ++k) sum[k] += s_part[w][tid][k];
         }
       if (deterministic) {
-        if (s_dst[tid] < p_cap) {
-          float* dst = partial + (long long)s_dst[tid] * kScreenStride;
-#pragma unroll
-          for (int k = 0; k < kVals; ++k) dst[k] = sum[k];
-          dst[kVals] = 0.f;
+        if (s_dst[tid] < p_cap) {  // 48-byte records: three 16-byte stores
+          float4* dst = reinterpret_cast<float4*>(partial + (long long)s_dst[tid] * kScreenStride);
+          dst[0] = make_float4(sum[0], sum[1], sum[2], sum[3]);
+          dst[1] = make_float4(sum[4], sum[5], sum[6], sum[7]);
+          dst[2] = make_float4(sum[8], sum[9], sum[10], 0.f);
         }
       } else if (m) {
         float* dst = screen + (long long)s_dst[tid] * kScreenStride;

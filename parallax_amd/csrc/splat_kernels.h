@@ -159,9 +159,15 @@ struct AdjointParams {
   int deterministic;
 };
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
-                          const float* screen, const uint32_t* pair_base, const uint32_t* counts,
-                          const float* partial, uint32_t p_cap, float* grads, float* sg, int* vc,
-                          hipStream_t stream);
+                          const float* screen, float* grads, float* sg, int* vc, hipStream_t stream);
+// Deterministic mode: screen[slot] = the slot's per-tile partials (its run
+// [offsets[i], offsets[i+1]) of depth-sorted position i, the last run ending
+// at *p_total, all clamped to p_cap) summed in a fixed order. `carry` holds
+// det_reduce_carry_floats(p_cap) floats.
+long long det_reduce_carry_floats(unsigned long long p_cap);
+hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
+                             long long m, uint32_t p_cap, const float* partial, float* screen, float* carry,
+                             hipStream_t stream);
 
 // ---- optimiser ----
 // The per-step scalars, read from device memory so a captured step can take
