@@ -47,16 +47,11 @@ __device__ __forceinline__ Groups groups_of(const AdamShape& s) {
   return G;
 }
 
-__global__ __launch_bounds__(256) void k_adam(AdamShape shape, const AdamStep* __restrict__ step,
-                                              const uint32_t* __restrict__ skip, float* __restrict__ params,
-                                              float* __restrict__ grads, float* __restrict__ m,
-                                              float* __restrict__ v, float* __restrict__ sg, int* __restrict__ vc,
-                                              float* __restrict__ stat_sg, int* __restrict__ stat_vc) {
-  if (skip && *skip) return;
-  const AdamStep P = *step;
-  const Groups G = groups_of(shape);
-  const long long v0 = 4 * ((long long)blockIdx.x * blockDim.x + threadIdx.x);
-  if (v0 >= G.vbase[6]) return;
+// One 4-element unit of the virtual index space.
+__device__ __forceinline__ void adam_unit(const Groups& G, const AdamStep& P, long long v0, float* __restrict__ params,
+                                          float* __restrict__ grads, float* __restrict__ m, float* __restrict__ v,
+                                          float* __restrict__ sg, int* __restrict__ vc, float* __restrict__ stat_sg,
+                                          int* __restrict__ stat_vc) {
   int g = 0;
 #pragma unroll
   for (int k = 1; k < 6; ++k)
@@ -119,6 +114,21 @@ __global__ __launch_bounds__(256) void k_adam(AdamShape shape, const AdamStep* _
   }
 }
 
+// A resident grid striding over the 4-element units.
+__global__ __launch_bounds__(256) void k_adam(AdamShape shape, const AdamStep* __restrict__ step,
+                                              const uint32_t* __restrict__ skip, float* __restrict__ params,
+                                              float* __restrict__ grads, float* __restrict__ m,
+                                              float* __restrict__ v, float* __restrict__ sg, int* __restrict__ vc,
+                                              float* __restrict__ stat_sg, int* __restrict__ stat_vc) {
+  if (skip && *skip) return;
+  const AdamStep P = *step;
+  const Groups G = groups_of(shape);
+  const long long units = G.vbase[6] / 4;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x; u < units; u += stride)
+    adam_unit(G, P, 4 * u, params, grads, m, v, sg, vc, stat_sg, stat_vc);
+}
+
 }  // namespace
 
 hipError_t launch_adam(const AdamShape& s, const AdamStep* step, const uint32_t* skip_flag, float* params,
@@ -128,9 +138,10 @@ hipError_t launch_adam(const AdamShape& s, const AdamStep* step, const uint32_t*
   const long long n = s.n;
   const long long virt = ((3 * n + 3) & ~3LL) * 3 + 4 * n + ((n + 3) & ~3LL) + (((long long)s.rest_stride * n + 3) & ~3LL);
   const long long threads = virt / 4;
-  const unsigned blocks = (unsigned)((threads + 255) / 256);
-  hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(256), 0, stream, s, step, skip_flag, params, grads, m, v, sg, vc,
-                     stat_sg, stat_vc);
+  long long blocks = (threads + 255) / 256;
+  if (blocks > 4096) blocks = 4096;  // 16 resident workgroups per CU on 256 CUs, grid-stride beyond
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, stream, s, step, skip_flag, params, grads, m, v,
+                     sg, vc, stat_sg, stat_vc);
   return hipGetLastError();
 }
 

@@ -126,28 +126,26 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
     }
     wave_sync_lds();
     const int cnt = __popcll(m);
-    if (!done) {
-      for (int j = 0; j < cnt; ++j) {
-        const float4 sa = s_a[j];
-        const float4 sb = s_b[j];
-        const float dx = fx - sa.x, dy = fy - sa.y;
-        const float power = -0.5f * (sa.z * dx * dx + sb.x * dy * dy) - sa.w * dx * dy;
-        if (power > 0.f) continue;
-        const float raw = sb.y * det_exp_nonpos(power);
-        const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-        if (alpha < p.min_alpha) continue;
-        const float2 sc = s_c[j];
-        const float w = T * alpha;
-        acc0 += w * sb.z;
-        acc1 += w * sb.w;
-        acc2 += w * sc.x;
-        T *= 1.f - alpha;
-        last = __float_as_int(sc.y) + 1;
-        if (T < p.cutoff) {
-          done = true;
-          break;
-        }
-      }
+    // Branch-free over the chunk: every lane evaluates every splat and keeps
+    // the result only where the engine's loop would (not yet terminated,
+    // power <= 0, alpha >= min_alpha); a kept step is the same float
+    // operations on the same operands, a dropped one adds an exact zero.
+    for (int j = 0; j < cnt; ++j) {
+      const float4 sa = s_a[j];
+      const float4 sb = s_b[j];
+      const float2 sc = s_c[j];
+      const float dx = fx - sa.x, dy = fy - sa.y;
+      const float power = -0.5f * (sa.z * dx * dx + sb.x * dy * dy) - sa.w * dx * dy;
+      const float raw = sb.y * det_exp_nonpos(power < 0.f ? power : 0.f);
+      const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
+      const bool use = !done && !(power > 0.f) && !(alpha < p.min_alpha);
+      const float w = use ? T * alpha : 0.f;
+      acc0 += w * sb.z;
+      acc1 += w * sb.w;
+      acc2 += w * sc.x;
+      T = use ? T * (1.f - alpha) : T;
+      last = use ? __float_as_int(sc.y) + 1 : last;
+      done = done || (use && T < p.cutoff);
     }
     wave_sync_lds();
   }
@@ -430,62 +428,61 @@ __device__ __forceinline__ void backward_tile(const RasterParams& p, const ViewD
         bits &= ~(1ull << bit);
         const int j = 64 * h + bit;
         const uint32_t e = lo + j;
-        bool contrib = inside && (int)(e - begin) < lastp;
-        float val[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) val[k] = 0.f;
         const float4 a = s_a[j];
         const float4 b = s_b[j];
         // the forward's decisions, recomputed with its exact float operations
-        float dx = 0.f, dy = 0.f, G = 0.f, raw = 0.f, alpha = 0.f;
-        if (contrib) {
-          dx = fx - a.x;
-          dy = fy - a.y;
-          const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-          if (power > 0.f) {
-            contrib = false;
-          } else {
-            G = det_exp_nonpos(power);
-            raw = b.y * G;
-            alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-            if (alpha < p.min_alpha) contrib = false;
-          }
-        }
-        if (contrib) {
+        const float dx = fx - a.x, dy = fy - a.y;
+        const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+        const float G = det_exp_nonpos(power < 0.f ? power : 0.f);
+        const float raw = b.y * G;
+        const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
+        const bool contrib =
+            inside && (int)(e - begin) < lastp && !(power > 0.f) && !(alpha < p.min_alpha);
+        float z = 0.f;
+        if (__ballot(contrib) != 0ull) {
+          // every lane computes, contributing lanes keep: the same expressions
+          // as a branch per lane, with exact zeros and unchanged state elsewhere
+          float val[16];
+#pragma unroll
+          for (int k = 11; k < 16; ++k) val[k] = 0.f;
+          {
 #pragma clang fp contract(fast)
-          const float c2 = s_c[j];
-          const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-          T = T * inv;
-          const float weight = alpha * T;
-          val[0] = weight * dL0;
-          val[1] = weight * dL1;
-          val[2] = weight * dL2;
-          suf0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
-          suf1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
-          suf2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
-          lc0 = b.z;
-          lc1 = b.w;
-          lc2 = c2;
-          float dL_dalpha = ((b.z - suf0) * dL0 + (b.w - suf1) * dL1 + (c2 - suf2) * dL2) * T;
-          last_alpha = alpha;
-          dL_dalpha -= T_final * inv * bg_term;
-          if (raw < kMaxAlpha) {
-            val[3] = G * dL_dalpha;
+            const float c2 = s_c[j];
+            const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+            const float Tn = T * inv;
+            const float weight = alpha * Tn;
+            const float sn0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
+            const float sn1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
+            const float sn2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
+            float dL_dalpha = ((b.z - sn0) * dL0 + (b.w - sn1) * dL1 + (c2 - sn2) * dL2) * Tn;
+            dL_dalpha -= T_final * inv * bg_term;
+            const bool shape = contrib && raw < kMaxAlpha;
             const float dL_dweight = b.y * dL_dalpha;
             const float gx = G * dx, gy = G * dy;
             const float mx = dL_dweight * (-gx * a.z - gy * a.w);
             const float my = dL_dweight * (-gy * b.x - gx * a.w);
-            val[4] = -mx;
-            val[5] = -my;
-            val[6] = __builtin_amdgcn_sqrtf(mx * mx + my * my);  // densify statistic only: hardware sqrt
-            val[7] = -0.5f * dL_dweight * gx * dx;
-            val[8] = -dL_dweight * gx * dy;
-            val[9] = -0.5f * dL_dweight * gy * dy;
-            val[10] = 1.f;
+            val[0] = contrib ? weight * dL0 : 0.f;
+            val[1] = contrib ? weight * dL1 : 0.f;
+            val[2] = contrib ? weight * dL2 : 0.f;
+            val[3] = shape ? G * dL_dalpha : 0.f;
+            val[4] = shape ? -mx : 0.f;
+            val[5] = shape ? -my : 0.f;
+            val[6] = shape ? __builtin_amdgcn_sqrtf(mx * mx + my * my) : 0.f;  // densify statistic: hardware sqrt
+            val[7] = shape ? -0.5f * dL_dweight * gx * dx : 0.f;
+            val[8] = shape ? -dL_dweight * gx * dy : 0.f;
+            val[9] = shape ? -0.5f * dL_dweight * gy * dy : 0.f;
+            val[10] = shape ? 1.f : 0.f;
+            T = contrib ? Tn : T;
+            suf0 = contrib ? sn0 : suf0;
+            suf1 = contrib ? sn1 : suf1;
+            suf2 = contrib ? sn2 : suf2;
+            lc0 = contrib ? b.z : lc0;
+            lc1 = contrib ? b.w : lc1;
+            lc2 = contrib ? c2 : lc2;
+            last_alpha = contrib ? alpha : last_alpha;
           }
+          z = butterfly16(val, lane);
         }
-        float z = 0.f;
-        if (__ballot(contrib) != 0ull) z = butterfly16(val, lane);
         if (writer) s_part[wave][j][my_slot] = z;
       }
     }

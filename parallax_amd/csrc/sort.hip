@@ -41,23 +41,48 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
   return x - v + before;
 }
 
+// One 4096-item tile per workgroup. Items are loaded coalesced (item
+// r * 256 + thread) into a padded LDS tile, each thread then scans its 16
+// consecutive items serially, one workgroup scan combines the thread totals,
+// and the prefixes go back out coalesced. (Every read precedes the first
+// barrier, so the scan may run in place.)
+__device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* in, const uint32_t* idx, uint32_t* out,
                                                            long long n, uint32_t* block_sums, uint32_t* total) {
   __shared__ uint32_t lds[kScanThreads / 64];
+  __shared__ uint32_t tile[kScanTile + kScanTile / 32];
   const long long base = (long long)blockIdx.x * kScanTile;
-  uint32_t running = 0;
+  const int tid = threadIdx.x;
+#pragma unroll
   for (int r = 0; r < kScanItems; ++r) {
-    const long long j = base + (long long)r * kScanThreads + threadIdx.x;
-    uint32_t v = 0;
-    if (j < n) v = in[idx ? idx[j] : j];
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, &tot, lds);
-    if (j < n) out[j] = running + ex;
-    running += tot;
+    const long long j = base + (long long)r * kScanThreads + tid;
+    tile[scan_pad(r * kScanThreads + tid)] = j < n ? in[idx ? idx[j] : j] : 0u;
   }
-  if (threadIdx.x == 0) {
-    if (block_sums) block_sums[blockIdx.x] = running;
-    if (total && gridDim.x == 1) *total = running;
+  __syncthreads();
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = tile[scan_pad(tid * kScanItems + k)];
+    sum += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(sum, &tot, lds);  // has barriers: the tile reads above are done
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    tile[scan_pad(tid * kScanItems + k)] = run;
+    run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScanItems; ++r) {
+    const long long j = base + (long long)r * kScanThreads + tid;
+    if (j < n) out[j] = tile[scan_pad(r * kScanThreads + tid)];
+  }
+  if (tid == 0) {
+    if (block_sums) block_sums[blockIdx.x] = tot;
+    if (total && gridDim.x == 1) *total = tot;
   }
 }
 

@@ -89,8 +89,13 @@ SPLAT_EXP_QUAL float exp(float x) {
 }
 
 /// exp(x) for x <= 0 and not NaN (the compositing loop's Gaussian falloff):
-/// the same bits as exp() on that domain, without its other range checks.
-SPLAT_EXP_QUAL float exp_nonpos(float x) { return x <= -80.0f ? 0.0f : exp_core(x); }
+/// the same bits as exp() on that domain, without its other range checks and
+/// without a branch (the core runs on a clamped argument, the result is
+/// selected).
+SPLAT_EXP_QUAL float exp_nonpos(float x) {
+  const float e = exp_core(x > -80.0f ? x : -80.0f);
+  return x <= -80.0f ? 0.0f : e;
+}
 
 /// Logistic function written with the same exponential.
 SPLAT_EXP_QUAL float sigmoid(float x) { return 1.0f / (1.0f + exp(-x)); }
