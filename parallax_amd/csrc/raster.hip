@@ -271,7 +271,7 @@ __device__ __forceinline__ float xchg(float v) {
 // Sum 16 per-lane values over the wave64 in one transposed butterfly: each
 // level exchanges half of the values a lane holds with its partner and keeps
 // the other half, so 16 values -> 1 in four DPP levels (lane bits 0-3), then
-// two cross-row steps. Every lane ends with the wave total of one value: slot
+// two cross-row permlane-swap steps. Every lane ends with the wave total of one value: slot
 // bitrev4(lane & 15). The summation order is fixed, so it is deterministic.
 __device__ __forceinline__ float butterfly16(const float (&v)[16], int lane) {
   const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
@@ -297,8 +297,13 @@ __device__ __forceinline__ float butterfly16(const float (&v)[16], int lane) {
   const float send = b3 ? y[0] : y[1];
   const float keep = b3 ? y[1] : y[0];
   float z = keep + xchg<3>(send);
-  z += __shfl_xor(z, 16, 64);
-  z += __shfl_xor(z, 32, 64);
+  // across rows: v_permlane16_swap / v_permlane32_swap with both operands z
+  // leave z[l] and z[l ^ 16] (then z[l ^ 32]) in the two results -- a VALU
+  // exchange, no LDS round trip
+  const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(z), __float_as_uint(z), false, false);
+  z = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);
+  const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(z), __float_as_uint(z), false, false);
+  z = __uint_as_float(r32[0]) + __uint_as_float(r32[1]);
   return z;
 }
 
