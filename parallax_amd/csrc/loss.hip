@@ -37,6 +37,7 @@ constexpr int kTaps = kRun + 2 * kR;      // 14 window values per run
 constexpr int kPitch = kWinW + 1;         // 75: odd, conflict-free row walks
 constexpr int kHPitch = kLW + 1;          // 65
 constexpr int kRowTasks = kWinH * (kLW / kRun);  // 416 row-pass runs
+constexpr int kWinPerThread = (kWinH * kWinW + 255) / 256;  // window values each of 256 threads fetches
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 
@@ -129,20 +130,35 @@ __global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc
   const float* tgt = targets + V.target_base;
   const int gx = L.ox + j;
   float l1 = 0.f, ssim = 0.f;
-  for (int c = 0; c < 3; ++c) {
-    for (int idx = tid; idx < kWinH * kWinW; idx += blockDim.x) {
+  // each channel's image/target window, prefetched into registers while the
+  // previous channel is blurred
+  float px_[kWinPerThread], py_[kWinPerThread];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int r = 0; r < kWinPerThread; ++r) {
+      const int idx = tid + r * 256;
       const int hy = idx / kWinW, hx = idx % kWinW;
       const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
-      float x = 0.f, y = 0.f;
-      if (qx >= 0 && qx < V.width && qy >= 0 && qy < V.height) {
+      px_[r] = py_[r] = 0.f;
+      if (idx < kWinH * kWinW && qx >= 0 && qx < V.width && qy >= 0 && qy < V.height) {
         const long long q = (long long)qy * V.width + qx;
-        x = img[3 * q + c];
-        y = tgt[3 * q + c];
+        px_[r] = img[3 * q + c];
+        py_[r] = tgt[3 * q + c];
       }
-      s_x[hy][hx] = x;
-      s_y[hy][hx] = y;
+    }
+  };
+  fetch(0);
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int r = 0; r < kWinPerThread; ++r) {
+      const int idx = tid + r * 256;
+      if (idx < kWinH * kWinW) {
+        s_x[idx / kWinW][idx % kWinW] = px_[r];
+        s_y[idx / kWinW][idx % kWinW] = py_[r];
+      }
     }
     __syncthreads();
+    if (c + 1 < 3) fetch(c + 1);
     float st[5][kRun];
     if (w > 0.f) {
 #pragma unroll
@@ -222,54 +238,69 @@ __global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc*
   const float* img = rgb + 3 * V.pixel_base;
   const float* tgt = targets + V.target_base;
   const int gx = L.ox + j;
-  for (int c = 0; c < 3; ++c) {
-    float d[kRun] = {0.f, 0.f, 0.f, 0.f};
-    float xs[kRun], ys[kRun];
+  // the 15 partial planes' windows, each prefetched into registers while the
+  // previous one is blurred
+  float pre[kWinPerThread];
+  auto fetch = [&](int q) {
+    const float* plane = terms + (long long)q * total_px + V.pixel_base;
 #pragma unroll
-    for (int k = 0; k < kRun; ++k) {
-      const int gy = L.oy + r0 + k;
-      xs[k] = ys[k] = 0.f;
-      if (gx < V.width && gy < V.height) {
-        const long long q = (long long)gy * V.width + gx;
-        xs[k] = img[3 * q + c];
-        ys[k] = tgt[3 * q + c];
-      }
+    for (int r = 0; r < kWinPerThread; ++r) {
+      const int idx = tid + r * 256;
+      const int hy = idx / kWinW, hx = idx % kWinW;
+      const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
+      pre[r] = (idx < kWinH * kWinW && qx >= 0 && qx < V.width && qy >= 0 && qy < V.height)
+                   ? plane[(long long)qy * V.width + qx]
+                   : 0.f;
     }
-#pragma unroll
-    for (int pl = 0; pl < 5; ++pl) {
-      const float* plane = terms + (long long)(c * 5 + pl) * total_px + V.pixel_base;
-      for (int idx = tid; idx < kWinH * kWinW; idx += blockDim.x) {
-        const int hy = idx / kWinW, hx = idx % kWinW;
-        const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
-        s_t[hy][hx] = (qx >= 0 && qx < V.width && qy >= 0 && qy < V.height)
-                          ? plane[(long long)qy * V.width + qx]
-                          : 0.f;
-      }
-      __syncthreads();
-      row_pass(0, s_t, s_t, p.kernel, s_h);
-      __syncthreads();
-      float bl[kRun];
-      col_pass(s_h, p.kernel, j, r0, bl);
+  };
+  fetch(0);
+  float d[kRun], xs[kRun], ys[kRun];
+  for (int q = 0; q < 15; ++q) {
+    const int c = q / 5, pl = q % 5;
+    if (pl == 0) {
 #pragma unroll
       for (int k = 0; k < kRun; ++k) {
-        if (pl == 0) d[k] += bl[k];
-        else if (pl == 1) d[k] += 2.f * xs[k] * bl[k];
-        else if (pl == 2) d[k] -= 2.f * bl[k];
-        else if (pl == 3) d[k] += ys[k] * bl[k];
-        else d[k] -= bl[k];
+        const int gy = L.oy + r0 + k;
+        d[k] = xs[k] = ys[k] = 0.f;
+        if (gx < V.width && gy < V.height) {
+          const long long qq = (long long)gy * V.width + gx;
+          xs[k] = img[3 * qq + c];
+          ys[k] = tgt[3 * qq + c];
+        }
       }
-      __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < kRun; ++k) {
-      const int gy = L.oy + r0 + k;
-      if (gx >= V.width || gy >= V.height) continue;
-      const float difference = xs[k] - ys[k];
-      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
-      float g = (1.f - w) * sign * inverse;
-      g -= w * d[k];
-      dL[3 * (V.pixel_base + (long long)gy * V.width + gx) + c] = g;
+    for (int r = 0; r < kWinPerThread; ++r) {
+      const int idx = tid + r * 256;
+      if (idx < kWinH * kWinW) s_t[idx / kWinW][idx % kWinW] = pre[r];
     }
+    __syncthreads();
+    if (q + 1 < 15) fetch(q + 1);
+    row_pass(0, s_t, s_t, p.kernel, s_h);
+    __syncthreads();
+    float bl[kRun];
+    col_pass(s_h, p.kernel, j, r0, bl);
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) {
+      if (pl == 0) d[k] += bl[k];
+      else if (pl == 1) d[k] += 2.f * xs[k] * bl[k];
+      else if (pl == 2) d[k] -= 2.f * bl[k];
+      else if (pl == 3) d[k] += ys[k] * bl[k];
+      else d[k] -= bl[k];
+    }
+    if (pl == 4) {
+#pragma unroll
+      for (int k = 0; k < kRun; ++k) {
+        const int gy = L.oy + r0 + k;
+        if (gx >= V.width || gy >= V.height) continue;
+        const float difference = xs[k] - ys[k];
+        const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+        float g = (1.f - w) * sign * inverse;
+        g -= w * d[k];
+        dL[3 * (V.pixel_base + (long long)gy * V.width + gx) + c] = g;
+      }
+    }
+    __syncthreads();
   }
 }
 
