@@ -131,7 +131,9 @@ def main():
     log(f"batched: {batched_s * 1e3:.2f} ms per step of {B} views ({cfg_batched})")
     if args.profile:
         ctx.close()
-        print(json.dumps({"profile": True, "batched_ms_per_step": round(batched_s * 1e3, 3), "batch_views": B}))
+        calibration_copy()
+        print(json.dumps({"profile": True, "batched_ms_per_step": round(batched_s * 1e3, 3), "batch_views": B,
+                          "calibration_copy_bytes": CALIB_BYTES}))
         return
 
     # ---- plain per-view launch sequence ----
@@ -236,6 +238,31 @@ def main():
     if cpu and cpu.get("value"):
         line["vs_cpu_per_view"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+
+
+CALIB_BYTES = 256 << 20
+
+
+def calibration_copy():
+    """One plain device-to-device copy of CALIB_BYTES (hipMemcpy between two
+    hipMalloc buffers: the runtime's __amd_rocclr_copyBuffer kernel), so a
+    profiler run of --profile holds a kernel of known HBM traffic to scale
+    FETCH_SIZE and WRITE_SIZE by (parallax_amd/rocprof_summary.py)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(CALIB_BYTES)) != 0:
+            raise RuntimeError("hipMalloc failed for the calibration copy")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(CALIB_BYTES)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed for the calibration copy")
+        if hip.hipMemcpy(dst, src, C.c_size_t(CALIB_BYTES), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemcpy (device to device) failed for the calibration copy")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
 
 
 def cpu_baseline(cloud, view):
