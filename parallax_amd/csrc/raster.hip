@@ -83,6 +83,40 @@ __device__ __forceinline__ bool box_hits(float mx, float my, float ex, float ey,
   return mx + ex >= q.bx0 && mx - ex <= q.bx1 && my + ey >= q.by0 && my - ey <= q.by1;
 }
 
+// The pixel-centre bounds of a tile's two quadrant columns and two quadrant
+// rows, as wave-uniform values in scalar registers (the float conversions run
+// on the vector unit; readfirstlane moves the results back to SGPRs so they
+// do not hold vector registers across the compositing loop).
+struct TileQuads {
+  float x0[2], x1[2], y0[2], y1[2];
+  bool xv[2], yv[2];  // column / row inside the image
+};
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ TileQuads tile_quads(const ViewDesc& V, int tx, int ty) {
+  TileQuads t;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int x0 = tx * kTile + 8 * c, y0 = ty * kTile + 8 * c;
+    t.xv[c] = x0 < V.width;
+    t.yv[c] = y0 < V.height;
+    t.x0[c] = uniform_f((float)x0 + 0.5f);
+    t.x1[c] = uniform_f((float)min(x0 + 7, V.width - 1) + 0.5f);
+    t.y0[c] = uniform_f((float)y0 + 0.5f);
+    t.y1[c] = uniform_f((float)min(y0 + 7, V.height - 1) + 0.5f);
+  }
+  return t;
+}
+// Bit k set: the cull box meets quadrant k (k = 2 * row + column).
+__device__ __forceinline__ uint32_t quad_hits(float mx, float my, float ex, float ey, const TileQuads& t) {
+  const bool c0 = t.xv[0] && mx + ex >= t.x0[0] && mx - ex <= t.x1[0];
+  const bool c1 = t.xv[1] && mx + ex >= t.x0[1] && mx - ex <= t.x1[1];
+  const bool r0 = t.yv[0] && my + ey >= t.y0[0] && my - ey <= t.y1[0];
+  const bool r1 = t.yv[1] && my + ey >= t.y0[1] && my - ey <= t.y1[1];
+  return (uint32_t)(r0 && c0) | (uint32_t)(r0 && c1) << 1 | (uint32_t)(r1 && c0) << 2 | (uint32_t)(r1 && c1) << 3;
+}
+
 // One wave's forward over one 8x8 quadrant (0..3, row-major) of one tile. The
 // LDS slices are the wave's own.
 __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDesc* __restrict__ views,
@@ -311,246 +345,230 @@ __device__ __forceinline__ int butterfly_slot(int lane) {
   return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
 }
 
-// The workgroup's shared storage for one batch of kBB splats.
-template <int kBB>
-struct BwdShared {
-  float4 a[kBB];
-  float4 b[kBB];
-  float c[kBB];
-  uint32_t dst[kBB];          // slot (atomic) or pair id (deterministic)
-  uint32_t mask[kBB];         // quadrants that visit the splat
-  float part[4][kBB][kVals];  // per-wave sums (valid where mask says)
-  int stop[4];                // per-quadrant last contributor
-};
-
-// The adjoint of one 16x16 tile by one 256-thread workgroup (four quadrant
-// waves); block-synchronous, so every thread of the workgroup calls it.
-template <int kBB>
-__device__ __forceinline__ void backward_tile(const RasterParams& p, const ViewDesc* __restrict__ views,
-                                              const uint32_t* __restrict__ ranges,
-                                              const uint32_t* __restrict__ pair_vals,
-                                              const SplatRec* __restrict__ rec, const float* __restrict__ dL_drgb,
-                                              const float* __restrict__ T_in, const int* __restrict__ last_in,
-                                              float* __restrict__ screen, const uint32_t* __restrict__ pair_base,
-                                              float* __restrict__ partial, uint32_t p_cap, int deterministic,
-                                              int tile, BwdShared<kBB>& sh) {
-  constexpr int kBatchB = kBB;
-  float4* s_a = sh.a;
-  float4* s_b = sh.b;
-  float* s_c = sh.c;
-  uint32_t* s_dst = sh.dst;
-  uint32_t* s_mask = sh.mask;
-  auto& s_part = sh.part;
-  int* s_stop = sh.stop;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// The adjoint of one 16x16 tile by one wave: each lane owns four pixels, one
+// per 8x8 quadrant (lane & 7, lane >> 3 within each), so a splat's terms over
+// the whole tile are reduced by a single wave butterfly and the four
+// quadrants never meet in LDS or at a barrier. The tile's list is walked back
+// to front from the tile's last contributor in chunks of kBB entries: the
+// lanes stage the chunk's records in LDS with a 4-bit mask of the quadrants
+// whose cull box they meet below that quadrant's last contributor; then, per
+// marked splat from the back, each marked quadrant's pixels are evaluated
+// (a wave-uniform branch per quadrant), the lane adds its pixels' terms, the
+// wave sums them, and after the chunk each lane issues its splat's result: 11
+// hardware float atomics (atomic mode) or one 48-byte partial store
+// (deterministic mode, summation order fixed by the code alone).
+template <int kBB, bool kQueue>
+__global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
+                                                 const uint32_t* __restrict__ ranges,
+                                                 const uint32_t* __restrict__ pair_vals,
+                                                 const SplatRec* __restrict__ rec, const float* __restrict__ dL_drgb,
+                                                 const float* __restrict__ T_in, const int* __restrict__ last_in,
+                                                 float* __restrict__ screen, const uint32_t* __restrict__ pair_base,
+                                                 float* __restrict__ partial, uint32_t p_cap, int deterministic,
+                                                 const uint32_t* __restrict__ order, uint32_t* __restrict__ counter) {
+  static_assert(kBB == 64 || kBB == 128, "chunks of one or two entries per lane");
+  constexpr int kHalves = kBB / 64;
+  __shared__ float4 s_a[kBB];  // mx, my, ca, cb
+  __shared__ float4 s_b[kBB];  // cc, opacity, r, g
+  __shared__ float s_c[kBB];   // b
+  __shared__ uint32_t s_mask[kBB];
+  __shared__ uint32_t s_dst[kBB];  // slot (atomic) or pair id (deterministic)
+  __shared__ float s_part[kBB][kVals];
+  __shared__ int s_unit;
+  const int lane = threadIdx.x;
   const int my_slot = butterfly_slot(lane);
   const bool writer = lane < 16 && my_slot < kVals;
-  const int v = find_view(views, p.n_views, tile);
-  const ViewDesc& V = views[v];
-  const int local = tile - V.tile_base;
-  const int tx = local % V.cols, ty = local / V.cols;
-  // the four quadrants' cull bounds, for the staging threads
-  Quad quads[4];
-  bool qvalid[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) qvalid[k] = quad_of(V, tx, ty, k, &quads[k]);
-  const int px = tx * kTile + 8 * (wave & 1) + (lane & 7), py = ty * kTile + 8 * (wave >> 1) + (lane >> 3);
-  const bool inside = px < V.width && py < V.height;
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
-  float dL0 = 0.f, dL1 = 0.f, dL2 = 0.f, T_final = 1.f;
-  int lastp = 0;
-  if (inside) {
-    const long long pixel = V.pixel_base + (long long)py * V.width + px;
-    dL0 = dL_drgb[3 * pixel];
-    dL1 = dL_drgb[3 * pixel + 1];
-    dL2 = dL_drgb[3 * pixel + 2];
-    T_final = T_in[pixel];
-    lastp = last_in[pixel];
-  }
-  __syncthreads();  // the previous tile's last reads of the shared storage are done
-  if (tid < 4) s_stop[tid] = 0;
-  __syncthreads();
-  if (lastp > 0) atomicMax(&s_stop[wave], lastp);
-  __syncthreads();
-  const int stops[4] = {s_stop[0], s_stop[1], s_stop[2], s_stop[3]};
-  const uint32_t stop = begin + (uint32_t)max(max(stops[0], stops[1]), max(stops[2], stops[3]));
-  // Pairs past every pixel's last contributor get zero partials.
-  if (deterministic) {
-    for (uint32_t e = stop + tid; e < end; e += blockDim.x) {
-      const uint32_t slot = pair_vals[e];
-      const SplatRec& r = rec[slot];
-      const uint32_t pid = pair_base[slot] + (uint32_t)((ty - r.y0) * (r.x1 - r.x0 + 1) + (tx - r.x0));
-      if (pid >= p_cap) continue;  // only when the step's capacity check has flagged it
-      float4* dst = reinterpret_cast<float4*>(partial + (long long)pid * kScreenStride);
-      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-      dst[0] = zero;
-      dst[1] = zero;
-      dst[2] = zero;
+  for (int iter = 0;; ++iter) {
+    int tile;
+    if (kQueue) {
+      if (lane == 0) s_unit = (int)atomicAdd(counter, 1u);
+      wave_sync_lds();
+      const int idx = __builtin_amdgcn_readfirstlane(s_unit);
+      wave_sync_lds();
+      if (idx >= p.total_tiles) break;
+      tile = (int)order[idx];
+    } else {
+      if (iter > 0) break;
+      tile = xcd_remap(blockIdx.x, p.total_tiles);
     }
-  }
-  const float bg_term = p.bg[0] * dL0 + p.bg[1] * dL1 + p.bg[2] * dL2;
-  float T = T_final;
-  float suf0 = 0.f, suf1 = 0.f, suf2 = 0.f;
-  float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
-  uint32_t hi = stop;
-  while (hi > begin) {
-    const uint32_t lo = (hi - begin) > (uint32_t)kBatchB ? hi - kBatchB : begin;
-    const int cnt = (int)(hi - lo);
-    __syncthreads();
-    if (tid < cnt) {
-      const uint32_t e = lo + tid;
-      const uint32_t slot = pair_vals[e];
-      const SplatRec* r = rec + slot;
-      const float4* r4 = reinterpret_cast<const float4*>(r);
-      const float4 a = r4[0], b = r4[1], c = r4[2];
-      s_a[tid] = a;
-      s_b[tid] = b;
-      s_c[tid] = c.x;
-      const int idx = (int)(e - begin);
-      uint32_t m = 0;
+    const ViewDesc& V = views[find_view(views, p.n_views, tile)];
+    const int local = tile - V.tile_base;
+    const int tx = local % V.cols, ty = local / V.cols;
+    const TileQuads tq = tile_quads(V, tx, ty);
+    const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
+    // the lane's four pixels: state of the engine's backward walk
+    // (SplatRaster.cpp:346-433), with the suffix colour kept folded (the
+    // value the next contributor's step reads, updated as soon as this one is
+    // done) so the previous contributor's alpha and colour need no registers
+    // pixel centres: quadrant 0's, +8 (exact) for the others
+    const float fx0 = (float)(tx * kTile + (lane & 7)) + 0.5f, fy0 = (float)(ty * kTile + (lane >> 3)) + 0.5f;
+    float dL0[4], dL1[4], dL2[4], tb[4], T[4], suf0[4], suf1[4], suf2[4];
+    int lastp[4], stops[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (qvalid[k] && idx < stops[k] && box_hits(a.x, a.y, c.y, c.z, quads[k])) m |= 1u << k;
-      s_mask[tid] = m;
-      if (deterministic)
-        s_dst[tid] = pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0));
-      else
-        s_dst[tid] = slot;
+    for (int k = 0; k < 4; ++k) {
+      const int px = tx * kTile + 8 * (k & 1) + (lane & 7), py = ty * kTile + 8 * (k >> 1) + (lane >> 3);
+      dL0[k] = dL1[k] = dL2[k] = 0.f;
+      float T_final = 1.f;
+      lastp[k] = 0;
+      if (px < V.width && py < V.height) {
+        const long long pixel = V.pixel_base + (long long)py * V.width + px;
+        dL0[k] = dL_drgb[3 * pixel];
+        dL1[k] = dL_drgb[3 * pixel + 1];
+        dL2[k] = dL_drgb[3 * pixel + 2];
+        T_final = T_in[pixel];
+        lastp[k] = last_in[pixel];
+      }
+      tb[k] = T_final * (p.bg[0] * dL0[k] + p.bg[1] * dL1[k] + p.bg[2] * dL2[k]);
+      T[k] = T_final;
+      suf0[k] = suf1[k] = suf2[k] = 0.f;
+      int s = lastp[k];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) s = max(s, __shfl_xor(s, off, 64));
+      stops[k] = __builtin_amdgcn_readfirstlane(s);
     }
-    __syncthreads();
-    // this wave's splats of the batch, visited from the back
-    constexpr int kHalves = kBatchB / 64;
-    uint64_t mine[kHalves];
+    const uint32_t stop = begin + (uint32_t)max(max(stops[0], stops[1]), max(stops[2], stops[3]));
+    // Pairs past every pixel's last contributor get zero partials.
+    if (deterministic) {
+      for (uint32_t e = stop + lane; e < end; e += 64) {
+        const uint32_t slot = pair_vals[e];
+        const SplatRec& r = rec[slot];
+        const uint32_t pid = pair_base[slot] + (uint32_t)((ty - r.y0) * (r.x1 - r.x0 + 1) + (tx - r.x0));
+        if (pid >= p_cap) continue;  // only when the step's capacity check has flagged it
+        float4* dst = reinterpret_cast<float4*>(partial + (long long)pid * kScreenStride);
+        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[0] = zero;
+        dst[1] = zero;
+        dst[2] = zero;
+      }
+    }
+    for (uint32_t hi = stop; hi > begin;) {
+      const uint32_t lo = hi - min((uint32_t)kBB, hi - begin);
+      const int cnt = (int)(hi - lo);
+      // stage the chunk
 #pragma unroll
-    for (int h = 0; h < kHalves; ++h)
-      mine[h] = __ballot(lane + 64 * h < cnt && ((s_mask[lane + 64 * h] >> wave) & 1u));
+      for (int h = 0; h < kHalves; ++h) {
+        const int j = lane + 64 * h;
+        if (j < cnt) {
+          const uint32_t e = lo + j;
+          const uint32_t slot = pair_vals[e];
+          const SplatRec* r = rec + slot;
+          const float4* r4 = reinterpret_cast<const float4*>(r);
+          const float4 a = r4[0], b = r4[1], c = r4[2];
+          s_a[j] = a;
+          s_b[j] = b;
+          s_c[j] = c.x;
+          const int idx = (int)(e - begin);
+          uint32_t m = quad_hits(a.x, a.y, c.y, c.z, tq);
 #pragma unroll
-    for (int h = kHalves - 1; h >= 0; --h) {
-      uint64_t bits = mine[h];
-      while (bits) {
-        const int bit = 63 - __builtin_clzll(bits);
-        bits &= ~(1ull << bit);
-        const int j = 64 * h + bit;
-        const uint32_t e = lo + j;
-        const float4 a = s_a[j];
-        const float4 b = s_b[j];
-        // the forward's decisions, recomputed with its exact float operations
-        const float dx = fx - a.x, dy = fy - a.y;
-        const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-        const float G = det_exp_nonpos(power < 0.f ? power : 0.f);
-        const float raw = b.y * G;
-        const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-        const bool contrib =
-            inside && (int)(e - begin) < lastp && !(power > 0.f) && !(alpha < p.min_alpha);
-        float z = 0.f;
-        if (__ballot(contrib) != 0ull) {
-          // every lane computes, contributing lanes keep: the same expressions
-          // as a branch per lane, with exact zeros and unchanged state elsewhere
-          float val[16];
+          for (int k = 0; k < 4; ++k)
+            if (idx >= stops[k]) m &= ~(1u << k);
+          s_mask[j] = m;
+          s_dst[j] = deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0))
+                                   : slot;
+        }
+      }
+      wave_sync_lds();
 #pragma unroll
-          for (int k = 11; k < 16; ++k) val[k] = 0.f;
-          {
+      for (int h = kHalves - 1; h >= 0; --h) {
+        uint64_t bits = __ballot(lane + 64 * h < cnt && s_mask[lane + 64 * h] != 0u);
+        while (bits) {
+          const int bit = 63 - __builtin_clzll(bits);
+          bits &= ~(1ull << bit);
+          const int j = 64 * h + bit;
+          const int idx = (int)(lo + j - begin);
+          const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[j]);
+          const float4 a = s_a[j];
+          const float4 b = s_b[j];
+          const float cblue = s_c[j];
+          float acc[kVals];
+#pragma unroll
+          for (int i = 0; i < kVals; ++i) acc[i] = 0.f;
+          bool any = false;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (!((m >> k) & 1u)) continue;  // wave-uniform
+            // the forward's decisions, recomputed with its exact float operations
+            const float dx = (k & 1 ? fx0 + 8.f : fx0) - a.x, dy = (k >> 1 ? fy0 + 8.f : fy0) - a.y;
+            const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+            const float G = det_exp_nonpos(power < 0.f ? power : 0.f);
+            const float raw = b.y * G;
+            const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
+            const bool contrib = idx < lastp[k] && !(power > 0.f) && !(alpha < p.min_alpha);
+            if (__ballot(contrib) == 0ull) continue;
+            any = true;
+            {
+            // every lane computes, contributing lanes keep: the engine's
+            // expressions with exact zeros and unchanged state elsewhere
 #pragma clang fp contract(fast)
-            const float c2 = s_c[j];
-            const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-            const float Tn = T * inv;
-            const float weight = alpha * Tn;
-            const float sn0 = last_alpha * lc0 + (1.f - last_alpha) * suf0;
-            const float sn1 = last_alpha * lc1 + (1.f - last_alpha) * suf1;
-            const float sn2 = last_alpha * lc2 + (1.f - last_alpha) * suf2;
-            float dL_dalpha = ((b.z - sn0) * dL0 + (b.w - sn1) * dL1 + (c2 - sn2) * dL2) * Tn;
-            dL_dalpha -= T_final * inv * bg_term;
             const bool shape = contrib && raw < kMaxAlpha;
-            const float dL_dweight = b.y * dL_dalpha;
+            const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+            const float Tn = T[k] * inv;
+            const float weight = contrib ? alpha * Tn : 0.f;
+            const float dL_dalpha = ((b.z - suf0[k]) * dL0[k] + (b.w - suf1[k]) * dL1[k] + (cblue - suf2[k]) * dL2[k]) *
+                                        Tn - tb[k] * inv;
+            const float dL_dweight = shape ? b.y * dL_dalpha : 0.f;
+            const float Gs = shape ? G : 0.f;
             const float gx = G * dx, gy = G * dy;
             const float mx = dL_dweight * (-gx * a.z - gy * a.w);
             const float my = dL_dweight * (-gy * b.x - gx * a.w);
-            val[0] = contrib ? weight * dL0 : 0.f;
-            val[1] = contrib ? weight * dL1 : 0.f;
-            val[2] = contrib ? weight * dL2 : 0.f;
-            val[3] = shape ? G * dL_dalpha : 0.f;
-            val[4] = shape ? -mx : 0.f;
-            val[5] = shape ? -my : 0.f;
-            val[6] = shape ? __builtin_amdgcn_sqrtf(mx * mx + my * my) : 0.f;  // densify statistic: hardware sqrt
-            val[7] = shape ? -0.5f * dL_dweight * gx * dx : 0.f;
-            val[8] = shape ? -dL_dweight * gx * dy : 0.f;
-            val[9] = shape ? -0.5f * dL_dweight * gy * dy : 0.f;
-            val[10] = shape ? 1.f : 0.f;
-            T = contrib ? Tn : T;
-            suf0 = contrib ? sn0 : suf0;
-            suf1 = contrib ? sn1 : suf1;
-            suf2 = contrib ? sn2 : suf2;
-            lc0 = contrib ? b.z : lc0;
-            lc1 = contrib ? b.w : lc1;
-            lc2 = contrib ? c2 : lc2;
-            last_alpha = contrib ? alpha : last_alpha;
+            acc[0] += weight * dL0[k];
+            acc[1] += weight * dL1[k];
+            acc[2] += weight * dL2[k];
+            acc[3] += Gs * dL_dalpha;
+            acc[4] -= mx;
+            acc[5] -= my;
+            acc[6] += __builtin_amdgcn_sqrtf(mx * mx + my * my);  // densify statistic: hardware sqrt
+            acc[7] -= 0.5f * dL_dweight * gx * dx;
+            acc[8] -= dL_dweight * gx * dy;
+            acc[9] -= 0.5f * dL_dweight * gy * dy;
+            acc[10] += shape ? 1.f : 0.f;
+            T[k] = contrib ? Tn : T[k];
+            const float keep = 1.f - alpha;
+            suf0[k] = contrib ? alpha * b.z + keep * suf0[k] : suf0[k];
+            suf1[k] = contrib ? alpha * b.w + keep * suf1[k] : suf1[k];
+            suf2[k] = contrib ? alpha * cblue + keep * suf2[k] : suf2[k];
+            }
           }
-          z = butterfly16(val, lane);
+          float z = 0.f;
+          if (any) {
+            float v16[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v16[i] = i < kVals ? acc[i] : 0.f;
+            z = butterfly16(v16, lane);
+          }
+          if (writer) s_part[j][my_slot] = z;
         }
-        if (writer) s_part[wave][j][my_slot] = z;
       }
-    }
-    __syncthreads();
-    if (tid < cnt) {
-      const uint32_t m = s_mask[tid];
-      float sum[kVals];
+      wave_sync_lds();
+      // each lane issues its staged splats' results
 #pragma unroll
-      for (int k = 0; k < kVals; ++k) sum[k] = 0.f;
-      // fixed order over the quadrants that visited the splat
+      for (int h = 0; h < kHalves; ++h) {
+        const int j = lane + 64 * h;
+        if (j < cnt) {
+          const uint32_t m = s_mask[j];
+          const uint32_t dsti = s_dst[j];
+          float sum[12];
 #pragma unroll
-      for (int w = 0; w < 4; ++w)
-        if ((m >> w) & 1u) {
+          for (int k = 0; k < kVals; ++k) sum[k] = m ? s_part[j][k] : 0.f;
+          sum[11] = 0.f;
+          if (deterministic) {
+            if (dsti < p_cap) {  // 48-byte records: three 16-byte stores
+              float4* dst = reinterpret_cast<float4*>(partial + (long long)dsti * kScreenStride);
+              dst[0] = make_float4(sum[0], sum[1], sum[2], sum[3]);
+              dst[1] = make_float4(sum[4], sum[5], sum[6], sum[7]);
+              dst[2] = make_float4(sum[8], sum[9], sum[10], 0.f);
+            }
+          } else if (m) {
+            float* dst = screen + (long long)dsti * kScreenStride;
 #pragma unroll
-          for (int k = 0; k < kVals; ++k) sum[k] += s_part[w][tid][k];
+            for (int k = 0; k < kVals; ++k)
+              if (sum[k] != 0.f) unsafeAtomicAdd(dst + k, sum[k]);
+          }
         }
-      if (deterministic) {
-        if (s_dst[tid] < p_cap) {  // 48-byte records: three 16-byte stores
-          float4* dst = reinterpret_cast<float4*>(partial + (long long)s_dst[tid] * kScreenStride);
-          dst[0] = make_float4(sum[0], sum[1], sum[2], sum[3]);
-          dst[1] = make_float4(sum[4], sum[5], sum[6], sum[7]);
-          dst[2] = make_float4(sum[8], sum[9], sum[10], 0.f);
-        }
-      } else if (m) {
-        float* dst = screen + (long long)s_dst[tid] * kScreenStride;
-#pragma unroll
-        for (int k = 0; k < kVals; ++k)
-          if (sum[k] != 0.f) unsafeAtomicAdd(dst + k, sum[k]);
       }
+      wave_sync_lds();  // the next chunk overwrites the staging
+      hi = lo;
     }
-    hi = lo;
-  }
-}
-
-
-// kQueue false: one workgroup per tile, tiles spread over the XCDs in
-// contiguous runs. kQueue true: a persistent grid taking tiles, longest list
-// first, from a counter over `order`.
-template <int kBB, bool kQueue>
-__global__ __launch_bounds__(256) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
-                                                  const uint32_t* __restrict__ ranges,
-                                                  const uint32_t* __restrict__ pair_vals,
-                                                  const SplatRec* __restrict__ rec,
-                                                  const float* __restrict__ dL_drgb, const float* __restrict__ T_in,
-                                                  const int* __restrict__ last_in, float* __restrict__ screen,
-                                                  const uint32_t* __restrict__ pair_base, float* __restrict__ partial,
-                                                  uint32_t p_cap, int deterministic, const uint32_t* __restrict__ order,
-                                                  uint32_t* __restrict__ counter) {
-  __shared__ BwdShared<kBB> sh;
-  __shared__ int s_unit;
-  if (!kQueue) {
-    backward_tile<kBB>(p, views, ranges, pair_vals, rec, dL_drgb, T_in, last_in, screen, pair_base, partial, p_cap,
-                       deterministic, xcd_remap(blockIdx.x, p.total_tiles), sh);
-    return;
-  }
-  for (;;) {
-    if (threadIdx.x == 0) s_unit = (int)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int idx = s_unit;
-    __syncthreads();
-    if (idx >= p.total_tiles) break;
-    backward_tile<kBB>(p, views, ranges, pair_vals, rec, dL_drgb, T_in, last_in, screen, pair_base, partial, p_cap,
-                       deterministic, (int)order[idx], sh);
   }
 }
 
@@ -600,9 +618,9 @@ hipError_t backward_variant(const RasterParams& p, const ViewDesc* views, const 
                             const int* last, float* screen, const uint32_t* pair_base, float* partial,
                             uint32_t p_cap, bool deterministic, const uint32_t* order, uint32_t* counter,
                             int cus, hipStream_t stream) {
-  static const int per_cu = resident_blocks(k_backward<kBB, kQueue>, 256);
+  static const int per_cu = resident_blocks(k_backward<kBB, kQueue>, 64);
   const int blocks = kQueue ? std::min(p.total_tiles, per_cu * cus) : p.total_tiles;
-  hipLaunchKernelGGL((k_backward<kBB, kQueue>), dim3(blocks), dim3(256), 0, stream, p, views, ranges, pair_vals, rec,
+  hipLaunchKernelGGL((k_backward<kBB, kQueue>), dim3(blocks), dim3(64), 0, stream, p, views, ranges, pair_vals, rec,
                      dL_drgb, T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0, order, counter);
   return hipGetLastError();
 }
