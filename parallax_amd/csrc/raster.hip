@@ -373,7 +373,6 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   __shared__ float s_c[kBB];   // b
   __shared__ uint32_t s_mask[kBB];
   __shared__ uint32_t s_dst[kBB];  // slot (atomic) or pair id (deterministic)
-  __shared__ float s_part[kBB][kVals];
   __shared__ int s_unit;
   const int lane = threadIdx.x;
   const int my_slot = butterfly_slot(lane);
@@ -463,8 +462,16 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           for (int k = 0; k < 4; ++k)
             if (idx >= stops[k]) m &= ~(1u << k);
           s_mask[j] = m;
-          s_dst[j] = deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0))
-                                   : slot;
+          const uint32_t dst =
+              deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
+          s_dst[j] = dst;
+          if (deterministic && m == 0u && dst < p_cap) {  // no quadrant visits it: a zero record
+            float4* out = reinterpret_cast<float4*>(partial + (long long)dst * kScreenStride);
+            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+            out[0] = zero;
+            out[1] = zero;
+            out[2] = zero;
+          }
         }
       }
       wave_sync_lds();
@@ -536,33 +543,15 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             for (int i = 0; i < 16; ++i) v16[i] = i < kVals ? acc[i] : 0.f;
             z = butterfly16(v16, lane);
           }
-          if (writer) s_part[j][my_slot] = z;
-        }
-      }
-      wave_sync_lds();
-      // each lane issues its staged splats' results
-#pragma unroll
-      for (int h = 0; h < kHalves; ++h) {
-        const int j = lane + 64 * h;
-        if (j < cnt) {
-          const uint32_t m = s_mask[j];
+          // lanes 0-15 hold the splat's 16 wave totals (slots 11-15 are
+          // zero): one instruction covers the splat's contiguous 48-byte
+          // record -- atomics (nonzero terms only) or the partial's stores
           const uint32_t dsti = s_dst[j];
-          float sum[12];
-#pragma unroll
-          for (int k = 0; k < kVals; ++k) sum[k] = m ? s_part[j][k] : 0.f;
-          sum[11] = 0.f;
           if (deterministic) {
-            if (dsti < p_cap) {  // 48-byte records: three 16-byte stores
-              float4* dst = reinterpret_cast<float4*>(partial + (long long)dsti * kScreenStride);
-              dst[0] = make_float4(sum[0], sum[1], sum[2], sum[3]);
-              dst[1] = make_float4(sum[4], sum[5], sum[6], sum[7]);
-              dst[2] = make_float4(sum[8], sum[9], sum[10], 0.f);
-            }
-          } else if (m) {
-            float* dst = screen + (long long)dsti * kScreenStride;
-#pragma unroll
-            for (int k = 0; k < kVals; ++k)
-              if (sum[k] != 0.f) unsafeAtomicAdd(dst + k, sum[k]);
+            if (lane < 16 && my_slot < kScreenStride && dsti < p_cap)
+              partial[(long long)dsti * kScreenStride + my_slot] = z;
+          } else if (any && writer && z != 0.f) {
+            unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
           }
         }
       }
