@@ -302,47 +302,51 @@ __device__ __forceinline__ float xchg(float v) {
   else return dpp<0x128>(v);
 }
 
-// Sum 16 per-lane values over the wave64 in one transposed butterfly: each
-// level exchanges half of the values a lane holds with its partner and keeps
-// the other half, so 16 values -> 1 in four DPP levels (lane bits 0-3), then
-// two cross-row permlane-swap steps. Every lane ends with the wave total of one value: slot
-// bitrev4(lane & 15). The summation order is fixed, so it is deterministic.
-__device__ __forceinline__ float butterfly16(const float (&v)[16], int lane) {
-  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
-  float w[8], x[4], y[2];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const float send = b0 ? v[k] : v[k + 8];
-    const float keep = b0 ? v[k + 8] : v[k];
-    w[k] = keep + xchg<0>(send);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float send = b1 ? w[k] : w[k + 4];
-    const float keep = b1 ? w[k + 4] : w[k];
-    x[k] = keep + xchg<1>(send);
-  }
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const float send = b2 ? x[k] : x[k + 2];
-    const float keep = b2 ? x[k + 2] : x[k];
-    y[k] = keep + xchg<2>(send);
-  }
-  const float send = b3 ? y[0] : y[1];
-  const float keep = b3 ? y[1] : y[0];
-  float z = keep + xchg<3>(send);
-  // across rows: v_permlane16_swap / v_permlane32_swap with both operands z
-  // leave z[l] and z[l ^ 16] (then z[l ^ 32]) in the two results -- a VALU
-  // exchange, no LDS round trip
-  const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(z), __float_as_uint(z), false, false);
-  z = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);
-  const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(z), __float_as_uint(z), false, false);
-  z = __uint_as_float(r32[0]) + __uint_as_float(r32[1]);
-  return z;
+// One transposed-butterfly step across lane bit 5 (or 4) with no lane
+// selects: v_permlane32_swap (v_permlane16_swap) exchanges the upper half
+// (odd rows) of x with the lower half (even rows) of y, so x' + y' holds,
+// in the lower lanes, the pair sum of x, and in the upper lanes the pair sum
+// of y.
+__device__ __forceinline__ float swap32_sum(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap16_sum(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
-__device__ __forceinline__ int butterfly_slot(int lane) {
-  return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+// Sum 11 per-lane values over the wave64 in one transposed butterfly: each
+// step exchanges half of the values a lane holds with its partner and keeps
+// the other half. Lane bits 5 and 4 go first, through permlane swaps (no
+// selects); bits 0 and 1 through DPP with selects; bits 2 and 3 then sum the
+// one remaining value. Lane l ends with the wave total of value
+// wave_sum_slot(l) (slots 11-15 hold no value). The summation order is fixed,
+// so it is deterministic.
+__device__ __forceinline__ float wave_sum11(const float (&v)[kVals], int lane) {
+  float a[8], c[4], d[2];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = k + 8 < kVals ? swap32_sum(v[k], v[k + 8]) : swap32_sum(v[k], v[k]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = swap16_sum(a[k], a[k + 4]);
+  const bool b0 = lane & 1, b1 = lane & 2;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const float send = b0 ? c[k] : c[k + 2];
+    const float keep = b0 ? c[k + 2] : c[k];
+    d[k] = keep + xchg<0>(send);
+  }
+  const float send = b1 ? d[0] : d[1];
+  const float keep = b1 ? d[1] : d[0];
+  float e = keep + xchg<1>(send);
+  e = e + xchg<2>(e);
+  e = e + xchg<3>(e);
+  return e;
+}
+
+// The value whose total wave_sum11 leaves in lane `lane`.
+__device__ __forceinline__ int wave_sum_slot(int lane) {
+  return ((lane >> 1) & 1) | ((lane & 1) << 1) | (((lane >> 4) & 1) << 2) | (((lane >> 5) & 1) << 3);
 }
 
 // The adjoint of one 16x16 tile by one wave: each lane owns four pixels, one
@@ -375,8 +379,9 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   __shared__ uint32_t s_dst[kBB];  // slot (atomic) or pair id (deterministic)
   __shared__ int s_unit;
   const int lane = threadIdx.x;
-  const int my_slot = butterfly_slot(lane);
-  const bool writer = lane < 16 && my_slot < kVals;
+  const int my_slot = wave_sum_slot(lane);
+  const bool lead = (lane & 12) == 0;  // one lane per slot after wave_sum11
+  const bool writer = lead && my_slot < kVals;
   for (int iter = 0;; ++iter) {
     int tile;
     if (kQueue) {
@@ -537,19 +542,14 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             }
           }
           float z = 0.f;
-          if (any) {
-            float v16[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) v16[i] = i < kVals ? acc[i] : 0.f;
-            z = butterfly16(v16, lane);
-          }
-          // lanes 0-15 hold the splat's 16 wave totals (slots 11-15 are
-          // zero): one instruction covers the splat's contiguous 48-byte
-          // record -- atomics (nonzero terms only) or the partial's stores
+          if (any) z = wave_sum11(acc, lane);
+          // the 16 lead lanes hold the splat's wave totals: one instruction
+          // covers the splat's contiguous 48-byte record -- atomics (nonzero
+          // terms only) or the partial's stores (slot 11 is padding, zero)
           const uint32_t dsti = s_dst[j];
           if (deterministic) {
-            if (lane < 16 && my_slot < kScreenStride && dsti < p_cap)
-              partial[(long long)dsti * kScreenStride + my_slot] = z;
+            if (lead && my_slot < kScreenStride && dsti < p_cap)
+              partial[(long long)dsti * kScreenStride + my_slot] = my_slot < kVals ? z : 0.f;
           } else if (any && writer && z != 0.f) {
             unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
           }
