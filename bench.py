@@ -189,6 +189,16 @@ def main():
                 "ms_per_launch": round(bwd_ms if bwd_ms >= fwd_ms else fwd_ms, 4),
                 "dominant_by_time": dominant}
     roofline["frac"] = round(roofline["achieved"] / PEAK_FP32_TFLOPS, 4)
+    # HBM bytes per launch of the variant the autotuner kept, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    rc = cfg_batched
+    variant = (f"k_backward<{rc['backward_batch']},{str(rc['backward_queue']).lower()}>"
+               if roofline["kernel"] == "backward" else
+               f"k_forward<{['16x16', '8x8', '32x8'].index(rc['forward_shape'])},{str(rc['forward_queue']).lower()}>")
+    t = profile_traffic(variant)
+    if t is not None:
+        roofline["traffic"] = t
+        roofline["traffic_source"] = f"{PROFILE_FILE} ({variant}, FETCH_SIZE + WRITE_SIZE)"
     # HBM-bound kernels beside it
     rs = 45
     adam_bytes = 32.0 * n * (14 + rs)
@@ -241,6 +251,20 @@ def main():
 
 
 CALIB_BYTES = 256 << 20
+PROFILE_FILE = "profiles/train_step_1M_sh3_1080p_batch8.txt"
+
+
+def profile_traffic(kernel):
+    """HBM bytes per dispatch (calibrated fetch + write) of `kernel` in the
+    committed rocprof summary of this workload, or None."""
+    try:
+        for line in open(os.path.join(ROOT, PROFILE_FILE)):
+            f = line.split()
+            if len(f) == 6 and f[0] == kernel:
+                return int((float(f[4]) + float(f[5])) * 1e6)
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def calibration_copy():

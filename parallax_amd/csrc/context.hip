@@ -413,7 +413,7 @@ splat_status tune_forward(splat_context* ctx) {
       c.fwd_shape = shape;
       c.fwd_queue = queue != 0;
       float ms = 0.f;
-      if (splat_status s = time_launches(ctx, 2, [&] {
+      if (splat_status s = time_launches(ctx, 4, [&] {
             return launch_forward(ctx->raster, c, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals,
                                   ctx->recs(), ctx->rgb.f(), ctx->T.f(), ctx->last.i(), ctx->tile_order,
                                   ctx->scalars.u() + 4, ctx->stream);
@@ -552,7 +552,7 @@ splat_status enqueue_backward(splat_context* ctx) {
         c.bwd_batch = batch;
         c.bwd_queue = queue != 0;
         float ms = 0.f;
-        if (splat_status s = time_launches(ctx, 2, [&] { return launch(c); }, &ms)) return s;
+        if (splat_status s = time_launches(ctx, 4, [&] { return launch(c); }, &ms)) return s;
         if (ms < best_ms) {
           best_ms = ms;
           best = c;
