@@ -83,6 +83,37 @@ __device__ __forceinline__ bool box_hits(float mx, float my, float ex, float ey,
   return mx + ex >= q.bx0 && mx - ex <= q.bx1 && my + ey >= q.by0 && my - ey <= q.by1;
 }
 
+// Whether any point of the rectangle [x0, x1] x [y0, y1] (pixel centres of a
+// quadrant) can lie in the splat's alpha >= min_alpha ellipse
+// Q(d) = a dx^2 + 2 b dx dy + c dy^2 <= q, q = 2 ln(opacity / min_alpha).
+// The minimum of the positive-definite Q over the rectangle is 0 when the mean
+// is inside, else on an edge, where each edge's minimiser is the clamped
+// stationary point. Conservative: the continuous minimum is at most the value
+// at any pixel centre, q is raised by 1% + 1e-3 (the hardware log, the float
+// minimum and the compositing loop's own float Q all err far less for
+// kappa = ac / det <= 1000), and a more elongated splat is always kept (the
+// cull box alone decides). A splat it rejects is skipped by every pixel of the
+// quadrant anyway, so results do not change.
+__device__ __forceinline__ bool ellipse_meets_rect(float a, float b, float c, float opacity, float min_alpha,
+                                                   float x0, float x1, float y0, float y1) {
+  const float det = a * c - b * b;
+  if (!(det > 0.f) || a * c > 1000.f * det) return true;
+  const float q = 2.f * 0.69314718f * (__builtin_amdgcn_logf(opacity) - __builtin_amdgcn_logf(min_alpha));
+  const float q_hi = q * 1.01f + 1e-3f;
+  if (x0 <= 0.f && x1 >= 0.f && y0 <= 0.f && y1 >= 0.f) return q_hi >= 0.f;
+  const float ia = __builtin_amdgcn_rcpf(a), ic = __builtin_amdgcn_rcpf(c);
+  float best = __builtin_inff();
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float xe = e ? x1 : x0, ye = e ? y1 : y0;
+    const float yc = fminf(fmaxf(-b * xe * ic, y0), y1);
+    const float xc = fminf(fmaxf(-b * ye * ia, x0), x1);
+    best = fminf(best, a * xe * xe + 2.f * b * xe * yc + c * yc * yc);
+    best = fminf(best, a * xc * xc + 2.f * b * xc * ye + c * ye * ye);
+  }
+  return best <= q_hi;
+}
+
 // The pixel-centre bounds of a tile's two quadrant columns and two quadrant
 // rows, as wave-uniform values in scalar registers (the float conversions run
 // on the vector unit; readfirstlane moves the results back to SGPRs so they
@@ -149,7 +180,8 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
       a = r[0];
       b = r[1];
       c = r[2];
-      hit = box_hits(a.x, a.y, c.y, c.z, q);
+      hit = box_hits(a.x, a.y, c.y, c.z, q) &&
+            ellipse_meets_rect(a.z, a.w, b.x, b.y, p.min_alpha, q.bx0 - a.x, q.bx1 - a.x, q.by0 - a.y, q.by1 - a.y);
     }
     const uint64_t m = __ballot(hit);
     if (hit) {
@@ -465,7 +497,10 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           uint32_t m = quad_hits(a.x, a.y, c.y, c.z, tq);
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if (idx >= stops[k]) m &= ~(1u << k);
+            if (idx >= stops[k] ||
+                ((m >> k) & 1u && !ellipse_meets_rect(a.z, a.w, b.x, b.y, p.min_alpha, tq.x0[k & 1] - a.x,
+                                                      tq.x1[k & 1] - a.x, tq.y0[k >> 1] - a.y, tq.y1[k >> 1] - a.y)))
+              m &= ~(1u << k);
           s_mask[j] = m;
           const uint32_t dst =
               deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
