@@ -407,8 +407,6 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   __shared__ float4 s_a[kBB];  // mx, my, ca, cb
   __shared__ float4 s_b[kBB];  // cc, opacity, r, g
   __shared__ float s_c[kBB];   // b
-  __shared__ uint32_t s_mask[kBB];
-  __shared__ uint32_t s_dst[kBB];  // slot (atomic) or pair id (deterministic)
   __shared__ int s_unit;
   const int lane = threadIdx.x;
   const int my_slot = wave_sum_slot(lane);
@@ -480,10 +478,15 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
     for (uint32_t hi = stop; hi > begin;) {
       const uint32_t lo = hi - min((uint32_t)kBB, hi - begin);
       const int cnt = (int)(hi - lo);
-      // stage the chunk
+      // stage the chunk: records to LDS; each entry's quadrant mask goes into
+      // four wave ballots (scalar registers) and its destination stays in the
+      // staging lane's register, so the compositing loop reads neither from LDS
+      uint64_t qbits[kHalves][4];
+      uint32_t dst_reg[kHalves];
 #pragma unroll
       for (int h = 0; h < kHalves; ++h) {
         const int j = lane + 64 * h;
+        uint32_t m = 0u, dst = 0u;
         if (j < cnt) {
           const uint32_t e = lo + j;
           const uint32_t slot = pair_vals[e];
@@ -494,17 +497,14 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           s_b[j] = b;
           s_c[j] = c.x;
           const int idx = (int)(e - begin);
-          uint32_t m = quad_hits(a.x, a.y, c.y, c.z, tq);
+          m = quad_hits(a.x, a.y, c.y, c.z, tq);
 #pragma unroll
           for (int k = 0; k < 4; ++k)
             if (idx >= stops[k] ||
                 ((m >> k) & 1u && !ellipse_meets_rect(a.z, a.w, b.x, b.y, p.min_alpha, tq.x0[k & 1] - a.x,
                                                       tq.x1[k & 1] - a.x, tq.y0[k >> 1] - a.y, tq.y1[k >> 1] - a.y)))
               m &= ~(1u << k);
-          s_mask[j] = m;
-          const uint32_t dst =
-              deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
-          s_dst[j] = dst;
+          dst = deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
           if (deterministic && m == 0u && dst < p_cap) {  // no quadrant visits it: a zero record
             float4* out = reinterpret_cast<float4*>(partial + (long long)dst * kScreenStride);
             const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -513,17 +513,21 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             out[2] = zero;
           }
         }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qbits[h][k] = __ballot((m >> k) & 1u);
+        dst_reg[h] = dst;
       }
       wave_sync_lds();
 #pragma unroll
       for (int h = kHalves - 1; h >= 0; --h) {
-        uint64_t bits = __ballot(lane + 64 * h < cnt && s_mask[lane + 64 * h] != 0u);
+        uint64_t bits = qbits[h][0] | qbits[h][1] | qbits[h][2] | qbits[h][3];
         while (bits) {
           const int bit = 63 - __builtin_clzll(bits);
           bits &= ~(1ull << bit);
           const int j = 64 * h + bit;
           const int idx = (int)(lo + j - begin);
-          const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[j]);
+          const uint32_t m = (uint32_t)((qbits[h][0] >> bit) & 1u) | (uint32_t)((qbits[h][1] >> bit) & 1u) << 1 |
+                             (uint32_t)((qbits[h][2] >> bit) & 1u) << 2 | (uint32_t)((qbits[h][3] >> bit) & 1u) << 3;
           const float4 a = s_a[j];
           const float4 b = s_b[j];
           const float cblue = s_c[j];
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           // the 16 lead lanes hold the splat's wave totals: one instruction
           // covers the splat's contiguous 48-byte record -- atomics (nonzero
           // terms only) or the partial's stores (slot 11 is padding, zero)
-          const uint32_t dsti = s_dst[j];
+          const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
           if (deterministic) {
             if (lead && my_slot < kScreenStride && dsti < p_cap)
               partial[(long long)dsti * kScreenStride + my_slot] = my_slot < kVals ? z : 0.f;
