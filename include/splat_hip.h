@@ -28,6 +28,9 @@
  *     SplatDensify.h:55-57, SplatTrainer.cpp:267-299
  *   resetOpacity  SplatTrainer.cpp:301-309                splat_reset_opacity
  *   accumulated statistics  SplatTrainer.cpp:153-154      splat_download_stats
+ *   initializeFromPointCloud  GaussianCloud.h:90-93        splat_init_from_points
+ *   writeGaussianPly / readGaussianPly  GaussianCloud.h:102-103
+ *                                                        splat_write_ply / splat_read_ply (+ splat_ply_info)
  *
  * Conventions follow the engine: results are returned through out-parameters,
  * every call returns a status and never throws, and on failure
@@ -238,6 +241,29 @@ splat_status splat_upload_moments(splat_context* ctx, const float* m, const floa
 /* Prune (opacity < min_opacity, non-finite or > 1e3 scale) and stable sort by
  * descending opacity logit (GaussianCloud.cpp:177-197), on the device. */
 splat_status splat_prune_and_sort(splat_context* ctx, float min_opacity);
+
+/* ---- seeding and the saved field (the stages either side of training) ---- */
+/* initializeFromPointCloud (GaussianCloud.cpp:199-256) into the context: one
+ * isotropic gaussian per point (xyz: 3n floats; rgb: 3n bytes or NULL) with
+ * log sigma from the sampled nearest-neighbour spacing, identity rotation,
+ * `initial_opacity` as a logit (the engine's default is 0.1) and DC from the
+ * colour. The capacity (splat_set_capacity) must hold n points at sh_degree;
+ * moments and statistics are cleared. log_sigma_out (nullable) receives the
+ * shared log sigma. */
+splat_status splat_init_from_points(splat_context* ctx, int32_t n, const float* xyz, const uint8_t* rgb,
+                                    int32_t sh_degree, float initial_opacity, float* log_sigma_out);
+/* writeGaussianPly (GaussianCloud.cpp:259-303): the context's cloud as a
+ * binary little-endian 3DGS PLY file, byte for byte as the engine writes it. */
+splat_status splat_write_ply(splat_context* ctx, const char* path);
+/* The gaussian count and SH degree of a 3DGS PLY file (to size
+ * splat_set_capacity before splat_read_ply); on failure the engine's message
+ * goes to `error` (nullable). Needs no context. */
+splat_status splat_ply_info(const char* path, int32_t* count, int32_t* sh_degree, char* error, size_t error_len);
+/* readGaussianPly (GaussianCloud.cpp:305-403) into the context: properties
+ * located by name, missing DC / scale read as zero, a missing rotation as the
+ * identity, rotations of norm > 1e-6 renormalised; moments and statistics
+ * cleared. The capacity must hold the count at the file's SH degree. */
+splat_status splat_read_ply(splat_context* ctx, const char* path);
 
 /* ---- measurement ---- */
 /* Per-kernel timing with HIP events on the context's stream: enable, then

@@ -233,6 +233,10 @@ def ref():
                                   C.POINTER(C.c_uint32), C.POINTER(CCloud), i32p, i32p]
         R.ref_train.argtypes = [C.POINTER(CCloud), C.c_int, C.POINTER(CView), C.POINTER(f32p),
                                 C.POINTER(CConfig), C.c_int, f32p, C.POINTER(CCloud), C.c_int]
+        R.ref_init_from_points.argtypes = [C.c_int, f32p, C.POINTER(C.c_ubyte), C.c_int, C.c_float,
+                                           C.POINTER(CCloud)]
+        R.ref_write_ply.argtypes = [C.POINTER(CCloud), C.c_char_p]
+        R.ref_read_ply.argtypes = [C.c_char_p, i32p, i32p, C.POINTER(CCloud), C.c_char_p, C.c_int]
         R.ref_time_steps.restype = C.c_double
         R.ref_time_steps.argtypes = [C.POINTER(CCloud), C.c_int, C.POINTER(CView), C.POINTER(f32p),
                                      C.POINTER(CConfig), C.c_int, f32p]

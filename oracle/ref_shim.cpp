@@ -8,6 +8,7 @@
 // calls the engine's functions unchanged. It is used to write the golden
 // vectors under tests/golden/ and as the reference CPU trainer in the
 // benchmark's CPU-baseline leg.
+#include "parallax/splat/GaussianCloud.h"
 #include "parallax/splat/SplatDensify.h"
 #include "parallax/splat/SplatRaster.h"
 #include "parallax/splat/SplatTrainer.h"
@@ -293,6 +294,48 @@ double ref_time_steps(const ref_cloud* initial, int n_views, const ref_view* vie
   const auto t1 = std::chrono::steady_clock::now();
   if (last_loss) *last_loss = trainer.lastLoss();
   return std::chrono::duration<double>(t1 - t0).count();
+}
+
+/* initializeFromPointCloud (GaussianCloud.cpp:199-256): n points (xyz),
+ * optional 8-bit colours (rgb, nullable); `out` arrays sized for n. */
+int ref_init_from_points(int n, const float* xyz, const unsigned char* rgb, int sh_degree, float initial_opacity,
+                         ref_cloud* out) {
+  std::vector<Vec3f> points(static_cast<size_t>(n));
+  std::vector<std::array<uint8_t, 3>> colors;
+  for (int i = 0; i < n; ++i) points[static_cast<size_t>(i)] = Vec3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  if (rgb) {
+    colors.resize(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) colors[static_cast<size_t>(i)] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
+  }
+  const GaussianCloud cloud = initializeFromPointCloud(points, colors, sh_degree, initial_opacity);
+  out->sh_degree = cloud.sh_degree;
+  fromCloud(cloud, out);
+  return out->n;
+}
+
+/* writeGaussianPly (GaussianCloud.cpp:259-303). Returns 1 on success. */
+int ref_write_ply(const ref_cloud* cloud, const char* path) { return writeGaussianPly(path, toCloud(*cloud)) ? 1 : 0; }
+
+/* readGaussianPly (GaussianCloud.cpp:305-403): first call with out == NULL
+ * for the count and degree, then with arrays sized for them. Returns 1 on
+ * success; the engine's error message goes to `error` (nullable). */
+int ref_read_ply(const char* path, int* n, int* sh_degree, ref_cloud* out, char* error, int error_len) {
+  GaussianCloud cloud;
+  std::string message;
+  if (!readGaussianPly(path, &cloud, &message)) {
+    if (error && error_len > 0) {
+      std::strncpy(error, message.c_str(), static_cast<size_t>(error_len - 1));
+      error[error_len - 1] = 0;
+    }
+    return 0;
+  }
+  *n = static_cast<int>(cloud.size());
+  *sh_degree = cloud.sh_degree;
+  if (out) {
+    out->sh_degree = cloud.sh_degree;
+    fromCloud(cloud, out);
+  }
+  return 1;
 }
 
 }  // extern "C"

@@ -103,6 +103,11 @@ SIGNATURES = {
     "splat_set_raster_config": (C.c_int, [C.c_void_p, C.c_void_p]),
     "splat_get_raster_config": (C.c_int, [C.c_void_p, C.c_void_p]),
     "splat_reset_opacity": (C.c_int, [C.c_void_p, C.c_float]),
+    "splat_init_from_points": (C.c_int, [C.c_void_p, C.c_int32, f32p, C.POINTER(C.c_uint8), C.c_int32, C.c_float,
+                                         C.POINTER(C.c_float)]),
+    "splat_write_ply": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "splat_ply_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]),
+    "splat_read_ply": (C.c_int, [C.c_void_p, C.c_char_p]),
     "splat_download_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
     "splat_prune_and_sort": (C.c_int, [C.c_void_p, C.c_float]),
     "splat_upload_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
@@ -351,6 +356,24 @@ class Context:
     def reset_opacity(self, ceiling: float):
         self._ck(lib().splat_reset_opacity(self.h, float(ceiling)))
 
+    def init_from_points(self, xyz, rgb=None, sh_degree: int = 0, initial_opacity: float = 0.1) -> float:
+        """initializeFromPointCloud on the device (capacity already set);
+        returns the shared log sigma."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        col = None if rgb is None else np.ascontiguousarray(rgb, np.uint8).reshape(-1, 3)
+        ls = C.c_float()
+        self._ck(lib().splat_init_from_points(self.h, len(xyz), _p(xyz),
+                                              None if col is None else col.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                              int(sh_degree), float(initial_opacity), C.byref(ls)))
+        return ls.value
+
+    def write_ply(self, path: str):
+        self._ck(lib().splat_write_ply(self.h, os.fsencode(path)))
+
+    def read_ply(self, path: str):
+        self._ck(lib().splat_read_ply(self.h, os.fsencode(path)))
+
+
     def stats(self):
         n = self.count
         sg = np.zeros(n, np.float32)
@@ -429,3 +452,14 @@ class Context:
 
     def synchronize(self):
         self._ck(lib().splat_synchronize(self.h))
+
+
+def ply_info(path: str):
+    """(count, sh_degree) of a 3DGS PLY file; raises SplatError with the
+    engine's message when the file is not one."""
+    n, d = C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(256)
+    st = lib().splat_ply_info(os.fsencode(path), C.byref(n), C.byref(d), err, 256)
+    if st != OK:
+        raise SplatError(st, err.value.decode())
+    return n.value, d.value

@@ -8,7 +8,13 @@ Each fixture holds the inputs and the reference's outputs:
   densify_*.npz  one densifyAndPrune call each (cloud, source_index, counts,
                  RNG): one dominated by pruning, one that reaches the cap;
   trainer.npz    SplatTrainer runs: per-step losses and the final cloud,
-                 including one with densification on.
+                 including one with densification on;
+  init_points.npz  initializeFromPointCloud on seeded point sets (with and
+                 without colours);
+  cloud_sh2.ply + ply_sh2.npz  writeGaussianPly's bytes for a seeded SH2
+                 cloud, and readGaussianPly's cloud for a hand-laid PLY
+                 (other property order, no normals or f_dc, unnormalised and
+                 zero quaternions).
 Run from the repository root: python tests/golden/make_golden.py
 """
 import ctypes as C
@@ -147,6 +153,66 @@ def ref_trainer():
     np.savez_compressed(os.path.join(HERE, "trainer.npz"), **data)
 
 
+def empty_ccloud(n, rest_stride, sh_degree=0):
+    arrs = [np.zeros(k * max(1, n), np.float32) for k in (3, 3, 4, 1, 3, max(1, rest_stride))]
+    return O._Keep(O.CCloud(n, sh_degree, rest_stride, *[O.ptr(a) for a in arrs]), arrs)
+
+
+def ref_init_points():
+    R = O.ref()
+    data = {}
+    for tag, n, seed, deg, opacity, colours in (("rgb_sh1", 3000, 31, 1, 0.1, True),
+                                                 ("plain_sh0", 700, 32, 0, 0.35, False),
+                                                 ("big_sh3", 8000, 33, 3, 0.1, True)):
+        rng = np.random.default_rng(seed)
+        xyz = (rng.normal(size=(n, 3)) * np.array([1.5, 0.4, 1.0])).astype(np.float32)
+        rgb = rng.integers(0, 256, size=(n, 3), dtype=np.uint8) if colours else None
+        rs = 3 * [0, 3, 8, 15][deg]
+        out = empty_ccloud(n, rs)
+        R.ref_init_from_points(n, O.ptr(xyz), None if rgb is None else rgb.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                               deg, opacity, C.byref(out.struct))
+        data[f"{tag}_xyz"] = xyz
+        if rgb is not None:
+            data[f"{tag}_rgb"] = rgb
+        data[f"{tag}_args"] = np.array([deg, opacity], np.float64)
+        for k, a in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), out.arrays):
+            data[f"{tag}_out_{k}"] = a[: a.size if k != "rest" or rs else 0]
+        print(f"init_points {tag}: log sigma {out.arrays[1][0]:.6f}")
+    np.savez_compressed(os.path.join(HERE, "init_points.npz"), **data)
+
+
+def ref_ply():
+    R = O.ref()
+    cloud = syn.random_cloud(500, 37, 2, (-2, -2, -2), (2, 2, 2))
+    k = O.ccloud(cloud)
+    path = os.path.join(HERE, "cloud_sh2.ply")
+    assert R.ref_write_ply(C.byref(k.struct), path.encode()) == 1
+    data = {f"in_{n}": a for n, a in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), k.arrays)}
+    # a hand-laid file: other property order, no normals, no f_dc, rotations
+    # that need renormalising (one zero)
+    rng = np.random.default_rng(38)
+    n = 64
+    props = ["opacity", "z", "y", "x"] + [f"f_rest_{i}" for i in range(9)] + \
+            ["rot_0", "rot_1", "rot_2", "rot_3", "scale_2", "scale_0", "scale_1"]
+    rows = rng.normal(size=(n, len(props))).astype(np.float32)
+    rows[5, props.index("rot_0"):props.index("rot_3") + 1] = 0.0
+    header = "ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n + \
+             "".join(f"property float {p}\n" for p in props) + "end_header\n"
+    laid = os.path.join(HERE, "laid_sh1.ply")
+    with open(laid, "wb") as f:
+        f.write(header.encode() + rows.tobytes())
+    cnt, deg = C.c_int(), C.c_int()
+    err = C.create_string_buffer(256)
+    assert R.ref_read_ply(laid.encode(), C.byref(cnt), C.byref(deg), None, err, 256) == 1
+    out = empty_ccloud(cnt.value, 3 * [0, 3, 8, 15][deg.value], deg.value)
+    assert R.ref_read_ply(laid.encode(), C.byref(cnt), C.byref(deg), C.byref(out.struct), err, 256) == 1
+    data["laid_degree"] = np.int32(deg.value)
+    for nm, a in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), out.arrays):
+        data[f"laid_out_{nm}"] = a
+    np.savez_compressed(os.path.join(HERE, "ply_sh2.npz"), **data)
+    print(f"ply: wrote {os.path.getsize(path)} bytes; read {cnt.value} gaussians of degree {deg.value}")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -164,6 +230,8 @@ def main():
     ref_densify("prune", 17, -2.0, 560, 3500)
     ref_densify("cap", 23, 1.0, 470, 200)
     ref_trainer()
+    ref_init_points()
+    ref_ply()
 
 
 if __name__ == "__main__":
