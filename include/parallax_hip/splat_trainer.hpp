@@ -228,4 +228,45 @@ bool trainSplatsOnGpu(GaussianCloud* cloud, const std::vector<SplatTrainingView>
 SplatImage renderSplats(const GaussianCloud& cloud, const SplatView& view, const RasterOptions& options,
                         std::string* error = nullptr);
 
+/// initializeFromPointCloud (GaussianCloud.h:90-93), the nearest-neighbour
+/// spacing and the per-point fill on the device. `colors` empty or one per
+/// point. An empty cloud (and `error` set) if the device cannot do it.
+GaussianCloud initializeFromPointCloud(const std::vector<std::array<float, 3>>& points,
+                                       const std::vector<std::array<uint8_t, 3>>& colors, int sh_degree = 0,
+                                       float initial_opacity = 0.1f, std::string* error = nullptr);
+
+/// writeGaussianPly / readGaussianPly (GaussianCloud.h:102-103): the 3DGS
+/// binary PLY, rows packed and unpacked on the device, bytes as the engine
+/// writes them.
+bool writeGaussianPly(const std::string& path, const GaussianCloud& cloud, std::string* error = nullptr);
+bool readGaussianPly(const std::string& path, GaussianCloud* out, std::string* error = nullptr);
+
+/// The device half of the splat stage (Pipeline.cpp:1844-2010) from the
+/// point cloud on: seed at most a quarter of the gaussian budget from the
+/// points (every step-th point), derive the schedule from the iteration count
+/// as the stage does, train on the GPU under the stage's cancel / time-budget
+/// callback rules, and write the field to each of `ply_paths`. Returns false
+/// with `error` set when the device cannot do it -- the caller then falls back
+/// to the CPU trainer as the stage does today -- or when cancelled.
+struct SplatStageSettings {
+  int iterations = 7000;
+  int sh_degree = 3;
+  int max_gaussians = 600000;
+  long long time_budget_millis = 0;  // 0: none
+  int batch_views = 1;
+  int device = 0;
+};
+struct SplatStageResult {
+  GaussianCloud cloud;
+  std::string backend;  // "hip"
+  int iterations_done = 0;
+  bool ran_out_of_time = false;
+  bool cancelled = false;
+  std::string note;
+};
+bool runSplatStageOnGpu(const std::vector<std::array<float, 3>>& points,
+                        const std::vector<std::array<uint8_t, 3>>& colors, std::vector<SplatTrainingView> views,
+                        const SplatStageSettings& settings, const std::function<bool()>& cancelled,
+                        const std::vector<std::string>& ply_paths, SplatStageResult* result, std::string* error);
+
 }  // namespace parallax_hip

@@ -8,6 +8,7 @@
 #include "../../include/parallax_hip/splat_trainer.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -473,6 +474,164 @@ SplatImage renderSplats(const GaussianCloud& cloud, const SplatView& view, const
   std::unique_ptr<SplatBackend> backend = makeHipSplatBackend(error);
   if (!backend) return SplatImage{};
   return backend->render(cloud, view, options);
+}
+
+namespace {
+
+// A context sized for `n` gaussians of `degree` (null and `error` set if the
+// device cannot be used).
+splat_context* makeContext(int n, int degree, int device, std::string* error) {
+  splat_context* ctx = nullptr;
+  if (splat_create(device, &ctx) != SPLAT_OK) {
+    if (error) *error = splat_last_error(nullptr);
+    return nullptr;
+  }
+  if (splat_set_capacity(ctx, std::max(1, n), degree) != SPLAT_OK) {
+    if (error) *error = splat_last_error(ctx);
+    splat_destroy(ctx);
+    return nullptr;
+  }
+  return ctx;
+}
+
+}  // namespace
+
+GaussianCloud initializeFromPointCloud(const std::vector<std::array<float, 3>>& points,
+                                       const std::vector<std::array<uint8_t, 3>>& colors, int sh_degree,
+                                       float initial_opacity, std::string* error) {
+  GaussianCloud cloud;
+  cloud.sh_degree = std::max(0, std::min(3, sh_degree));
+  if (points.empty()) return cloud;
+  const int n = static_cast<int>(points.size());
+  splat_context* ctx = makeContext(n, cloud.sh_degree, 0, error);
+  if (!ctx) return cloud;
+  std::vector<float> xyz(3 * points.size());
+  for (size_t i = 0; i < points.size(); ++i)
+    for (int k = 0; k < 3; ++k) xyz[3 * i + static_cast<size_t>(k)] = points[i][static_cast<size_t>(k)];
+  std::vector<uint8_t> rgb;
+  if (colors.size() == points.size()) {
+    rgb.resize(3 * colors.size());
+    for (size_t i = 0; i < colors.size(); ++i)
+      for (int k = 0; k < 3; ++k) rgb[3 * i + static_cast<size_t>(k)] = colors[i][static_cast<size_t>(k)];
+  }
+  if (splat_init_from_points(ctx, n, xyz.data(), rgb.empty() ? nullptr : rgb.data(), cloud.sh_degree,
+                             initial_opacity, nullptr) != SPLAT_OK) {
+    if (error) *error = splat_last_error(ctx);
+  } else {
+    cloud = download(ctx, cloud.sh_degree);
+  }
+  splat_destroy(ctx);
+  return cloud;
+}
+
+bool writeGaussianPly(const std::string& path, const GaussianCloud& cloud, std::string* error) {
+  if (cloud.empty()) {
+    if (error) *error = "the cloud is empty";
+    return false;
+  }
+  SoA soa(cloud);
+  splat_context* ctx = makeContext(soa.n, cloud.sh_degree, 0, error);
+  if (!ctx) return false;
+  const splat_cloud c = soa.abi();
+  const bool ok = splat_upload_cloud(ctx, &c) == SPLAT_OK && splat_write_ply(ctx, path.c_str()) == SPLAT_OK;
+  if (!ok && error) *error = splat_last_error(ctx);
+  splat_destroy(ctx);
+  return ok;
+}
+
+bool readGaussianPly(const std::string& path, GaussianCloud* out, std::string* error) {
+  int32_t n = 0, degree = 0;
+  char message[256] = {0};
+  if (splat_ply_info(path.c_str(), &n, &degree, message, sizeof message) != SPLAT_OK) {
+    if (error) *error = message;
+    return false;
+  }
+  splat_context* ctx = makeContext(n, degree, 0, error);
+  if (!ctx) return false;
+  const bool ok = splat_read_ply(ctx, path.c_str()) == SPLAT_OK;
+  if (!ok) {
+    if (error) *error = splat_last_error(ctx);
+  } else if (out) {
+    *out = download(ctx, degree);
+  }
+  splat_destroy(ctx);
+  return ok;
+}
+
+bool runSplatStageOnGpu(const std::vector<std::array<float, 3>>& points,
+                        const std::vector<std::array<uint8_t, 3>>& colors, std::vector<SplatTrainingView> views,
+                        const SplatStageSettings& settings, const std::function<bool()>& cancelled,
+                        const std::vector<std::string>& ply_paths, SplatStageResult* result, std::string* error) {
+  SplatStageResult local;
+  SplatStageResult& r = result ? *result : local;
+  r = SplatStageResult{};
+  const auto fail = [error](const std::string& m) {
+    if (error) *error = m;
+    return false;
+  };
+  if (points.empty()) return fail("no point cloud to seed the splat field from");
+  if (views.size() < 2) return fail("at least two posed views are needed to train splats");
+  // seed with a quarter of the budget (Pipeline.cpp:1860-1876)
+  const size_t seed_budget = std::max<size_t>(1, static_cast<size_t>(settings.max_gaussians) / 4);
+  std::vector<std::array<float, 3>> kept_points;
+  std::vector<std::array<uint8_t, 3>> kept_colors;
+  const std::vector<std::array<float, 3>>* seed_points = &points;
+  const std::vector<std::array<uint8_t, 3>>* seed_colors = &colors;
+  if (points.size() > seed_budget) {
+    const size_t step = (points.size() + seed_budget - 1) / seed_budget;
+    for (size_t i = 0; i < points.size(); i += step) {
+      kept_points.push_back(points[i]);
+      kept_colors.push_back(i < colors.size() ? colors[i] : std::array<uint8_t, 3>{128, 128, 128});
+    }
+    seed_points = &kept_points;
+    seed_colors = &kept_colors;
+  }
+  std::string init_error;
+  GaussianCloud cloud = initializeFromPointCloud(*seed_points, *seed_colors, settings.sh_degree, 0.1f, &init_error);
+  if (cloud.empty()) return fail(init_error.empty() ? "the point cloud produced no gaussians" : init_error);
+  // the stage's schedule (Pipeline.cpp:1920-1926)
+  SplatTrainingConfig config;
+  config.iterations = std::max(1, settings.iterations);
+  config.sh_degree = settings.sh_degree;
+  config.max_gaussians = settings.max_gaussians;
+  config.densify_until_iteration = config.iterations / 2;
+  config.opacity_reset_interval = std::max(1, config.iterations / 3);
+  config.sh_increase_interval = std::max(1, config.iterations / 8);
+  config.batch_views = std::max(1, settings.batch_views);
+  config.device = settings.device;
+  const std::string invalid = config.validate();
+  if (!invalid.empty()) return fail(invalid);
+  // cancel and time budget through the per-iteration callback (:1929-1954)
+  const auto started = std::chrono::steady_clock::now();
+  const SplatTrainingCallback callback = [&](const SplatTrainingProgress& p) {
+    if (cancelled && cancelled()) {
+      r.cancelled = true;
+      return false;
+    }
+    r.iterations_done = p.iteration;
+    if (settings.time_budget_millis > 0) {
+      const auto elapsed =
+          std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - started).count();
+      if (elapsed >= settings.time_budget_millis) {
+        r.ran_out_of_time = true;
+        return false;
+      }
+    }
+    return true;
+  };
+  std::string train_error;
+  if (!trainSplatsOnGpu(&cloud, views, config, callback, &train_error)) return fail(train_error);
+  if (r.cancelled) return fail("cancelled");
+  r.backend = "hip";
+  for (const std::string& path : ply_paths) {
+    std::string write_error;
+    if (!writeGaussianPly(path, cloud, &write_error)) return fail("could not write " + path + ": " + write_error);
+  }
+  r.note = std::to_string(cloud.size()) + " gaussians (hip)";
+  if (r.ran_out_of_time)
+    r.note += ", stopped at iteration " + std::to_string(r.iterations_done) + " when the time budget ran out";
+  r.cloud = std::move(cloud);
+  return true;
 }
 
 }  // namespace parallax_hip

@@ -341,6 +341,81 @@ int main() {
     check(ok, t, "twice != 2x once");
   });
 
+  run("init_from_points_spacing", [](const char* t) {
+    // a 6x6x6 grid of pitch 0.05: every sampled point's nearest neighbour is
+    // one pitch away, so log sigma = log(0.025)
+    std::vector<std::array<float, 3>> pts;
+    std::vector<std::array<uint8_t, 3>> cols;
+    for (int z = 0; z < 6; ++z)
+      for (int y = 0; y < 6; ++y)
+        for (int x = 0; x < 6; ++x) {
+          pts.push_back({0.05f * x, 0.05f * y, 0.05f * z});
+          cols.push_back({static_cast<uint8_t>(40 * x), 128, 255});
+        }
+    std::string err;
+    const GaussianCloud c = initializeFromPointCloud(pts, cols, 1, 0.1f, &err);
+    check(c.size() == pts.size() && c.sh_degree == 1, t, "size " + std::to_string(c.size()) + " " + err);
+    bool ok = !c.empty();
+    for (const Gaussian& g : c.gaussians) {
+      ok = ok && std::abs(g.log_scale[0] - std::log(0.025f)) < 1e-5f && g.rotation[0] == 1.f &&
+           g.sh_rest.size() == 9 && std::abs(g.opacity_logit - std::log(0.1f / 0.9f)) < 1e-6f;
+    }
+    check(ok, t, "fields");
+    check(std::abs(c.gaussians[0].dc[2] - (1.f - 0.5f) / kShC0) < 1e-6f, t, "colour");
+  });
+
+  run("ply_round_trip", [](const char* t) {
+    GaussianCloud c = roughStart(40, 9);
+    c.sh_degree = 1;
+    for (Gaussian& g : c.gaussians) g.sh_rest.assign(9, 0.25f);
+    const std::string path = "/tmp/parallax_hip_facade_test.ply";
+    std::string err;
+    check(writeGaussianPly(path, c, &err), t, "write: " + err);
+    GaussianCloud back;
+    check(readGaussianPly(path, &back, &err), t, "read: " + err);
+    bool same = back.size() == c.size() && back.sh_degree == 1;
+    for (size_t i = 0; same && i < c.size(); ++i)
+      same = back.gaussians[i].position == c.gaussians[i].position &&
+             back.gaussians[i].sh_rest == c.gaussians[i].sh_rest;
+    check(same, t, "round trip");
+    std::remove(path.c_str());
+    check(!readGaussianPly(path, &back, &err) && !err.empty(), t, "missing file reports an error");
+  });
+
+  run("splat_stage_on_gpu", [](const char* t) {
+    const GaussianCloud truth = truthCloud();
+    const auto set = renderTrainingSet(truth, ringOfViews(4, 2.5, 48, 40));
+    std::vector<std::array<float, 3>> pts;
+    std::vector<std::array<uint8_t, 3>> cols;
+    Rng rng(5);
+    for (int i = 0; i < 400; ++i) {
+      pts.push_back({static_cast<float>(rng.uniform(-0.4, 0.4)), static_cast<float>(rng.uniform(-0.3, 0.3)),
+                     static_cast<float>(rng.uniform(-0.3, 0.3))});
+      cols.push_back({200, 100, 50});
+    }
+    SplatStageSettings s;
+    s.iterations = 60;
+    s.sh_degree = 0;
+    s.max_gaussians = 400;  // seeds a quarter: every 4th point
+    const std::string path = "/tmp/parallax_hip_stage_test.ply";
+    SplatStageResult r;
+    std::string err;
+    const bool ok = runSplatStageOnGpu(pts, cols, set, s, nullptr, {path}, &r, &err);
+    check(ok && r.backend == "hip" && r.iterations_done == 60, t, err + " " + r.note);
+    check(!r.cloud.empty() && r.cloud.size() <= 100, t, "seeded from a quarter: " + std::to_string(r.cloud.size()));
+    GaussianCloud back;
+    check(readGaussianPly(path, &back, &err) && back.size() == r.cloud.size(), t, "field written: " + err);
+    std::remove(path.c_str());
+    // a zero time budget stops at once and keeps the field; cancellation fails
+    s.time_budget_millis = 1;
+    SplatStageResult timed;
+    check(runSplatStageOnGpu(pts, cols, set, s, nullptr, {}, &timed, &err) && timed.ran_out_of_time, t,
+          "time budget: " + timed.note);
+    check(!runSplatStageOnGpu(pts, cols, set, s, [] { return true; }, {}, &timed, &err) && timed.cancelled, t,
+          "cancel");
+    check(!runSplatStageOnGpu({}, cols, set, s, nullptr, {}, &timed, &err), t, "no points");
+  });
+
   std::printf("%d failure(s)\n", failures);
   return failures;
 }
