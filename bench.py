@@ -48,8 +48,9 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="default compositing launch shape")
     ap.add_argument("--profile", action="store_true",
                     help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
-    ap.add_argument("--workload", default="headline", choices=["headline", "c1", "c3", "c4", "c5"],
-                    help="headline: the benchmark line; c1/c3/c4/c5: the other configurations' measurements")
+    ap.add_argument("--workload", default="headline", choices=["headline", "c1", "c3", "c4", "c5", "formats"],
+                    help="headline: the benchmark line; c1/c3/c4/c5: the other configurations' measurements; "
+                         "formats: seeding from a point cloud and the PLY file, against the reference")
     ap.add_argument("--train-steps", type=int, default=2000, help="c4: steps of the densifying run")
     ap.add_argument("--no-check", action="store_true", help="c4: skip the CPU densify comparison")
     ap.add_argument("--grad-quantile", type=float, default=None,
@@ -76,7 +77,8 @@ def main():
         raise SystemExit(f"no usable device: {reason}")
     log("device", name)
     if args.workload != "headline":
-        out = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args, S, syn)
+        out = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+               "formats": bench_formats}[args.workload](args, S, syn)
         print(json.dumps(out), flush=True)
         return
     n = args.gaussians
@@ -335,6 +337,64 @@ def cpu_baseline(cloud, view):
             "host_cpu": cpu_name, "host_threads": os.cpu_count()}
 
 
+
+
+def bench_formats(args, S, syn):
+    """The stages either side of training at 1M SH3: seeding from 1M coloured
+    points (initializeFromPointCloud) and writing / reading the 3DGS PLY
+    (236 MB), on the device against the reference engine's own functions on
+    one host core. Times include the host-device copies and the file I/O."""
+    import ctypes as C
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    n = 1_000_000
+    rng = np.random.default_rng(3)
+    xyz = (rng.normal(size=(n, 3)) * np.array([2.0, 0.6, 1.5])).astype(np.float32)
+    rgb = rng.integers(0, 256, size=(n, 3), dtype=np.uint8)
+    cloud = syn.random_cloud(n, 9, 3, (-2, -2, -2), (2, 2, 2))
+    ctx = S.Context(0)
+    ctx.set_capacity(n, 3)
+    out = {"workload": "seeding from 1M coloured points and the 1M SH3 3DGS PLY (236 MB), device vs the reference "
+                       "engine on one host core; times include copies and file I/O"}
+
+    def best(fn, reps=3):
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        return min(t)
+
+    out["init_ms"] = round(1e3 * best(lambda: ctx.init_from_points(xyz, rgb, 3, 0.1)), 2)
+    ctx.upload_cloud(cloud)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "field.ply")
+        out["write_ply_ms"] = round(1e3 * best(lambda: ctx.write_ply(path)), 2)
+        out["read_ply_ms"] = round(1e3 * best(lambda: ctx.read_ply(path)), 2)
+        out["ply_bytes"] = os.path.getsize(path)
+        if not args.no_cpu and O.ref_available():
+            R = O.ref()
+            k = O.ccloud(cloud)
+            rs = cloud.rest_stride
+            dst = [np.zeros(m * n, np.float32) for m in (3, 3, 4, 1, 3, rs)]
+            kc = O.CCloud(n, 3, rs, *[O.ptr(a) for a in dst])
+            t0 = time.perf_counter()
+            R.ref_init_from_points(n, O.ptr(xyz), rgb.ctypes.data_as(C.POINTER(C.c_ubyte)), 3, 0.1, C.byref(kc))
+            out["cpu_reference_init_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            ref_path = os.path.join(d, "ref.ply")
+            t0 = time.perf_counter()
+            R.ref_write_ply(C.byref(k.struct), ref_path.encode())
+            out["cpu_reference_write_ply_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            cnt, deg = C.c_int(), C.c_int()
+            err = C.create_string_buffer(256)
+            t0 = time.perf_counter()
+            R.ref_read_ply(ref_path.encode(), C.byref(cnt), C.byref(deg), C.byref(kc), err, 256)
+            out["cpu_reference_read_ply_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            out["files_identical"] = open(ref_path, "rb").read() == open(path, "rb").read()
+    ctx.close()
+    return out
 
 
 # --------------------------------------------------- the other workloads --
