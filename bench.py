@@ -50,7 +50,8 @@ def parse():
                     help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
     ap.add_argument("--workload", default="headline", choices=["headline", "c1", "c3", "c4", "c5", "formats"],
                     help="headline: the benchmark line; c1/c3/c4/c5: the other configurations' measurements; "
-                         "formats: seeding from a point cloud and the PLY file, against the reference")
+                         "formats: seeding from a point cloud and the PLY, .spz and .splat files, against the "
+                         "reference")
     ap.add_argument("--train-steps", type=int, default=2000, help="c4: steps of the densifying run")
     ap.add_argument("--no-check", action="store_true", help="c4: skip the CPU densify comparison")
     ap.add_argument("--grad-quantile", type=float, default=None,
@@ -341,9 +342,10 @@ def cpu_baseline(cloud, view):
 
 def bench_formats(args, S, syn):
     """The stages either side of training at 1M SH3: seeding from 1M coloured
-    points (initializeFromPointCloud) and writing / reading the 3DGS PLY
-    (236 MB), on the device against the reference engine's own functions on
-    one host core. Times include the host-device copies and the file I/O."""
+    points (initializeFromPointCloud), writing / reading the 3DGS PLY
+    (236 MB), the .spz and the .splat, on the device against the reference
+    engine's own functions on one host core. Times include the host-device
+    copies, the .spz deflate / inflate on the host and the file I/O."""
     import ctypes as C
     import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -356,8 +358,9 @@ def bench_formats(args, S, syn):
     cloud = syn.random_cloud(n, 9, 3, (-2, -2, -2), (2, 2, 2))
     ctx = S.Context(0)
     ctx.set_capacity(n, 3)
-    out = {"workload": "seeding from 1M coloured points and the 1M SH3 3DGS PLY (236 MB), device vs the reference "
-                       "engine on one host core; times include copies and file I/O"}
+    out = {"workload": "seeding from 1M coloured points, and the 1M SH3 field as 3DGS PLY (236 MB), .spz and "
+                       ".splat; device vs the reference engine on one host core; times include copies, the .spz "
+                       "deflate / inflate and file I/O"}
 
     def best(fn, reps=3):
         t = []
@@ -393,6 +396,45 @@ def bench_formats(args, S, syn):
             R.ref_read_ply(ref_path.encode(), C.byref(cnt), C.byref(deg), C.byref(kc), err, 256)
             out["cpu_reference_read_ply_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
             out["files_identical"] = open(ref_path, "rb").read() == open(path, "rb").read()
+        # the compact exporters: .spz (quantised on the device, deflated on
+        # the host) and .splat (32-byte records, read back at SH degree 0)
+        spz = os.path.join(d, "field.spz")
+        ctx.upload_cloud(cloud)  # the PLY read renormalised the rotations
+        out["write_spz_ms"] = round(1e3 * best(lambda: ctx.write_spz(spz, 12, False)), 2)
+        out["read_spz_ms"] = round(1e3 * best(lambda: ctx.read_spz(spz)), 2)
+        out["spz_bytes"] = os.path.getsize(spz)
+        ctx.upload_cloud(cloud)
+        spl = os.path.join(d, "field.splat")
+        out["write_splat_ms"] = round(1e3 * best(lambda: ctx.write_splat(spl)), 2)
+        ctx0 = S.Context(0)
+        ctx0.set_capacity(n, 0)
+        out["read_splat_ms"] = round(1e3 * best(lambda: ctx0.read_splat(spl)), 2)
+        ctx0.close()
+        if not args.no_cpu and O.ref_available():
+            ref_spz = os.path.join(d, "ref.spz")
+            t0 = time.perf_counter()
+            R.ref_write_spz(C.byref(k.struct), ref_spz.encode(), 12, 0)
+            out["cpu_reference_write_spz_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            aa = C.c_int()
+            t0 = time.perf_counter()
+            R.ref_read_spz(ref_spz.encode(), C.byref(cnt), C.byref(deg), C.byref(aa), C.byref(kc), err, 256)
+            out["cpu_reference_read_spz_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            out["spz_identical"] = open(ref_spz, "rb").read() == open(spz, "rb").read()
+            ref_spl = os.path.join(d, "ref.splat")
+            t0 = time.perf_counter()
+            R.ref_write_splat(C.byref(k.struct), ref_spl.encode())
+            out["cpu_reference_write_splat_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            k0 = O.CCloud(n, 0, 0, *[O.ptr(a) for a in dst])
+            t0 = time.perf_counter()
+            R.ref_read_splat(ref_spl.encode(), C.byref(cnt), C.byref(deg), C.byref(k0), err, 256)
+            out["cpu_reference_read_splat_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            a = np.fromfile(ref_spl, np.uint8).reshape(-1, 32)
+            b = np.fromfile(spl, np.uint8).reshape(-1, 32)
+            sig = np.abs(a[:, 12:24].copy().view(np.int32).astype(np.int64) -
+                         b[:, 12:24].copy().view(np.int32).astype(np.int64))
+            out["splat_records_identical_but_sigma"] = bool(np.array_equal(a[:, :12], b[:, :12]) and
+                                                            np.array_equal(a[:, 24:], b[:, 24:]))
+            out["splat_sigma_max_ulp"] = int(sig.max())
     ctx.close()
     return out
 

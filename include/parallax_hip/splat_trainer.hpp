@@ -52,6 +52,9 @@ struct Gaussian {
 struct GaussianCloud {
   std::vector<Gaussian> gaussians;
   int sh_degree = 0;
+  /// Trained with the antialiased rasteriser: carried by .spz as a flag
+  /// (GaussianCloud.h:58-60).
+  bool antialiased = false;
   size_t size() const { return gaussians.size(); }
   bool empty() const { return gaussians.empty(); }
 };
@@ -240,6 +243,19 @@ GaussianCloud initializeFromPointCloud(const std::vector<std::array<float, 3>>& 
 /// writes them.
 bool writeGaussianPly(const std::string& path, const GaussianCloud& cloud, std::string* error = nullptr);
 bool readGaussianPly(const std::string& path, GaussianCloud* out, std::string* error = nullptr);
+
+/// writeSplat / readSplat (GaussianCloud.h:117-118): antimatter15's 32-byte
+/// records, quantised and decoded on the device. Higher SH bands are dropped
+/// on write; a read cloud has SH degree 0.
+bool writeSplat(const std::string& path, const GaussianCloud& cloud, std::string* error = nullptr);
+bool readSplat(const std::string& path, GaussianCloud* out, std::string* error = nullptr);
+
+/// writeSpz / readSpz (GaussianCloud.h:130-131): Niantic's .spz v2. The
+/// attribute planes are quantised and decoded on the device; the gzip stream
+/// is deflated and inflated on the host with the engine's zlib settings.
+bool writeSpz(const std::string& path, const GaussianCloud& cloud, int fractional_bits = 12,
+              std::string* error = nullptr);
+bool readSpz(const std::string& path, GaussianCloud* out, std::string* error = nullptr);
 
 /// The device half of the splat stage (Pipeline.cpp:1844-2010) from the
 /// point cloud on: seed at most a quarter of the gaussian budget from the

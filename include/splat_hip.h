@@ -31,6 +31,9 @@
  *   initializeFromPointCloud  GaussianCloud.h:90-93        splat_init_from_points
  *   writeGaussianPly / readGaussianPly  GaussianCloud.h:102-103
  *                                                        splat_write_ply / splat_read_ply (+ splat_ply_info)
+ *   writeSplat / readSplat      GaussianCloud.h:117-118    splat_write_splat / splat_read_splat
+ *                                                        (+ splat_splat_info)
+ *   writeSpz / readSpz          GaussianCloud.h:130-131    splat_write_spz / splat_read_spz (+ splat_spz_info)
  *
  * Conventions follow the engine: results are returned through out-parameters,
  * every call returns a status and never throws, and on failure
@@ -264,6 +267,34 @@ splat_status splat_ply_info(const char* path, int32_t* count, int32_t* sh_degree
  * identity, rotations of norm > 1e-6 renormalised; moments and statistics
  * cleared. The capacity must hold the count at the file's SH degree. */
 splat_status splat_read_ply(splat_context* ctx, const char* path);
+
+/* writeSplat (GaussianCloud.cpp:416-446): the context's cloud as .splat
+ * records (32 bytes each: position, exp(log sigma), base colour and opacity
+ * as bytes, the normalised quaternion as q * 128 + 128), quantised on the
+ * device. The bytes are the engine's but for the sigma floats, which are the
+ * correctly rounded exponential (the engine's libm expf is within 1 ulp). */
+splat_status splat_write_splat(splat_context* ctx, const char* path);
+/* The record count of a .splat file (its SH degree is always 0); on failure
+ * the engine's message goes to `error` (nullable). Needs no context. */
+splat_status splat_splat_info(const char* path, int32_t* count, char* error, size_t error_len);
+/* readSplat (GaussianCloud.cpp:448-493) into the context, decoded on the
+ * device. The capacity must be at SH degree 0 and hold the count. */
+splat_status splat_read_splat(splat_context* ctx, const char* path);
+/* writeSpz (GaussianCloud.cpp:497-560): .spz v2 -- the attribute planes are
+ * quantised on the device, then the host deflates them with the engine's
+ * zlib settings; the file is byte for byte the engine's. `fractional_bits`
+ * is clamped to 0..20 (the engine's default is 12); `antialiased` sets the
+ * header flag. */
+splat_status splat_write_spz(splat_context* ctx, const char* path, int32_t fractional_bits, int32_t antialiased);
+/* The count, SH degree and antialiased flag of a .spz file, after readSpz's
+ * checks of the whole inflated stream; on failure the engine's message goes to
+ * `error` (nullable). Needs no context. */
+splat_status splat_spz_info(const char* path, int32_t* count, int32_t* sh_degree, int32_t* antialiased, char* error,
+                            size_t error_len);
+/* readSpz (GaussianCloud.cpp:562-641) into the context: inflated on the host,
+ * decoded on the device. The capacity must hold the count at the file's SH
+ * degree; `antialiased` (nullable) receives the header flag. */
+splat_status splat_read_spz(splat_context* ctx, const char* path, int32_t* antialiased);
 
 /* ---- measurement ---- */
 /* Per-kernel timing with HIP events on the context's stream: enable, then

@@ -237,6 +237,10 @@ def ref():
                                            C.POINTER(CCloud)]
         R.ref_write_ply.argtypes = [C.POINTER(CCloud), C.c_char_p]
         R.ref_read_ply.argtypes = [C.c_char_p, i32p, i32p, C.POINTER(CCloud), C.c_char_p, C.c_int]
+        R.ref_write_splat.argtypes = [C.POINTER(CCloud), C.c_char_p]
+        R.ref_write_spz.argtypes = [C.POINTER(CCloud), C.c_char_p, C.c_int, C.c_int]
+        R.ref_read_splat.argtypes = [C.c_char_p, i32p, i32p, C.POINTER(CCloud), C.c_char_p, C.c_int]
+        R.ref_read_spz.argtypes = [C.c_char_p, i32p, i32p, i32p, C.POINTER(CCloud), C.c_char_p, C.c_int]
         R.ref_time_steps.restype = C.c_double
         R.ref_time_steps.argtypes = [C.POINTER(CCloud), C.c_int, C.POINTER(CView), C.POINTER(f32p),
                                      C.POINTER(CConfig), C.c_int, f32p]

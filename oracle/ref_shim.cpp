@@ -338,4 +338,47 @@ int ref_read_ply(const char* path, int* n, int* sh_degree, ref_cloud* out, char*
   return 1;
 }
 
+/* writeSplat (GaussianCloud.cpp:416-446) and writeSpz (:497-560). Return 1 on
+ * success. The cloud's `antialiased` flag is set from the argument. */
+int ref_write_splat(const ref_cloud* cloud, const char* path) { return writeSplat(path, toCloud(*cloud)) ? 1 : 0; }
+
+int ref_write_spz(const ref_cloud* cloud, const char* path, int fractional_bits, int antialiased) {
+  GaussianCloud c = toCloud(*cloud);
+  c.antialiased = antialiased != 0;
+  return writeSpz(path, c, fractional_bits) ? 1 : 0;
+}
+
+/* readSplat (:448-493) and readSpz (:562-641), called as ref_read_ply is:
+ * first with out == NULL for the count and degree. `antialiased` (nullable)
+ * receives the .spz flag. */
+static int read_with(bool (*reader)(const std::string&, GaussianCloud*, std::string*), const char* path, int* n,
+                     int* sh_degree, int* antialiased, ref_cloud* out, char* error, int error_len) {
+  GaussianCloud cloud;
+  std::string message;
+  if (!reader(path, &cloud, &message)) {
+    if (error && error_len > 0) {
+      std::strncpy(error, message.c_str(), static_cast<size_t>(error_len - 1));
+      error[error_len - 1] = 0;
+    }
+    return 0;
+  }
+  *n = static_cast<int>(cloud.size());
+  *sh_degree = cloud.sh_degree;
+  if (antialiased) *antialiased = cloud.antialiased ? 1 : 0;
+  if (out) {
+    out->sh_degree = cloud.sh_degree;
+    fromCloud(cloud, out);
+  }
+  return 1;
+}
+
+int ref_read_splat(const char* path, int* n, int* sh_degree, ref_cloud* out, char* error, int error_len) {
+  return read_with(readSplat, path, n, sh_degree, nullptr, out, error, error_len);
+}
+
+int ref_read_spz(const char* path, int* n, int* sh_degree, int* antialiased, ref_cloud* out, char* error,
+                 int error_len) {
+  return read_with(readSpz, path, n, sh_degree, antialiased, out, error, error_len);
+}
+
 }  // extern "C"

@@ -558,6 +558,81 @@ bool readGaussianPly(const std::string& path, GaussianCloud* out, std::string* e
   return ok;
 }
 
+namespace {
+
+// Uploads `cloud` into a fresh context and runs `write` on it.
+bool writeWith(const GaussianCloud& cloud, std::string* error,
+               const std::function<splat_status(splat_context*)>& write) {
+  if (cloud.empty()) {
+    if (error) *error = "the cloud is empty";
+    return false;
+  }
+  SoA soa(cloud);
+  splat_context* ctx = makeContext(soa.n, cloud.sh_degree, 0, error);
+  if (!ctx) return false;
+  const splat_cloud c = soa.abi();
+  const bool ok = splat_upload_cloud(ctx, &c) == SPLAT_OK && write(ctx) == SPLAT_OK;
+  if (!ok && error) *error = splat_last_error(ctx);
+  splat_destroy(ctx);
+  return ok;
+}
+
+// Sizes a fresh context for `n` gaussians of `degree`, runs `read` on it and
+// downloads the result.
+bool readWith(int n, int degree, GaussianCloud* out, std::string* error,
+              const std::function<splat_status(splat_context*)>& read) {
+  splat_context* ctx = makeContext(n, degree, 0, error);
+  if (!ctx) return false;
+  const bool ok = read(ctx) == SPLAT_OK;
+  if (!ok) {
+    if (error) *error = splat_last_error(ctx);
+  } else if (out) {
+    *out = download(ctx, degree);
+  }
+  splat_destroy(ctx);
+  return ok;
+}
+
+}  // namespace
+
+bool writeSplat(const std::string& path, const GaussianCloud& cloud, std::string* error) {
+  return writeWith(cloud, error, [&](splat_context* ctx) { return splat_write_splat(ctx, path.c_str()); });
+}
+
+bool readSplat(const std::string& path, GaussianCloud* out, std::string* error) {
+  int32_t n = 0;
+  char message[256] = {0};
+  if (splat_splat_info(path.c_str(), &n, message, sizeof message) != SPLAT_OK) {
+    if (error) *error = message;
+    return false;
+  }
+  return readWith(n, 0, out, error, [&](splat_context* ctx) { return splat_read_splat(ctx, path.c_str()); });
+}
+
+bool writeSpz(const std::string& path, const GaussianCloud& cloud, int fractional_bits, std::string* error) {
+  if (cloud.sh_degree < 0 || cloud.sh_degree > 3) {
+    if (error) *error = "sh_degree must be 0..3";
+    return false;
+  }
+  return writeWith(cloud, error, [&](splat_context* ctx) {
+    return splat_write_spz(ctx, path.c_str(), fractional_bits, cloud.antialiased ? 1 : 0);
+  });
+}
+
+bool readSpz(const std::string& path, GaussianCloud* out, std::string* error) {
+  int32_t n = 0, degree = 0, antialiased = 0;
+  char message[256] = {0};
+  if (splat_spz_info(path.c_str(), &n, &degree, &antialiased, message, sizeof message) != SPLAT_OK) {
+    if (error) *error = message;
+    return false;
+  }
+  const bool ok = readWith(n, degree, out, error, [&](splat_context* ctx) {
+    return splat_read_spz(ctx, path.c_str(), nullptr);
+  });
+  if (ok && out) out->antialiased = antialiased != 0;
+  return ok;
+}
+
 bool runSplatStageOnGpu(const std::vector<std::array<float, 3>>& points,
                         const std::vector<std::array<uint8_t, 3>>& colors, std::vector<SplatTrainingView> views,
                         const SplatStageSettings& settings, const std::function<bool()>& cancelled,

@@ -108,6 +108,13 @@ SIGNATURES = {
     "splat_write_ply": (C.c_int, [C.c_void_p, C.c_char_p]),
     "splat_ply_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]),
     "splat_read_ply": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "splat_write_splat": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "splat_splat_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]),
+    "splat_read_splat": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "splat_write_spz": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
+    "splat_spz_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.c_char_p, C.c_size_t]),
+    "splat_read_spz": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32)]),
     "splat_download_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
     "splat_prune_and_sort": (C.c_int, [C.c_void_p, C.c_float]),
     "splat_upload_stats": (C.c_int, [C.c_void_p, f32p, i32p]),
@@ -373,6 +380,21 @@ class Context:
     def read_ply(self, path: str):
         self._ck(lib().splat_read_ply(self.h, os.fsencode(path)))
 
+    def write_splat(self, path: str):
+        self._ck(lib().splat_write_splat(self.h, os.fsencode(path)))
+
+    def read_splat(self, path: str):
+        self._ck(lib().splat_read_splat(self.h, os.fsencode(path)))
+
+    def write_spz(self, path: str, fractional_bits: int = 12, antialiased: bool = False):
+        self._ck(lib().splat_write_spz(self.h, os.fsencode(path), int(fractional_bits), int(bool(antialiased))))
+
+    def read_spz(self, path: str) -> bool:
+        """Reads the file into the context; returns its antialiased flag."""
+        aa = C.c_int32()
+        self._ck(lib().splat_read_spz(self.h, os.fsencode(path), C.byref(aa)))
+        return bool(aa.value)
+
 
     def stats(self):
         n = self.count
@@ -463,3 +485,25 @@ def ply_info(path: str):
     if st != OK:
         raise SplatError(st, err.value.decode())
     return n.value, d.value
+
+
+def splat_info(path: str) -> int:
+    """The record count of a .splat file (SH degree 0); raises SplatError
+    with the engine's message when the file is not one."""
+    n = C.c_int32()
+    err = C.create_string_buffer(256)
+    st = lib().splat_splat_info(os.fsencode(path), C.byref(n), err, 256)
+    if st != OK:
+        raise SplatError(st, err.value.decode())
+    return n.value
+
+
+def spz_info(path: str):
+    """(count, sh_degree, antialiased) of a .spz file; raises SplatError with
+    the engine's message when the file is not one."""
+    n, d, aa = C.c_int32(), C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(256)
+    st = lib().splat_spz_info(os.fsencode(path), C.byref(n), C.byref(d), C.byref(aa), err, 256)
+    if st != OK:
+        raise SplatError(st, err.value.decode())
+    return n.value, d.value, bool(aa.value)

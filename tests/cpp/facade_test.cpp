@@ -382,6 +382,38 @@ int main() {
     check(!readGaussianPly(path, &back, &err) && !err.empty(), t, "missing file reports an error");
   });
 
+  run("compact_round_trips", [](const char* t) {
+    GaussianCloud c = roughStart(40, 11);
+    c.sh_degree = 1;
+    c.antialiased = true;
+    for (Gaussian& g : c.gaussians) g.sh_rest.assign(9, 0.5f);
+    std::string err;
+    // .spz: positions on the 12-bit grid, SH bytes exact for multiples of 1/128
+    const std::string spz = "/tmp/parallax_hip_facade_test.spz";
+    check(writeSpz(spz, c, 12, &err), t, "write spz: " + err);
+    GaussianCloud back;
+    check(readSpz(spz, &back, &err), t, "read spz: " + err);
+    bool ok = back.size() == c.size() && back.sh_degree == 1 && back.antialiased;
+    for (size_t i = 0; ok && i < c.size(); ++i) {
+      for (int k = 0; k < 3; ++k)
+        ok = ok && std::fabs(back.gaussians[i].position[k] - c.gaussians[i].position[k]) <= 0.5f / 4096.f;
+      ok = ok && back.gaussians[i].sh_rest == c.gaussians[i].sh_rest;
+    }
+    check(ok, t, ".spz round trip");
+    std::remove(spz.c_str());
+    // .splat: positions exact, SH dropped
+    const std::string splat = "/tmp/parallax_hip_facade_test.splat";
+    check(writeSplat(splat, c, &err), t, "write splat: " + err);
+    check(readSplat(splat, &back, &err), t, "read splat: " + err);
+    ok = back.size() == c.size() && back.sh_degree == 0;
+    for (size_t i = 0; ok && i < c.size(); ++i)
+      ok = back.gaussians[i].position == c.gaussians[i].position && back.gaussians[i].sh_rest.empty();
+    check(ok, t, ".splat round trip");
+    std::remove(splat.c_str());
+    check(!readSpz(splat, &back, &err) && err.find("cannot read") != std::string::npos, t,
+          "missing file reports the engine's message");
+  });
+
   run("splat_stage_on_gpu", [](const char* t) {
     const GaussianCloud truth = truthCloud();
     const auto set = renderTrainingSet(truth, ringOfViews(4, 2.5, 48, 40));

@@ -18,6 +18,7 @@ Each fixture holds the inputs and the reference's outputs:
 Run from the repository root: python tests/golden/make_golden.py
 """
 import ctypes as C
+import gzip
 import os
 import sys
 
@@ -213,6 +214,124 @@ def ref_ply():
     print(f"ply: wrote {os.path.getsize(path)} bytes; read {cnt.value} gaussians of degree {deg.value}")
 
 
+def compact_cloud(n, seed, deg):
+    """A cloud that reaches every branch of the compact writers: positions past
+    the 24-bit range and on rounding ties, log scales past both ends of the
+    byte, unnormalised quaternions with w < 0 and one zero quaternion, colours
+    and SH outside [0, 1] and [-1, 1]."""
+    c = syn.random_cloud(n, seed, deg, (-3, -3, -3), (3, 3, 3))
+    rng = np.random.default_rng(seed + 1000)
+    c.pos[: n // 8] *= np.float32(1000.0)
+    c.pos[n // 8: n // 4] = (rng.integers(-20000, 20000, size=(n // 4 - n // 8, 3)) + 0.5).astype(np.float32) / 4096
+    c.log_scale[:] = rng.uniform(-12, 7, size=c.log_scale.shape).astype(np.float32)
+    c.rot[:] = (rng.normal(size=c.rot.shape) * rng.uniform(0.3, 3, size=(n, 1))).astype(np.float32)
+    c.rot[7] = 0.0
+    c.opacity[:] = (rng.normal(size=n) * 5).astype(np.float32)
+    c.dc[:] = (rng.normal(size=c.dc.shape) * 2.5).astype(np.float32)
+    c.rest[:] = (rng.normal(size=c.rest.shape) * 0.7).astype(np.float32)
+    return c
+
+
+def tie_cloud():
+    """Gaussians of a 4M-gaussian seeded cloud where the literal quantiser
+    expressions and the forms the engine's optimised build evaluates round to
+    different bytes: round(255 * (1 / (1 + e))) against round(255 / (1 + e)),
+    and round(v * 127.5 + 127.5) against round((v + 1) * 127.5) for the .spz
+    rotation; plus 16 ordinary gaussians."""
+    f32 = np.float32
+    big = syn.random_cloud(4_000_000, 9, 3, (-2, -2, -2), (2, 2, 2))
+
+    def rnd(v):
+        v = np.asarray(v, np.float64)
+        return np.sign(v) * np.floor(np.abs(v) + 0.5)
+
+    e = np.exp(-big.opacity.astype(np.float64)).astype(f32)
+    alpha = rnd((f32(1) / (f32(1) + e)) * f32(255)) != rnd(f32(255) / (f32(1) + e))
+    w, x, y, z = (big.rot[:, k] for k in range(4))
+    n = np.sqrt((x * x + z * z) + (y * y + w * w)).astype(f32)
+    q = big.rot / n[:, None]
+    q = np.where(q[:, :1] < 0, -q, q)[:, 1:]
+    rot = (rnd(q * f32(127.5) + f32(127.5)) != rnd((q + f32(1)) * f32(127.5))).any(1)
+    idx = np.concatenate([np.arange(16), np.nonzero(alpha | rot)[0]])
+    print(f"ties: {int(alpha.sum())} opacity and {int(rot.sum())} rotation cases in 4M gaussians")
+    return syn.Cloud(big.pos[idx], big.log_scale[idx], big.rot[idx], big.opacity[idx], big.dc[idx],
+                     big.rest[idx], 3).contiguous()
+
+
+def read_ref(fn, path, spz=False):
+    cnt, deg, aa = C.c_int(), C.c_int(), C.c_int()
+    err = C.create_string_buffer(256)
+    head = (path.encode(), C.byref(cnt), C.byref(deg)) + ((C.byref(aa),) if spz else ())
+    assert fn(*head, None, err, 256) == 1, err.value
+    out = empty_ccloud(cnt.value, 3 * [0, 3, 8, 15][deg.value], deg.value)
+    assert fn(*head, C.byref(out.struct), err, 256) == 1
+    return out, deg.value, aa.value
+
+
+def ref_compact():
+    """writeSplat / writeSpz of two seeded clouds, and readSplat / readSpz of
+    those files and of hand-laid files covering every byte value."""
+    R = O.ref()
+    data = {}
+    for tag, c, bits, aa in (("sh3", compact_cloud(600, 41, 3), 12, 1), ("sh1", compact_cloud(200, 42, 1), 16, 0),
+                             ("ties", tie_cloud(), 12, 0)):
+        deg = c.sh_degree
+        k = O.ccloud(c)
+        for nm, a in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), k.arrays):
+            data[f"{tag}_in_{nm}"] = a
+        data[f"{tag}_args"] = np.array([deg, bits, aa], np.int32)
+        spz = os.path.join(HERE, f"compact_{tag}.spz")
+        assert R.ref_write_spz(C.byref(k.struct), spz.encode(), bits, aa) == 1
+        out, d, a = read_ref(R.ref_read_spz, spz, True)
+        assert (d, a) == (deg, aa)
+        for nm, arr in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), out.arrays):
+            data[f"{tag}_spz_out_{nm}"] = arr
+        if tag in ("sh3", "ties"):
+            sp = os.path.join(HERE, f"compact_{tag}.splat")
+            assert R.ref_write_splat(C.byref(k.struct), sp.encode()) == 1
+            out, d, _ = read_ref(R.ref_read_splat, sp)
+            for nm, arr in zip(("pos", "log_scale", "rot", "opacity", "dc"), out.arrays):
+                data[f"{tag}_splat_out_{nm}"] = arr
+    # hand-laid .splat: every byte value in every byte field, a zero
+    # quaternion, sigmas of 0, denormal, 1e-20 and ordinary sizes
+    rng = np.random.default_rng(43)
+    n = 512
+    rec = np.zeros((n, 32), np.uint8)
+    rec[:, :12] = rng.normal(size=(n, 3)).astype(np.float32).view(np.uint8).reshape(n, 12)
+    sig = np.exp(rng.uniform(-14, 3, size=(n, 3))).astype(np.float32)
+    sig[:4, 0] = np.array([0.0, 1e-40, 1e-20, 1.0], np.float32)
+    rec[:, 12:24] = sig.view(np.uint8).reshape(n, 12)
+    rec[:, 24:32] = rng.integers(0, 256, size=(n, 8), dtype=np.uint8)
+    rec[:256, 24:32] = np.arange(256, dtype=np.uint8)[:, None]
+    rec[300, 28:32] = 128
+    laid = os.path.join(HERE, "laid.splat")
+    rec.tofile(laid)
+    out, d, _ = read_ref(R.ref_read_splat, laid)
+    for nm, arr in zip(("pos", "log_scale", "rot", "opacity", "dc"), out.arrays):
+        data[f"laid_splat_out_{nm}"] = arr
+    # hand-laid .spz (SH2, 11 fractional bits, antialiased): every byte value
+    # in every plane and a block of rotation bytes around the identity
+    n, rpc = 512, 8
+    posb = rng.integers(0, 256, size=(n, 9), dtype=np.uint8)
+    posb[:256] = np.arange(256, dtype=np.uint8)[:, None]
+    planes = [rng.integers(0, 256, size=(n, w), dtype=np.uint8) for w in (3, 3, 1, 3, 3 * rpc)]
+    for p in planes:
+        p[:256] = np.arange(256, dtype=np.uint8)[:, None]
+    g = np.stack(np.meshgrid(*[np.arange(125, 131)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    planes[1][256:256 + len(g)] = g
+    raw = np.array([0x5053474e, 2, n], np.uint32).tobytes() + bytes([2, 11, 1, 0]) + posb.tobytes() + \
+        b"".join(p.tobytes() for p in planes)
+    laid = os.path.join(HERE, "laid_sh2.spz")
+    with open(laid, "wb") as f:
+        f.write(gzip.compress(raw, mtime=0))
+    out, d, a = read_ref(R.ref_read_spz, laid, True)
+    assert (d, a) == (2, 1)
+    for nm, arr in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), out.arrays):
+        data[f"laid_spz_out_{nm}"] = arr
+    np.savez_compressed(os.path.join(HERE, "compact.npz"), **data)
+    print("compact: .splat and .spz written and read by the reference")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -232,6 +351,7 @@ def main():
     ref_trainer()
     ref_init_points()
     ref_ply()
+    ref_compact()
 
 
 if __name__ == "__main__":
