@@ -27,24 +27,62 @@ __device__ __forceinline__ float adam_delta(float& m, float& v, float g, float l
 // Groups: 0 pos (3n), 1 log_scale (3n), 2 rot (4n, one quaternion per
 // thread), 3 opacity (n), 4 dc (3n), 5 rest (rs n). Each group's elements are
 // padded to a multiple of 4 in a virtual index space that one flat grid
-// covers with 4 elements per thread, so no workgroup is launched idle.
+// covers with 4 elements per thread, so no workgroup is launched idle. The
+// group of a unit is selected with compares, never by indexing a local array
+// (which would live in scratch memory).
 struct Groups {
-  long long off[6], len[6], vbase[7];
+  long long cap, n, rs;
+  long long vb[6];  // virtual start of groups 1..5, then the end
 };
+
+__device__ __forceinline__ long long padded(long long len) { return (len + 3) & ~3LL; }
 
 __device__ __forceinline__ Groups groups_of(const AdamShape& s) {
   Groups G;
-  const long long cap = s.cap, n = s.n;
-  const long long off[6] = {0, 3 * cap, 6 * cap, 10 * cap, 11 * cap, 14 * cap};
-  const long long len[6] = {3 * n, 3 * n, 4 * n, n, 3 * n, (long long)s.rest_stride * n};
-  G.vbase[0] = 0;
-#pragma unroll
-  for (int g = 0; g < 6; ++g) {
-    G.off[g] = off[g];
-    G.len[g] = len[g];
-    G.vbase[g + 1] = G.vbase[g] + ((len[g] + 3) & ~3LL);
-  }
+  G.cap = s.cap;
+  G.n = s.n;
+  G.rs = s.rest_stride;
+  G.vb[0] = padded(3 * G.n);
+  G.vb[1] = G.vb[0] + padded(3 * G.n);
+  G.vb[2] = G.vb[1] + 4 * G.n;
+  G.vb[3] = G.vb[2] + padded(G.n);
+  G.vb[4] = G.vb[3] + padded(3 * G.n);
+  G.vb[5] = G.vb[4] + padded(G.rs * G.n);
   return G;
+}
+
+struct Unit {
+  int g;             // group
+  long long local;   // element index within the group
+  long long o;       // element index in the parameter buffer
+  long long len;     // elements in the group
+};
+
+__device__ __forceinline__ Unit unit_of(const Groups& G, long long v0) {
+  Unit u;
+  const int g = (v0 >= G.vb[0]) + (v0 >= G.vb[1]) + (v0 >= G.vb[2]) + (v0 >= G.vb[3]) + (v0 >= G.vb[4]);
+  const long long cap = G.cap, n = G.n;
+  long long vbase = 0, off = 0, len = 3 * n;
+  if (g == 1) { vbase = G.vb[0]; off = 3 * cap; }
+  if (g == 2) { vbase = G.vb[1]; off = 6 * cap; len = 4 * n; }
+  if (g == 3) { vbase = G.vb[2]; off = 10 * cap; len = n; }
+  if (g == 4) { vbase = G.vb[3]; off = 11 * cap; }
+  if (g == 5) { vbase = G.vb[4]; off = 14 * cap; len = G.rs * n; }
+  u.g = g;
+  u.local = v0 - vbase;
+  u.o = off + u.local;
+  u.len = len;
+  return u;
+}
+
+__device__ __forceinline__ float lr_of(const AdamStep& P, int g) {
+  float lr = P.lr[0];
+  lr = g == 1 ? P.lr[1] : lr;
+  lr = g == 2 ? P.lr[2] : lr;
+  lr = g == 3 ? P.lr[3] : lr;
+  lr = g == 4 ? P.lr[4] : lr;
+  lr = g == 5 ? P.lr[5] : lr;
+  return lr;
 }
 
 // One 4-element unit of the virtual index space.
@@ -52,15 +90,11 @@ __device__ __forceinline__ void adam_unit(const Groups& G, const AdamStep& P, lo
                                           float* __restrict__ grads, float* __restrict__ m, float* __restrict__ v,
                                           float* __restrict__ sg, int* __restrict__ vc, float* __restrict__ stat_sg,
                                           int* __restrict__ stat_vc) {
-  int g = 0;
-#pragma unroll
-  for (int k = 1; k < 6; ++k)
-    if (v0 >= G.vbase[k]) g = k;
-  const long long local = v0 - G.vbase[g];
-  const float lr = P.lr[g];
-  const long long o = G.off[g] + local;
-  if (g == 2) {
-    const long long i = local / 4;  // gaussian
+  const Unit U = unit_of(G, v0);
+  const float lr = lr_of(P, U.g);
+  const long long o = U.o;
+  if (U.g == 2) {
+    const long long i = U.local / 4;  // gaussian
     float q[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -88,7 +122,7 @@ __device__ __forceinline__ void adam_unit(const Groups& G, const AdamStep& P, lo
     vc[i] = 0;
     return;
   }
-  const long long count = G.len[g] - local;  // elements of this group left from here
+  const long long count = U.len - U.local;  // elements of this group left from here
   if (count >= 4 && (o & 3) == 0) {
     float4 pp = *reinterpret_cast<const float4*>(params + o);
     const float4 gg = *reinterpret_cast<const float4*>(grads + o);
@@ -114,7 +148,7 @@ __device__ __forceinline__ void adam_unit(const Groups& G, const AdamStep& P, lo
   }
 }
 
-// A resident grid striding over the 4-element units.
+// One thread per 4-element unit.
 __global__ __launch_bounds__(256) void k_adam(AdamShape shape, const AdamStep* __restrict__ step,
                                               const uint32_t* __restrict__ skip, float* __restrict__ params,
                                               float* __restrict__ grads, float* __restrict__ m,
@@ -123,10 +157,8 @@ __global__ __launch_bounds__(256) void k_adam(AdamShape shape, const AdamStep* _
   if (skip && *skip) return;
   const AdamStep P = *step;
   const Groups G = groups_of(shape);
-  const long long units = G.vbase[6] / 4;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x; u < units; u += stride)
-    adam_unit(G, P, 4 * u, params, grads, m, v, sg, vc, stat_sg, stat_vc);
+  const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < G.vb[5] / 4) adam_unit(G, P, 4 * u, params, grads, m, v, sg, vc, stat_sg, stat_vc);
 }
 
 }  // namespace
@@ -138,8 +170,7 @@ hipError_t launch_adam(const AdamShape& s, const AdamStep* step, const uint32_t*
   const long long n = s.n;
   const long long virt = ((3 * n + 3) & ~3LL) * 3 + 4 * n + ((n + 3) & ~3LL) + (((long long)s.rest_stride * n + 3) & ~3LL);
   const long long threads = virt / 4;
-  long long blocks = (threads + 255) / 256;
-  if (blocks > 4096) blocks = 4096;  // 16 resident workgroups per CU on 256 CUs, grid-stride beyond
+  const long long blocks = (threads + 255) / 256;
   hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, stream, s, step, skip_flag, params, grads, m, v,
                      sg, vc, stat_sg, stat_vc);
   return hipGetLastError();

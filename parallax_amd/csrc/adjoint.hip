@@ -141,16 +141,39 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
   float sg_sum = 0.f;
   int vc_sum = 0;
 
-  for (int v = 0; v < p.n_views; ++v) {
+  // The next view's record fields and screen gradients are loaded while this
+  // view is processed (both buffers hold every slot of the batch).
+  struct Next {
+    float4 r0, r1, s0, s1, s2;
+    uint32_t flags;
+  };
+  auto fetch = [&](int v) {
+    Next x;
     const long long slot = (long long)v * p.n + g;
-    const SplatRec& r = rec[slot];
+    const float4* rr = reinterpret_cast<const float4*>(rec + slot);
+    x.r0 = rr[0];  // mx, my, ca, cb
+    x.r1 = rr[1];  // cc, opacity, cr, cg
+    x.flags = rec[slot].flags;
+    const float4* src = reinterpret_cast<const float4*>(screen + slot * kScreenStride);
+    x.s0 = src[0];
+    x.s1 = src[1];
+    x.s2 = src[2];
+    return x;
+  };
+  Next next = fetch(0);
+  for (int v = 0; v < p.n_views; ++v) {
+    const Next cur = next;
+    if (v + 1 < p.n_views) next = fetch(v + 1);
+    struct {
+      float ca, cb, cc, opacity;
+      uint32_t flags;
+    } r = {cur.r0.z, cur.r0.w, cur.r1.x, cur.r1.y, cur.flags};
     if (!(r.flags & kProjected)) continue;
     const ViewDesc& V = views[v];
     vc_sum += 1;
     float acc[11];
     {
-      const float4* src = reinterpret_cast<const float4*>(screen + slot * kScreenStride);
-      const float4 a = src[0], b = src[1], c = src[2];
+      const float4 a = cur.s0, b = cur.s1, c = cur.s2;
       acc[0] = a.x;
       acc[1] = a.y;
       acc[2] = a.z;

@@ -127,7 +127,7 @@ splat_context::~splat_context() {
 
 std::vector<DeviceBuffer*> splat_context::all_buffers() {
   return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
-          &vals_alt, &counts, &offsets, &pair_base, &scounts, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
+          &vals_alt, &counts, &offsets, &pair_base, &scounts, &scounts_alt, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
           &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &carry, &scan_ws, &hist_ws, &scalars,
           &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam,
@@ -259,6 +259,7 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
   CK(ctx->offsets.reserve(4 * Mc), "alloc offsets");
   CK(ctx->pair_base.reserve(4 * Mc), "alloc pair base");
   CK(ctx->scounts.reserve(4 * Mc), "alloc counts");
+  CK(ctx->scounts_alt.reserve(4 * Mc), "alloc counts");
   CK(ctx->soffsets.reserve(4 * Mc), "alloc offsets");
   CK(ctx->sranges.reserve(sizeof(uint32_t) * 2 * std::max(1, supers)), "alloc ranges");
   CK(ctx->nseg.reserve(sizeof(uint32_t) * std::max(1, supers)), "alloc segments");
@@ -314,12 +315,22 @@ splat_status enqueue_front(splat_context* ctx) {
                                   o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(), ctx->vals.u(),
                                   ctx->counts.u(), ctx->scounts.u(), ctx->stream));
   bool alt = false;
+  // the super-tile counts ride along with the slots, so their scan reads them
+  // in depth order without a gather
   TIMED("depth_sort", launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
-                                        ctx->sort_ws, &alt, ctx->stream));
+                                        ctx->sort_ws, &alt, ctx->stream, nullptr, ctx->scounts.u(),
+                                        ctx->scounts_alt.u()));
   ctx->sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
-  TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), ctx->sorted_slots, ctx->offsets.u(), M, d_total,
-                                           ctx->sort_ws.scan, ctx->stream));
-  TIMED("pair_scan", launch_exclusive_scan(ctx->scounts.u(), ctx->sorted_slots, ctx->soffsets.u(), M, d_total + 1,
+  const uint32_t* sorted_scounts = alt ? ctx->scounts_alt.u() : ctx->scounts.u();
+  // The tile-pair offsets in depth order are the deterministic adjoint's pair
+  // ids; otherwise only their total is needed.
+  if (o->deterministic) {
+    TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), ctx->sorted_slots, ctx->offsets.u(), M, d_total,
+                                             ctx->sort_ws.scan, ctx->stream));
+  } else {
+    TIMED("pair_scan", launch_sum_u32(ctx->counts.u(), M, d_total, ctx->stream));
+  }
+  TIMED("pair_scan", launch_exclusive_scan(sorted_scounts, nullptr, ctx->soffsets.u(), M, d_total + 1,
                                            ctx->sort_ws.scan, ctx->stream));
   return SPLAT_OK;
 }

@@ -10,9 +10,11 @@
 // Two passes over 64x16-pixel loss tiles of every view of the batch, one
 // 256-thread workgroup each, one channel at a time:
 //   stats: the image/target window (74x26, zero outside the image) goes to
-//          LDS; per statistics plane (x, y, xx, yy, xy) a row pass over the
-//          26 rows and a column pass down to the tile's 16 rows, every thread
-//          sliding an 11-tap window over a run of 4 outputs in registers; then
+//          LDS; one row pass over the 26 rows blurs all five statistics
+//          planes (x, y, xx, yy, xy) from a single read of each run's window,
+//          then a column pass per plane down to the tile's 16 rows, every
+//          thread sliding an 11-tap window over a run of 4 outputs in
+//          registers; then
 //          per pixel the SSIM value, the five partial planes (to HBM), and the
 //          L1 terms; per-tile L1 and SSIM sums.
 //   grad:  per partial plane, its 74x26 window to LDS, the same two passes,
@@ -36,7 +38,6 @@ constexpr int kRun = 4;                   // outputs per thread along a blur
 constexpr int kTaps = kRun + 2 * kR;      // 14 window values per run
 constexpr int kPitch = kWinW + 1;         // 75: odd, conflict-free row walks
 constexpr int kHPitch = kLW + 1;          // 65
-constexpr int kRowTasks = kWinH * (kLW / kRun);  // 416 row-pass runs
 constexpr int kWinPerThread = (kWinH * kWinW + 255) / 256;  // window values each of 256 threads fetches
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
@@ -82,8 +83,10 @@ __device__ __forceinline__ LossTile loss_tile_of(const LossParams& p, const View
 // blurs a, 1 blurs b, 2 a*a, 3 b*b, 4 a*b (a, b: LDS windows).
 __device__ __forceinline__ void row_pass(int plane, const float (*a)[kPitch], const float (*b)[kPitch],
                                          const float* kernel, float (*hbuf)[kHPitch]) {
-  for (int task = threadIdx.x; task < kRowTasks; task += blockDim.x) {
-    const int r = task % kWinH, j0 = (task / kWinH) * kRun;
+  // a wave covers 4 window rows x 16 runs: with both pitches odd mod 4 the
+  // 64 lanes of every LDS access land in 64 different banks
+  for (int r = (int)(threadIdx.x >> 4); r < kWinH; r += (int)(blockDim.x >> 4)) {
+    const int j0 = (int)(threadIdx.x & 15) * kRun;
     float win[kTaps], o[kRun];
 #pragma unroll
     for (int i = 0; i < kTaps; ++i) {
@@ -98,6 +101,30 @@ __device__ __forceinline__ void row_pass(int plane, const float (*a)[kPitch], co
     blur_run(win, kernel, o);
 #pragma unroll
     for (int k = 0; k < kRun; ++k) hbuf[r][j0 + k] = o[k];
+  }
+}
+
+// The row passes of all five statistics planes at once: each run's window of
+// a and b is read from LDS once.
+__device__ __forceinline__ void row_pass5(const float (*a)[kPitch], const float (*b)[kPitch], const float* kernel,
+                                          float (*hbuf)[kWinH][kHPitch]) {
+  for (int r = (int)(threadIdx.x >> 4); r < kWinH; r += (int)(blockDim.x >> 4)) {
+    const int j0 = (int)(threadIdx.x & 15) * kRun;
+    float x[kTaps], y[kTaps], win[kTaps], o[kRun];
+#pragma unroll
+    for (int i = 0; i < kTaps; ++i) {
+      x[i] = a[r][j0 + i];
+      y[i] = b[r][j0 + i];
+    }
+#pragma unroll
+    for (int pl = 0; pl < 5; ++pl) {
+#pragma unroll
+      for (int i = 0; i < kTaps; ++i)
+        win[i] = pl == 0 ? x[i] : (pl == 1 ? y[i] : (pl == 2 ? x[i] * x[i] : (pl == 3 ? y[i] * y[i] : x[i] * y[i])));
+      blur_run(win, kernel, o);
+#pragma unroll
+      for (int k = 0; k < kRun; ++k) hbuf[pl][r][j0 + k] = o[k];
+    }
   }
 }
 
@@ -117,7 +144,7 @@ __global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc
                                                     float* __restrict__ block_sums) {
   __shared__ float s_x[kWinH][kPitch];
   __shared__ float s_y[kWinH][kPitch];
-  __shared__ float s_h[kWinH][kHPitch];
+  __shared__ float s_h[5][kWinH][kHPitch];
   __shared__ float s_red[2][4];
   const LossTile L = loss_tile_of(p, views);
   const ViewDesc& V = *L.V;
@@ -161,13 +188,10 @@ __global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc
     if (c + 1 < 3) fetch(c + 1);
     float st[5][kRun];
     if (w > 0.f) {
+      row_pass5(s_x, s_y, p.kernel, s_h);
+      __syncthreads();
 #pragma unroll
-      for (int pl = 0; pl < 5; ++pl) {
-        row_pass(pl, s_x, s_y, p.kernel, s_h);
-        __syncthreads();
-        col_pass(s_h, p.kernel, j, r0, st[pl]);
-        __syncthreads();
-      }
+      for (int pl = 0; pl < 5; ++pl) col_pass(s_h[pl], p.kernel, j, r0, st[pl]);
     }
 #pragma unroll
     for (int k = 0; k < kRun; ++k) {

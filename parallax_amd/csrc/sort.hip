@@ -12,7 +12,9 @@
 // table, and a stable scatter whose in-block ranks come from wave64 ballots
 // (one ballot per digit bit selects the lanes sharing a digit, and a popcount
 // below the lane gives the rank). The scatter sorts each block by digit in
-// LDS first, so its global stores go out as contiguous per-digit runs.
+// LDS first, so its global stores go out as contiguous per-digit runs. An
+// optional second value array moves with the first (the depth sort carries
+// each slot's super-tile count, so its scan reads them in order).
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -146,10 +148,14 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
 constexpr int kRsWaveItems = kScanTile / 4;  // 1024 items per wave
 constexpr int kRsRounds = kRsWaveItems / 64;  // 16
 
+// With kV2, a second value array rides along with the first.
+template <bool kV2>
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ vals,
+                                                       const uint32_t* __restrict__ vals2,
                                                        uint32_t* __restrict__ keys_out,
-                                                       uint32_t* __restrict__ vals_out, long long n_cap,
+                                                       uint32_t* __restrict__ vals_out,
+                                                       uint32_t* __restrict__ vals2_out, long long n_cap,
                                                        const uint32_t* __restrict__ n_dev, int shift, int bits,
                                                        const uint32_t* __restrict__ hist, int nblocks) {
   const long long n = item_count(n_cap, n_dev);
@@ -159,6 +165,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   __shared__ uint32_t s_global[256];   // global start of each digit for this block
   __shared__ uint32_t s_keys[kScanTile];
   __shared__ uint32_t s_vals[kScanTile];
+  __shared__ uint32_t s_vals2[kV2 ? kScanTile : 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t mask = (1u << bits) - 1u;
   const long long base = (long long)blockIdx.x * kScanTile;
@@ -168,13 +175,14 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   s_global[tid] = hist[(long long)tid * nblocks + blockIdx.x];
   __syncthreads();
   const uint64_t below = __lanemask_lt();
-  uint32_t k[kRsRounds], v[kRsRounds], rank[kRsRounds];
+  uint32_t k[kRsRounds], v[kRsRounds], v2[kV2 ? kRsRounds : 1], rank[kRsRounds];
 #pragma unroll
   for (int r = 0; r < kRsRounds; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
     const bool valid = j < n;
     k[r] = valid ? keys[j] : 0xFFFFFFFFu;
     v[r] = valid ? vals[j] : 0u;
+    if constexpr (kV2) v2[r] = valid ? vals2[j] : 0u;
     const uint32_t d = (k[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
@@ -219,6 +227,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     const uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
     s_keys[pos] = k[r];
     s_vals[pos] = v[r];
+    if constexpr (kV2) s_vals2[pos] = v2[r];
   }
   __syncthreads();
   const long long remaining = n - base;
@@ -229,7 +238,19 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     const uint32_t dst = s_global[d] + (uint32_t)p - s_start[d];
     keys_out[dst] = kk;
     vals_out[dst] = s_vals[p];
+    if constexpr (kV2) vals2_out[dst] = s_vals2[p];
   }
+}
+
+// *out += the sum of in[0..n) (integer atomics: the total is exact whatever
+// the order).
+__global__ __launch_bounds__(256) void k_sum_u32(const uint32_t* __restrict__ in, long long n, uint32_t* out) {
+  uint32_t sum = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    sum += in[i];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) sum += (uint32_t)__shfl_xor((int)sum, d, 64);
+  if ((threadIdx.x & 63) == 0 && sum) atomicAdd(out, sum);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
@@ -271,12 +292,12 @@ hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx,
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
-                             hipStream_t stream, const uint32_t* n_dev) {
+                             hipStream_t stream, const uint32_t* n_dev, uint32_t* vals2, uint32_t* vals2_alt) {
   *result_in_alt = false;
   if (end_bit <= begin_bit || n <= (n_dev ? 0 : 1)) return hipSuccess;
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (256 * nb > ws.hist_capacity) return hipErrorInvalidValue;
-  uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+  uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt, *wi = vals2, *wo = vals2_alt;
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     const int bits = (end_bit - shift) < 8 ? (end_bit - shift) : 8;
     const uint32_t mask = (1u << bits) - 1u;
@@ -284,14 +305,25 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                        (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     SPLAT_HIP_CHECK(launch_exclusive_scan(ws.hist, nullptr, ws.hist, 256 * nb, nullptr, ws.scan, stream));
-    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, ko, vo, n, n_dev, shift,
-                       bits, ws.hist, (int)nb);
+    if (vals2)
+      hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, wi, ko, vo, wo, n,
+                         n_dev, shift, bits, ws.hist, (int)nb);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, nullptr, ko, vo,
+                         nullptr, n, n_dev, shift, bits, ws.hist, (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     uint32_t* t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;
+    t = wi; wi = wo; wo = t;
     *result_in_alt = !*result_in_alt;
   }
   return hipSuccess;
+}
+
+hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, stream, in, n, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* ptr, uint32_t value, long long n, hipStream_t stream) {

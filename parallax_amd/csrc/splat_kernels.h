@@ -44,9 +44,14 @@ long long sort_hist_elems(long long max_n);
 // pair holds the sorted output. With n_dev, the item count is read on the
 // device (clamped to n, which then only sizes the launches), so the sort can
 // sit in a captured graph whose item count changes from replay to replay.
+// With vals2 (and vals2_alt), a second value array is permuted alongside and
+// ends in the same buffer of the pair as the first.
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
-                             hipStream_t stream, const uint32_t* n_dev = nullptr);
+                             hipStream_t stream, const uint32_t* n_dev = nullptr, uint32_t* vals2 = nullptr,
+                             uint32_t* vals2_alt = nullptr);
+// *out (device) += the sum of in[0..n).
+hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream);
 
 // ---- binning (binning.hip) ----
 constexpr int kBinSegment = 1024;  // splats per segment of a super-tile list
