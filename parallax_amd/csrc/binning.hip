@@ -14,14 +14,14 @@
 //   1. emit:    for every depth-sorted slot, one (super-tile id, slot) pair per
 //               super-tile its rect touches (stable radix sort by id next);
 //   2. segment: each super-tile's list is cut into segments of 1024 splats;
-//   3. count:   one wave per segment builds, per splat, the 64-bit mask of the
-//               super-tile's tiles it covers (a row mask times a column mask,
-//               from the rect bits the emitter packed into the key) and counts,
-//               per tile, the segment's splats (ballot + popcount);
+//   3. count:   one wave per segment; per chunk of 64 splats, eight row and
+//               eight column ballots (from the rect bits the emitter packed
+//               into the key) give each tile's covering lanes as one AND, and
+//               lane t counts tile t's with a popcount;
 //   4. scan:    an exclusive scan over the (super-tile, tile, segment) table
 //               gives every (tile, segment) run its place, and the tile ranges;
 //   5. write:   the same wave pass again, writing each splat into each covered
-//               tile's run at its rank among the lanes (ballot + mbcnt), in
+//               tile's run at its rank among the covering lanes (mbcnt), in
 //               depth order.
 // Every tile's list thus lands contiguously and in the order of the super-tile
 // list, which is depth order; pair memory is 4 bytes per (tile, splat).
@@ -161,25 +161,36 @@ __device__ __forceinline__ Segment segment_of(uint32_t g, const uint32_t* srange
   return S;
 }
 
-// Bit (ly * 8 + lx) set for every tile (lx, ly) of the super-tile that the
-// splat's rect covers, from the rect bits of its super-tile key.
-__device__ __forceinline__ uint64_t cover_mask(uint32_t key) {
+// The coverage ballots of one chunk of 64 list entries: bit j of rows[k] is
+// set when lane j's rect (from the rect bits of its super-tile key) spans
+// tile row k of the super-tile, bit j of cols[k] when it spans tile column k.
+// Tile (lx, ly) is covered by exactly the lanes of rows[ly] & cols[lx].
+struct ChunkCover {
+  uint64_t rows[kSuper], cols[kSuper];
+};
+
+__device__ __forceinline__ ChunkCover chunk_cover(bool valid, uint32_t key) {
   const int lx0 = (key >> kStBits) & 7, lx1 = (key >> (kStBits + 3)) & 7;
   const int ly0 = (key >> (kStBits + 6)) & 7, ly1 = (key >> (kStBits + 9)) & 7;
-  const uint64_t all = 0x0101010101010101ull;
-  const uint64_t rows = (all << (8 * ly0)) & (all >> (8 * (kSuper - 1 - ly1)));
-  const uint64_t cols = (uint64_t)(((2u << lx1) - 1u) & ~((1u << lx0) - 1u));
-  return rows * cols;
+  ChunkCover c;
+#pragma unroll
+  for (int k = 0; k < kSuper; ++k) {
+    c.rows[k] = __ballot(valid && ly0 <= k && k <= ly1);
+    c.cols[k] = __ballot(valid && lx0 <= k && k <= lx1);
+  }
+  return c;
 }
 
-__device__ __forceinline__ uint64_t wave_or(uint64_t m) {
-  uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+// Lane t: how many lanes of the chunk cover tile t = ly * 8 + lx.
+__device__ __forceinline__ uint32_t lane_tile_count(const ChunkCover& c, int lane) {
+  const int ly = lane >> 3, lx = lane & 7;
+  uint64_t r = 0ull, q = 0ull;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    lo |= (uint32_t)__shfl_xor((int)lo, d, 64);
-    hi |= (uint32_t)__shfl_xor((int)hi, d, 64);
+  for (int k = 0; k < kSuper; ++k) {
+    r = ly == k ? c.rows[k] : r;
+    q = lx == k ? c.cols[k] : q;
   }
-  return ((uint64_t)hi << 32) | lo;
+  return (uint32_t)__popcll(r & q);
 }
 
 // Per (tile, segment) splat counts into the table at
@@ -204,21 +215,17 @@ __global__ __launch_bounds__(256) void k_tile_count(const uint32_t* __restrict__
     uint32_t cnt = 0;  // lane t: splats of this segment covering tile t
     for (uint32_t c = S.b0; c < S.b1; c += 64) {
       const uint32_t i = c + (uint32_t)lane;
-      const uint64_t mask = i < S.b1 ? cover_mask(skeys[i]) : 0ull;
-      uint64_t u = wave_or(mask);
-      while (u) {
-        const int t = __builtin_ctzll(u);
-        u &= u - 1;
-        const uint64_t b = __ballot((mask >> t) & 1ull);
-        if (lane == t) cnt += (uint32_t)__popcll(b);
-      }
+      const bool valid = i < S.b1;
+      cnt += lane_tile_count(chunk_cover(valid, valid ? skeys[i] : 0u), lane);
     }
     table[S.table0 + (uint64_t)lane * S.nsg + S.s] = cnt;
   }
 }
 
 // Writes every splat of every segment into each covered tile's run. Same
-// traversal as k_tile_count; `scan` is the exclusive scan of its table.
+// traversal as k_tile_count; `scan` is the exclusive scan of its table. Per
+// chunk and covered tile, the covering lanes store at the tile's next slot
+// plus their rank among those lanes (mbcnt), so each run is in list order.
 __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ svals,
                                                     const uint32_t* __restrict__ sranges,
@@ -236,24 +243,19 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
     uint32_t base = scan[S.table0 + (uint64_t)lane * S.nsg + S.s];  // lane t: next slot of tile t
     for (uint32_t c = S.b0; c < S.b1; c += 64) {
       const uint32_t i = c + (uint32_t)lane;
-      uint32_t slot = 0;
-      uint64_t mask = 0ull;
-      if (i < S.b1) {
-        slot = svals[i];
-        mask = cover_mask(skeys[i]);
-      }
-      uint64_t u = wave_or(mask);
-      while (u) {
-        const int t = __builtin_ctzll(u);
-        u &= u - 1;
-        const bool mine = (mask >> t) & 1ull;
-        const uint64_t b = __ballot(mine);
+      const bool valid = i < S.b1;
+      const uint32_t slot = valid ? svals[i] : 0u;
+      const ChunkCover cv = chunk_cover(valid, valid ? skeys[i] : 0u);
+#pragma unroll
+      for (int t = 0; t < 64; ++t) {
+        const uint64_t b = cv.rows[t >> 3] & cv.cols[t & 7];
+        if (b == 0ull) continue;  // wave-uniform
         const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
         const uint32_t to =
             at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-        if (mine && to < p_cap) pair_vals[to] = slot;  // past p_cap: the step is flagged
-        if (lane == t) base += (uint32_t)__popcll(b);
+        if (((b >> lane) & 1ull) && to < p_cap) pair_vals[to] = slot;  // past p_cap: the step is flagged
       }
+      base += lane_tile_count(cv, lane);
     }
   }
 }
