@@ -165,7 +165,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   __shared__ uint32_t s_global[256];   // global start of each digit for this block
   __shared__ uint32_t s_keys[kScanTile];
   __shared__ uint32_t s_vals[kScanTile];
-  __shared__ uint32_t s_vals2[kV2 ? kScanTile : 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t mask = (1u << bits) - 1u;
   const long long base = (long long)blockIdx.x * kScanTile;
@@ -227,7 +226,8 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     const uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
     s_keys[pos] = k[r];
     s_vals[pos] = v[r];
-    if constexpr (kV2) s_vals2[pos] = v2[r];
+    // the second value goes straight to its final place (no LDS staging)
+    if constexpr (kV2) vals2_out[s_global[d] + (pos - s_start[d])] = v2[r];
   }
   __syncthreads();
   const long long remaining = n - base;
@@ -238,7 +238,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     const uint32_t dst = s_global[d] + (uint32_t)p - s_start[d];
     keys_out[dst] = kk;
     vals_out[dst] = s_vals[p];
-    if constexpr (kV2) vals2_out[dst] = s_vals2[p];
   }
 }
 
