@@ -241,15 +241,21 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   }
 }
 
-// *out += the sum of in[0..n) (integer atomics: the total is exact whatever
-// the order).
+// *out += the sum of in[0..n): per workgroup a register and LDS reduction,
+// then one integer atomic (exact whatever the order).
 __global__ __launch_bounds__(256) void k_sum_u32(const uint32_t* __restrict__ in, long long n, uint32_t* out) {
+  __shared__ uint32_t s_part[4];
   uint32_t sum = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     sum += in[i];
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) sum += (uint32_t)__shfl_xor((int)sum, d, 64);
-  if ((threadIdx.x & 63) == 0 && sum) atomicAdd(out, sum);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t total = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    if (total) atomicAdd(out, total);
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
@@ -321,7 +327,7 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
 
 hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, stream, in, n, out);
+  hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, stream, in, n, out);
   return hipGetLastError();
 }
 
