@@ -46,12 +46,15 @@ __device__ __forceinline__ int find_view_st(const ViewDesc* views, int n_views, 
 // in its low kStBits bits and the splat's rect inside that super-tile (three
 // bits per bound) above them, where the sort by id carries it along. With
 // pair_base, also records each slot's first tile-pair index, the
-// deterministic adjoint's pair id base.
+// deterministic adjoint's pair id base. The rects are read from the compact
+// 8-byte-a-slot copy the projection writes: in depth order these reads are
+// random, and the compact array stays in the last-level cache where the
+// 64-byte records would not.
 __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__ sorted_slots,
                                                     const uint32_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ soffsets,
                                                     const uint32_t* __restrict__ s_total,
-                                                    const SplatRec* __restrict__ rec,
+                                                    const uint2* __restrict__ rects,
                                                     const ViewDesc* __restrict__ views, long long per_view,
                                                     long long m, uint32_t s_cap, uint32_t* __restrict__ skeys,
                                                     uint32_t* __restrict__ svals, uint32_t* __restrict__ pair_base) {
@@ -64,7 +67,7 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
     off = soffsets[s];
     c = (s + 1 < m ? soffsets[s + 1] : *s_total) - off;
     if (c) {
-      const int2 packed = *reinterpret_cast<const int2*>(&rec[slot].x0);  // x0, x1, y0, y1 as int16
+      const uint2 packed = rects[slot];  // x0, x1, y0, y1 as int16
       x0 = (int)(int16_t)(packed.x & 0xFFFF);
       x1 = (int)(int16_t)(packed.x >> 16);
       y0 = (int)(int16_t)(packed.y & 0xFFFF);
@@ -295,12 +298,12 @@ hipError_t launch_check_caps(uint32_t* totals, uint32_t p_cap, uint32_t s_cap, h
 }
 
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             const uint32_t* s_total, const uint2* rects, const ViewDesc* views, long long per_view,
                              long long m, uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
                              hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
-                     soffsets, s_total, rec, views, per_view, m, s_cap, skeys, svals, pair_base);
+                     soffsets, s_total, rects, views, per_view, m, s_cap, skeys, svals, pair_base);
   return hipGetLastError();
 }
 

@@ -127,7 +127,7 @@ splat_context::~splat_context() {
 
 std::vector<DeviceBuffer*> splat_context::all_buffers() {
   return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
-          &vals_alt, &counts, &offsets, &pair_base, &scounts, &scounts_alt, &soffsets, &skeys, &svals, &skeys_alt, &svals_alt,
+          &vals_alt, &counts, &offsets, &pair_base, &scounts, &scounts_alt, &soffsets, &rects, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
           &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &carry, &scan_ws, &hist_ws, &scalars,
           &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam,
@@ -260,6 +260,7 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
   CK(ctx->pair_base.reserve(4 * Mc), "alloc pair base");
   CK(ctx->scounts.reserve(4 * Mc), "alloc counts");
   CK(ctx->scounts_alt.reserve(4 * Mc), "alloc counts");
+  CK(ctx->rects.reserve(8 * Mc), "alloc rects");
   CK(ctx->soffsets.reserve(4 * Mc), "alloc offsets");
   CK(ctx->sranges.reserve(sizeof(uint32_t) * 2 * std::max(1, supers)), "alloc ranges");
   CK(ctx->nseg.reserve(sizeof(uint32_t) * std::max(1, supers)), "alloc segments");
@@ -313,7 +314,8 @@ splat_status enqueue_front(splat_context* ctx) {
   CK(launch_fill_u32(d_total, 0u, 4, ctx->stream), "clear counters");
   TIMED("project", launch_project(ctx->params.f(), ctx->cap, (int)n, ctx->rs, ctx->sh_use, ctx->views_dev(), n_views,
                                   o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(), ctx->vals.u(),
-                                  ctx->counts.u(), ctx->scounts.u(), ctx->stream));
+                                  ctx->counts.u(), ctx->scounts.u(), reinterpret_cast<uint2*>(ctx->rects.u()),
+                                  ctx->stream));
   bool alt = false;
   // the super-tile counts ride along with the slots, so their scan reads them
   // in depth order without a gather
@@ -348,7 +350,8 @@ splat_status enqueue_back(splat_context* ctx) {
   uint32_t* d_total = ctx->scalars.u();
   CK(launch_check_caps(d_total, p_cap, s_cap, ctx->stream), "capacity check");
   TIMED("emit_super", launch_emit_super(ctx->sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), d_total + 1,
-                                        ctx->recs(), ctx->views_dev(), n, M, s_cap, ctx->skeys.u(), ctx->svals.u(),
+                                        reinterpret_cast<const uint2*>(ctx->rects.u()), ctx->views_dev(), n, M, s_cap,
+                                        ctx->skeys.u(), ctx->svals.u(),
                                         ctx->opts.deterministic ? ctx->pair_base.u() : nullptr, ctx->stream));
   const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
   bool salt = false;

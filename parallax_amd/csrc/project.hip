@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
                                                  int n_views, float dilation, float min_alpha,
                                                  SplatRec* __restrict__ rec, uint32_t* __restrict__ keys,
                                                  uint32_t* __restrict__ vals, uint32_t* __restrict__ counts,
-                                                 uint32_t* __restrict__ scounts) {
+                                                 uint32_t* __restrict__ scounts, uint2* __restrict__ rects) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const float* P = params;
@@ -250,6 +250,8 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
     vals[slot] = (uint32_t)slot;
     counts[slot] = count;
     scounts[slot] = scount;
+    rects[slot] = make_uint2((uint32_t)(uint16_t)o.x0 | (uint32_t)(uint16_t)o.x1 << 16,
+                             (uint32_t)(uint16_t)o.y0 | (uint32_t)(uint16_t)o.y1 << 16);
   }
 }
 
@@ -257,7 +259,8 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
 
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
-                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, hipStream_t stream) {
+                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, uint2* rects,
+                          hipStream_t stream) {
   if (n <= 0 || n_views <= 0) return hipSuccess;
   const int blocks = (n + 255) / 256;
   // the cloud's coefficients must cover the degree in use
@@ -265,19 +268,19 @@ hipError_t launch_project(const float* params, long long cap, int n, int rest_st
   switch (sh_use) {
     case 0:
       hipLaunchKernelGGL(k_project<0>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
       break;
     case 1:
       hipLaunchKernelGGL(k_project<1>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
       break;
     case 2:
       hipLaunchKernelGGL(k_project<2>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
       break;
     default:
       hipLaunchKernelGGL(k_project<3>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts);
+                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
       break;
   }
   return hipGetLastError();

@@ -73,6 +73,7 @@ struct splat_context {
 
   DeviceBuffer rec, keys, vals, keys_alt, vals_alt, counts, offsets, pair_base;
   DeviceBuffer scounts, scounts_alt, soffsets;      // super-tile pairs per slot (sorted along), their scan
+  DeviceBuffer rects;                                // each slot's tile rect, 8 bytes
   DeviceBuffer skeys, svals, skeys_alt, svals_alt;  // (super-tile, slot) pairs
   DeviceBuffer sranges, nseg, seg_base, seg_st, bin_table;
   DeviceBuffer pair_vals, ranges;                   // per-tile lists (slots), [begin, end) per tile

@@ -9,10 +9,13 @@
 namespace splat {
 
 // ---- projection ----
-// counts[slot] = tiles of the slot's rect, scounts[slot] = super-tiles of it.
+// counts[slot] = tiles of the slot's rect, scounts[slot] = super-tiles of it;
+// rects[slot] = the record's tile rect again ((x0 | x1 << 16, y0 | y1 << 16),
+// 8 bytes a slot) for the binning's gathers in depth order.
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
-                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, hipStream_t stream);
+                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, uint2* rects,
+                          hipStream_t stream);
 
 // ---- scan / sort ----
 constexpr int kScanThreads = 256;
@@ -64,7 +67,7 @@ constexpr uint32_t kStMask = (1u << kStBits) - 1u;
 // that super-tile above them. With pair_base: pair_base[slot] = offsets[s].
 // Pairs at or past s_cap are dropped (the capacity check flags the step).
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* s_total, const SplatRec* rec, const ViewDesc* views, long long per_view,
+                             const uint32_t* s_total, const uint2* rects, const ViewDesc* views, long long per_view,
                              long long m, uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
                              hipStream_t stream);
 // sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted
