@@ -140,8 +140,9 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
 
 // Stable scatter of one 4096-item block for one digit. Each wave ranks its
 // own contiguous 1024 items with no block barriers: per round of 64, the
-// lanes sharing a digit are found with one ballot per digit bit, and a
-// wave-private LDS counter per digit carries the count across rounds. Then
+// lanes sharing a digit are found with one ballot per digit bit, and the
+// group's lowest lane adds the group's size to a wave-private LDS counter per
+// digit with a returning atomic, whose old value is the group's base. Then
 // per-digit wave offsets and block digit starts place every item at its
 // stable position in an LDS copy of the block, sorted by digit; the block is
 // written out from LDS so each digit's run goes to global memory contiguously.
@@ -175,6 +176,8 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   __syncthreads();
   const uint64_t below = __lanemask_lt();
   uint32_t k[kRsRounds], v[kRsRounds], v2[kV2 ? kRsRounds : 1], rank[kRsRounds];
+  int leader[kRsRounds];
+  uint32_t old[kRsRounds];
 #pragma unroll
   for (int r = 0; r < kRsRounds; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
@@ -189,10 +192,19 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
       const uint64_t bb = __ballot(valid && bit);
       peers &= bit ? bb : ~bb;
     }
-    const uint32_t before = s_cnt[wave][d];
-    rank[r] = before + (uint32_t)__popcll(peers & below);
-    if (valid && (peers & below) == 0ull) s_cnt[wave][d] = before + (uint32_t)__popcll(peers);
-    if (!valid) rank[r] = 0xFFFFFFFFu;
+    // The group's lowest lane adds the group's size to the wave's digit
+    // counter; the counter's old value is the group's base. The rounds'
+    // atomics go out back to back (LDS applies them in order) and the bases
+    // are fetched after the last one.
+    rank[r] = valid ? (uint32_t)__popcll(peers & below) : 0xFFFFFFFFu;
+    leader[r] = valid ? __builtin_ctzll(peers) : lane;
+    old[r] = 0u;
+    if (valid && (peers & below) == 0ull) old[r] = atomicAdd(&s_cnt[wave][d], (uint32_t)__popcll(peers));
+  }
+#pragma unroll
+  for (int r = 0; r < kRsRounds; ++r) {
+    const uint32_t base = (uint32_t)__shfl((int)old[r], leader[r], 64);
+    if (rank[r] != 0xFFFFFFFFu) rank[r] += base;
   }
   __syncthreads();
   // per digit: offsets of each wave inside the digit, and the digit's total
