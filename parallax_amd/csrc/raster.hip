@@ -551,33 +551,39 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             // every lane computes, contributing lanes keep: the engine's
             // expressions with exact zeros and unchanged state elsewhere
 #pragma clang fp contract(fast)
+            // (algebraically the engine's expressions, SplatRaster.cpp:403-428,
+            // arranged for fused multiply-adds: alpha_c is alpha where this
+            // pixel contributes and 0 elsewhere, so a dropped step changes
+            // nothing; the suffix colour alpha c + (1 - alpha) suf is
+            // suf + alpha_c (c - suf), reusing dL/dalpha's differences)
             const bool shape = contrib && raw < kMaxAlpha;
+            const float alpha_c = contrib ? alpha : 0.f;
             const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
             const float Tn = T[k] * inv;
-            const float weight = contrib ? alpha * Tn : 0.f;
-            const float dL_dalpha = ((b.z - suf0[k]) * dL0[k] + (b.w - suf1[k]) * dL1[k] + (cblue - suf2[k]) * dL2[k]) *
-                                        Tn - tb[k] * inv;
+            const float weight = alpha_c * Tn;
+            const float d0 = b.z - suf0[k], d1 = b.w - suf1[k], d2 = cblue - suf2[k];
+            const float dL_dalpha = (d0 * dL0[k] + d1 * dL1[k] + d2 * dL2[k]) * Tn - tb[k] * inv;
             const float dL_dweight = shape ? b.y * dL_dalpha : 0.f;
             const float Gs = shape ? G : 0.f;
-            const float gx = G * dx, gy = G * dy;
-            const float mx = dL_dweight * (-gx * a.z - gy * a.w);
-            const float my = dL_dweight * (-gy * b.x - gx * a.w);
+            const float pw = dL_dweight * G;  // dL/dpower
+            const float px = pw * dx, py = pw * dy;
+            const float u = px * a.z + py * a.w;  // -dL/dmean_x
+            const float w = py * b.x + px * a.w;  // -dL/dmean_y
             acc[0] += weight * dL0[k];
             acc[1] += weight * dL1[k];
             acc[2] += weight * dL2[k];
             acc[3] += Gs * dL_dalpha;
-            acc[4] -= mx;
-            acc[5] -= my;
-            acc[6] += __builtin_amdgcn_sqrtf(mx * mx + my * my);  // densify statistic: hardware sqrt
-            acc[7] -= 0.5f * dL_dweight * gx * dx;
-            acc[8] -= dL_dweight * gx * dy;
-            acc[9] -= 0.5f * dL_dweight * gy * dy;
+            acc[4] += u;
+            acc[5] += w;
+            acc[6] += __builtin_amdgcn_sqrtf(u * u + w * w);  // densify statistic: hardware sqrt
+            acc[7] -= 0.5f * px * dx;
+            acc[8] -= px * dy;
+            acc[9] -= 0.5f * py * dy;
             acc[10] += shape ? 1.f : 0.f;
             T[k] = contrib ? Tn : T[k];
-            const float keep = 1.f - alpha;
-            suf0[k] = contrib ? alpha * b.z + keep * suf0[k] : suf0[k];
-            suf1[k] = contrib ? alpha * b.w + keep * suf1[k] : suf1[k];
-            suf2[k] = contrib ? alpha * cblue + keep * suf2[k] : suf2[k];
+            suf0[k] += alpha_c * d0;
+            suf1[k] += alpha_c * d1;
+            suf2[k] += alpha_c * d2;
             }
           }
           float z = 0.f;
