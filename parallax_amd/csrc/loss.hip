@@ -79,6 +79,30 @@ __device__ __forceinline__ LossTile loss_tile_of(const LossParams& p, const View
   return t;
 }
 
+// Where this thread's window elements (tid + 256 r, row-major over the
+// 74 x 26 window) come from: the pixel offset within the view (0 where the
+// element lies outside the image or past the window), and a bit per element
+// that is set when it lies inside both.
+struct WindowMap {
+  int off[kWinPerThread];
+  uint32_t inside;
+};
+
+__device__ __forceinline__ WindowMap window_map(const LossTile& L, const ViewDesc& V) {
+  WindowMap m;
+  m.inside = 0u;
+#pragma unroll
+  for (int r = 0; r < kWinPerThread; ++r) {
+    const int idx = (int)threadIdx.x + r * 256;
+    const int hy = idx / kWinW, hx = idx % kWinW;
+    const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
+    const bool in = idx < kWinH * kWinW && qx >= 0 && qx < V.width && qy >= 0 && qy < V.height;
+    m.off[r] = in ? qy * V.width + qx : 0;
+    m.inside |= in ? 1u << r : 0u;
+  }
+  return m;
+}
+
 // Row pass of one plane over the window's 26 rows into hbuf[26][65]: plane 0
 // blurs a, 1 blurs b, 2 a*a, 3 b*b, 4 a*b (a, b: LDS windows).
 __device__ __forceinline__ void row_pass(int plane, const float (*a)[kPitch], const float (*b)[kPitch],
@@ -159,19 +183,15 @@ __global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc
   float l1 = 0.f, ssim = 0.f;
   // each channel's image/target window, prefetched into registers while the
   // previous channel is blurred
+  const WindowMap win = window_map(L, V);
   float px_[kWinPerThread], py_[kWinPerThread];
   auto fetch = [&](int c) {
 #pragma unroll
     for (int r = 0; r < kWinPerThread; ++r) {
-      const int idx = tid + r * 256;
-      const int hy = idx / kWinW, hx = idx % kWinW;
-      const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
-      px_[r] = py_[r] = 0.f;
-      if (idx < kWinH * kWinW && qx >= 0 && qx < V.width && qy >= 0 && qy < V.height) {
-        const long long q = (long long)qy * V.width + qx;
-        px_[r] = img[3 * q + c];
-        py_[r] = tgt[3 * q + c];
-      }
+      const float x = img[3 * win.off[r] + c], y = tgt[3 * win.off[r] + c];
+      const bool in = (win.inside >> r) & 1u;
+      px_[r] = in ? x : 0.f;
+      py_[r] = in ? y : 0.f;
     }
   };
   fetch(0);
@@ -263,18 +283,17 @@ __global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc*
   const float* tgt = targets + V.target_base;
   const int gx = L.ox + j;
   // the 15 partial planes' windows, each prefetched into registers while the
-  // previous one is blurred
+  // previous one is blurred; the window's element offsets (0 outside the
+  // image, then masked) are the same for every plane, so the loads need no
+  // branches
+  const WindowMap win = window_map(L, V);
   float pre[kWinPerThread];
   auto fetch = [&](int q) {
     const float* plane = terms + (long long)q * total_px + V.pixel_base;
 #pragma unroll
     for (int r = 0; r < kWinPerThread; ++r) {
-      const int idx = tid + r * 256;
-      const int hy = idx / kWinW, hx = idx % kWinW;
-      const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
-      pre[r] = (idx < kWinH * kWinW && qx >= 0 && qx < V.width && qy >= 0 && qy < V.height)
-                   ? plane[(long long)qy * V.width + qx]
-                   : 0.f;
+      const float x = plane[win.off[r]];
+      pre[r] = (win.inside >> r) & 1u ? x : 0.f;
     }
   };
   fetch(0);
