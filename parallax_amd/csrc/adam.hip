@@ -5,8 +5,8 @@
 // shared betas and bias corrections; the quaternion is renormalised after its
 // update (identity if its norm collapses). Also folds in accumulateStatistics
 // (:258-263) for the densify signal. Elementwise over the capacity-laid
-// parameter buffer, four elements per thread (16-byte accesses where the
-// group is aligned). HBM-bound: 28 B/param of
+// parameter buffer, four elements per thread (16-byte non-temporal accesses
+// where the group is aligned: the step streams them once). HBM-bound: 28 B/param of
 // compulsory traffic (read p, g, m, v; write p, m, v) plus the 4 B gradient
 // clear. Arithmetic is IEEE float without contraction, so with equal
 // gradients the update matches the CPU trainer bit for bit.
@@ -124,18 +124,22 @@ __device__ __forceinline__ void adam_unit(const Groups& G, const AdamStep& P, lo
   }
   const long long count = U.len - U.local;  // elements of this group left from here
   if (count >= 4 && (o & 3) == 0) {
-    float4 pp = *reinterpret_cast<const float4*>(params + o);
-    const float4 gg = *reinterpret_cast<const float4*>(grads + o);
-    float4 mm = *reinterpret_cast<const float4*>(m + o);
-    float4 vv = *reinterpret_cast<const float4*>(v + o);
-    pp.x += adam_delta(mm.x, vv.x, gg.x, lr, P);
-    pp.y += adam_delta(mm.y, vv.y, gg.y, lr, P);
-    pp.z += adam_delta(mm.z, vv.z, gg.z, lr, P);
-    pp.w += adam_delta(mm.w, vv.w, gg.w, lr, P);
-    *reinterpret_cast<float4*>(params + o) = pp;
-    *reinterpret_cast<float4*>(m + o) = mm;
-    *reinterpret_cast<float4*>(v + o) = vv;
-    *reinterpret_cast<float4*>(grads + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v pp = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(params + o));
+    const f4v gg = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(grads + o));
+    f4v mm = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(m + o));
+    f4v vv = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(v + o));
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      float ma = mm[a], va = vv[a];
+      pp[a] += adam_delta(ma, va, gg[a], lr, P);
+      mm[a] = ma;
+      vv[a] = va;
+    }
+    __builtin_nontemporal_store(pp, reinterpret_cast<f4v*>(params + o));
+    __builtin_nontemporal_store(mm, reinterpret_cast<f4v*>(m + o));
+    __builtin_nontemporal_store(vv, reinterpret_cast<f4v*>(v + o));
+    __builtin_nontemporal_store((f4v){0.f, 0.f, 0.f, 0.f}, reinterpret_cast<f4v*>(grads + o));
     return;
   }
   const int todo = count < 4 ? (int)count : 4;
