@@ -202,12 +202,17 @@ def main():
     if t is not None:
         roofline["traffic"] = t
         roofline["traffic_source"] = f"{PROFILE_FILE} ({variant}, FETCH_SIZE + WRITE_SIZE)"
-    # HBM-bound kernels beside it
+    # HBM-bound kernels beside it, each with its compulsory bytes per launch
     rs = 45
-    adam_bytes = 32.0 * n * (14 + rs)
-    proj_bytes = n * (56 + 4 * rs) + n * B * 76.0
+    px_batch = sum(views[i].pixels for i in batches[0])
+    adam_bytes = 32.0 * n * (14 + rs)  # read p, g, m, v; write p, m, v; clear g
+    proj_bytes = n * (56 + 4 * rs) + n * B * 84.0  # parameters; 64 B record + keys, counts, rect per slot
+    adjoint_bytes = 3.0 * n * (56 + 4 * rs) + n * B * (64 + 48)  # parameters, gradients read and written; record + screen
+    loss_bytes = px_batch * (84.0 + 96.0)  # stats: rgb, target in, 5 planes out; grad: 5 planes, rgb, target in, dL out
+    list_bytes = 4.0 * pairs  # the per-tile lists written
     hbm = {}
-    for k, b in (("adam", adam_bytes), ("project", proj_bytes)):
+    for k, b in (("adam", adam_bytes), ("project", proj_bytes), ("adjoint", adjoint_bytes), ("loss", loss_bytes),
+                 ("tile_write", list_bytes)):
         if k in kernels:
             ms = kernels[k][0] / kernels[k][1]
             hbm[k] = {"GB/s": round(b / (ms * 1e-3) / 1e9, 1), "frac": round(b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
