@@ -350,27 +350,28 @@ __device__ __forceinline__ float swap16_sum(float x, float y) {
 
 // Sum 11 per-lane values over the wave64 in one transposed butterfly: each
 // step exchanges half of the values a lane holds with its partner and keeps
-// the other half. Lane bits 5 and 4 go first, through permlane swaps (no
-// selects); bits 0 and 1 through DPP with selects; bits 2 and 3 then sum the
-// one remaining value. Lane l ends with the wave total of value
-// wave_sum_slot(l) (slots 11-15 hold no value). The summation order is fixed,
-// so it is deterministic.
+// the other half. The 11 values (and a zero) go through 6, then 3 registers
+// across lane bits 5 and 4 (permlane swaps, no selects), then 2 and 1 across
+// bits 0 and 1 (DPP; the third register pairs with itself); bits 2 and 3
+// then sum the one remaining value. Lane l ends with the wave total of value
+// wave_sum_slot(l) (slot 11 is the zero); lanes with bits 0 and 1 both set
+// repeat their bit-0 partner's slot. The summation order is fixed, so it is
+// deterministic.
 __device__ __forceinline__ float wave_sum11(const float (&v)[kVals], int lane) {
-  float a[8], c[4], d[2];
+  float r1[6], r2[3];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) a[k] = k + 8 < kVals ? swap32_sum(v[k], v[k + 8]) : swap32_sum(v[k], v[k]);
+  for (int i = 0; i < 5; ++i) r1[i] = swap32_sum(v[2 * i], v[2 * i + 1]);
+  r1[5] = swap32_sum(v[10], 0.f);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) c[k] = swap16_sum(a[k], a[k + 4]);
+  for (int m = 0; m < 3; ++m) r2[m] = swap16_sum(r1[2 * m], r1[2 * m + 1]);
   const bool b0 = lane & 1, b1 = lane & 2;
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const float send = b0 ? c[k] : c[k + 2];
-    const float keep = b0 ? c[k + 2] : c[k];
-    d[k] = keep + xchg<0>(send);
-  }
-  const float send = b1 ? d[0] : d[1];
-  const float keep = b1 ? d[1] : d[0];
-  float e = keep + xchg<1>(send);
+  const float send0 = b0 ? r2[0] : r2[1];
+  const float keep0 = b0 ? r2[1] : r2[0];
+  const float r3a = keep0 + xchg<0>(send0);
+  const float r3b = r2[2] + xchg<0>(r2[2]);
+  const float send1 = b1 ? r3a : r3b;
+  const float keep1 = b1 ? r3b : r3a;
+  float e = keep1 + xchg<1>(send1);
   e = e + xchg<2>(e);
   e = e + xchg<3>(e);
   return e;
@@ -378,7 +379,8 @@ __device__ __forceinline__ float wave_sum11(const float (&v)[kVals], int lane) {
 
 // The value whose total wave_sum11 leaves in lane `lane`.
 __device__ __forceinline__ int wave_sum_slot(int lane) {
-  return ((lane >> 1) & 1) | ((lane & 1) << 1) | (((lane >> 4) & 1) << 2) | (((lane >> 5) & 1) << 3);
+  const int b0 = lane & 1, b1 = (lane >> 1) & 1, b4 = (lane >> 4) & 1, b5 = (lane >> 5) & 1;
+  return 4 * (b1 ? 2 : b0) + b5 + 2 * b4;
 }
 
 // The adjoint of one 16x16 tile by one wave: each lane owns four pixels, one
@@ -410,7 +412,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   __shared__ int s_unit;
   const int lane = threadIdx.x;
   const int my_slot = wave_sum_slot(lane);
-  const bool lead = (lane & 12) == 0;  // one lane per slot after wave_sum11
+  const bool lead = (lane & 12) == 0 && (lane & 3) != 3;  // one lane per slot after wave_sum11
   const bool writer = lead && my_slot < kVals;
   for (int iter = 0;; ++iter) {
     int tile;
@@ -588,7 +590,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           }
           float z = 0.f;
           if (any) z = wave_sum11(acc, lane);
-          // the 16 lead lanes hold the splat's wave totals: one instruction
+          // the 12 lead lanes hold the splat's wave totals: one instruction
           // covers the splat's contiguous 48-byte record -- atomics (nonzero
           // terms only) or the partial's stores (slot 11 is padding, zero)
           const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
