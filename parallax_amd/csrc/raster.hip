@@ -412,6 +412,9 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   __shared__ int s_unit;
   const int lane = threadIdx.x;
   const int my_slot = wave_sum_slot(lane);
+  // terms 7 and 9 (dL/dconic xx, yy) are summed without their factor 1/2,
+  // applied to the total (exact: a power of two)
+  const float slot_scale = (my_slot == 7 || my_slot == 9) ? 0.5f : 1.f;
   const bool lead = (lane & 12) == 0 && (lane & 3) != 3;  // one lane per slot after wave_sum11
   const bool writer = lead && my_slot < kVals;
   for (int iter = 0;; ++iter) {
@@ -565,22 +568,22 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             const float weight = alpha_c * Tn;
             const float d0 = b.z - suf0[k], d1 = b.w - suf1[k], d2 = cblue - suf2[k];
             const float dL_dalpha = (d0 * dL0[k] + d1 * dL1[k] + d2 * dL2[k]) * Tn - tb[k] * inv;
-            const float dL_dweight = shape ? b.y * dL_dalpha : 0.f;
             const float Gs = shape ? G : 0.f;
-            const float pw = dL_dweight * G;  // dL/dpower
+            const float gd = Gs * dL_dalpha;  // dL/dopacity term
+            const float pw = b.y * gd;        // dL/dpower
             const float px = pw * dx, py = pw * dy;
             const float u = px * a.z + py * a.w;  // -dL/dmean_x
             const float w = py * b.x + px * a.w;  // -dL/dmean_y
             acc[0] += weight * dL0[k];
             acc[1] += weight * dL1[k];
             acc[2] += weight * dL2[k];
-            acc[3] += Gs * dL_dalpha;
+            acc[3] += gd;
             acc[4] += u;
             acc[5] += w;
             acc[6] += __builtin_amdgcn_sqrtf(u * u + w * w);  // densify statistic: hardware sqrt
-            acc[7] -= 0.5f * px * dx;
+            acc[7] -= px * dx;  // halved after the wave sum
             acc[8] -= px * dy;
-            acc[9] -= 0.5f * py * dy;
+            acc[9] -= py * dy;  // halved after the wave sum
             acc[10] += shape ? 1.f : 0.f;
             T[k] = contrib ? Tn : T[k];
             suf0[k] += alpha_c * d0;
@@ -589,7 +592,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             }
           }
           float z = 0.f;
-          if (any) z = wave_sum11(acc, lane);
+          if (any) z = wave_sum11(acc, lane) * slot_scale;
           // the 12 lead lanes hold the splat's wave totals: one instruction
           // covers the splat's contiguous 48-byte record -- atomics (nonzero
           // terms only) or the partial's stores (slot 11 is padding, zero)
