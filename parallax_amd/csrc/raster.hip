@@ -212,6 +212,8 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
       T = use ? T * (1.f - alpha) : T;
       last = use ? __float_as_int(sc.y) + 1 : last;
       done = done || (use && T < p.cutoff);
+      // every 4 splats: leave once all the wave's pixels have terminated
+      if ((j & 3) == 3 && __ballot(!done) == 0ull) break;
     }
     wave_sync_lds();
   }
