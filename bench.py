@@ -67,8 +67,8 @@ def log(*a):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (data-parallel sharding by view is "
-              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a batch's views would shard data-parallel "
+              "across GPUs, with one gradient all-reduce per step); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from parallax_amd import splat as S
     from parallax_amd import synthetic as syn
