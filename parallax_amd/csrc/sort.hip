@@ -280,6 +280,13 @@ __global__ void k_fill_f32(float* p, float v, long long n) {
     p[i] = v;
 }
 
+// 16-byte stores over a 16-byte-aligned buffer of n4 float4s.
+__global__ void k_fill_f32x4(float4* p, float v, long long n4) {
+  const float4 q = make_float4(v, v, v, v);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+    p[i] = q;
+}
+
 unsigned grid_for(long long n, int threads, long long max_blocks = 65535LL * 16) {
   long long b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -351,7 +358,12 @@ hipError_t launch_fill_u32(uint32_t* ptr, uint32_t value, long long n, hipStream
 
 hipError_t launch_fill_f32(float* ptr, float value, long long n, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256)), dim3(256), 0, stream, ptr, value, n);
+  if ((reinterpret_cast<uintptr_t>(ptr) & 15) == 0 && (n & 3) == 0) {
+    hipLaunchKernelGGL(k_fill_f32x4, dim3(grid_for(n / 4, 256)), dim3(256), 0, stream, reinterpret_cast<float4*>(ptr),
+                       value, n / 4);
+  } else {
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256)), dim3(256), 0, stream, ptr, value, n);
+  }
   return hipGetLastError();
 }
 

@@ -57,7 +57,7 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
 hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream);
 
 // ---- binning (binning.hip) ----
-constexpr int kBinSegment = 1024;  // splats per segment of a super-tile list
+constexpr int kBinSegment = 256;   // splats per segment of a super-tile list
 constexpr int kStBits = 20;        // super-tile id bits of a super-tile key
 constexpr uint32_t kStMask = (1u << kStBits) - 1u;
 
