@@ -13,7 +13,7 @@
 // depth-ordered list directly:
 //   1. emit:    for every depth-sorted slot, one (super-tile id, slot) pair per
 //               super-tile its rect touches (stable radix sort by id next);
-//   2. segment: each super-tile's list is cut into segments of 1024 splats;
+//   2. segment: each super-tile's list is cut into segments of kBinSegment (256) splats;
 //   3. count:   one wave per segment; per chunk of 64 splats, eight row and
 //               eight column ballots (from the rect bits the emitter packed
 //               into the key) give each tile's covering lanes as one AND, and
