@@ -208,7 +208,7 @@ def main():
     adam_bytes = 32.0 * n * (14 + rs)  # read p, g, m, v; write p, m, v; clear g
     proj_bytes = n * (56 + 4 * rs) + n * B * 84.0  # parameters; 64 B record + keys, counts, rect per slot
     adjoint_bytes = 3.0 * n * (56 + 4 * rs) + n * B * (64 + 48)  # parameters, gradients read and written; record + screen
-    loss_bytes = px_batch * (84.0 + 96.0)  # stats: rgb, target in, 5 planes out; grad: 5 planes, rgb, target in, dL out
+    loss_bytes = px_batch * (60.0 + 72.0)  # stats: rgb, target in, 3x3 planes out; grad: 3x3 planes, rgb, target in, dL out
     list_bytes = 4.0 * pairs  # the per-tile lists written
     hbm = {}
     for k, b in (("adam", adam_bytes), ("project", proj_bytes), ("adjoint", adjoint_bytes), ("loss", loss_bytes),

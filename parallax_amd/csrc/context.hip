@@ -476,7 +476,7 @@ splat_status reserve_loss(splat_context* ctx, float w) {
   CK(ctx->dL.reserve(sizeof(float) * 3 * std::max(1LL, px)), "alloc dL");
   CK(ctx->block_sums.reserve(sizeof(float) * 2 * std::max(1, ctx->loss_tiles)), "alloc loss");
   CK(ctx->loss_sums.reserve(sizeof(float) * 2 * V), "alloc loss");
-  if (w > 0.f) CK(ctx->terms.reserve(sizeof(float) * 15 * std::max(1LL, px)), "alloc ssim terms");
+  if (w > 0.f) CK(ctx->terms.reserve(sizeof(float) * kSsimPlanes * std::max(1LL, px)), "alloc ssim terms");
   return SPLAT_OK;
 }
 

@@ -5,7 +5,12 @@
 // 11-tap sigma 1.5 Gaussian window applied as a zero-padded separable blur
 // (rows first, then columns), C1 = 0.01^2, C2 = 0.03^2, means over 3 * pixels.
 // The gradient of the SSIM term is itself a blur of five per-pixel partial
-// planes (the blur is self-adjoint on a zero-padded domain).
+// planes (the blur is self-adjoint on a zero-padded domain):
+//   dSSIM/dx = B(t_mu) + 2x B(t_var) - 2 B(t_var mu_x) + y B(t_cov) - B(t_cov mu_y)
+// and, B being linear, the three planes that carry no per-pixel factor are
+// summed before the blur: A = t_mu - 2 t_var mu_x - t_cov mu_y, so three
+// planes per channel are written, read and blurred instead of five
+// (dSSIM/dx = B(A) + 2x B(t_var) + y B(t_cov)).
 //
 // Two passes over 64x16-pixel loss tiles of every view of the batch, one
 // 256-thread workgroup each, one channel at a time:
@@ -15,15 +20,16 @@
 //          then a column pass per plane down to the tile's 16 rows, every
 //          thread sliding an 11-tap window over a run of 4 outputs in
 //          registers; then
-//          per pixel the SSIM value, the five partial planes (to HBM), and the
-//          L1 terms; per-tile L1 and SSIM sums.
-//   grad:  per partial plane, its 74x26 window to LDS, the same two passes,
-//          and the blurred planes combined into dL/drgb.
-// Each blur tap is one fused multiply-add in tap order -5..5 and every other
-// expression is the engine's, exactly as the CPU checker evaluates them, so
-// dL/drgb matches the checker to float rounding of identical operations.
-// Compulsory HBM traffic per pixel: 24 B read and 60 B written by the first
-// pass, 84 B read and 12 B written by the second (halos hit in cache).
+//          per pixel the SSIM value, the three partial planes (to HBM), and
+//          the L1 terms; per-tile L1 and SSIM sums.
+//   grad:  per partial plane, its 74x26 window to LDS (two buffers, so a
+//          plane costs two workgroup barriers), the same two passes, and the
+//          blurred planes combined into dL/drgb.
+// Each blur tap is one fused multiply-add in tap order -5..5 and the loss
+// value's expressions are the engine's, as the CPU checker evaluates them;
+// dL/drgb differs from the checker's only by the rounding of the plane sum A.
+// Compulsory HBM traffic per pixel: 24 B read and 36 B written by the first
+// pass, 60 B read and 12 B written by the second (halos hit in cache).
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -234,12 +240,10 @@ __global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc
         const float term_mu = inverse * (2.f * mu_y * b / (cc * d) - 2.f * mu_x * a * b / (cc * cc * d));
         const float term_var = inverse * (-(a * b) / (cc * d * d));
         const float term_cov = inverse * (2.f * a / (cc * d));
-        float* tp = terms + (long long)(c * 5) * total_px + pixel;
-        tp[0] = term_mu;
+        float* tp = terms + (long long)(c * 3) * total_px + pixel;
+        tp[0] = (term_mu - 2.f * term_var * mu_x) - term_cov * mu_y;
         tp[total_px] = term_var;
-        tp[2 * total_px] = term_var * mu_x;
-        tp[3 * total_px] = term_cov;
-        tp[4 * total_px] = term_cov * mu_y;
+        tp[2 * total_px] = term_cov;
       } else {
         const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
         dL[3 * pixel + c] = sign * inverse;
@@ -270,8 +274,11 @@ __global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc*
                                                    const float* __restrict__ rgb, const float* __restrict__ targets,
                                                    const float* __restrict__ terms, long long total_px,
                                                    float* __restrict__ dL) {
-  __shared__ float s_t[kWinH][kPitch];
-  __shared__ float s_h[kWinH][kHPitch];
+  // plane q stages through buffer q & 1: the barrier after a plane's window
+  // store also orders the previous plane's column reads before this plane's
+  // row pass overwrites the other buffer
+  __shared__ float s_t[2][kWinH][kPitch];
+  __shared__ float s_h[2][kWinH][kHPitch];
   const LossTile L = loss_tile_of(p, views);
   const ViewDesc& V = *L.V;
   const int tid = threadIdx.x;
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc*
   const float* img = rgb + 3 * V.pixel_base;
   const float* tgt = targets + V.target_base;
   const int gx = L.ox + j;
-  // the 15 partial planes' windows, each prefetched into registers while the
+  // the 9 partial planes' windows, each prefetched into registers while the
   // previous one is blurred; the window's element offsets (0 outside the
   // image, then masked) are the same for every plane, so the loads need no
   // branches
@@ -297,53 +304,50 @@ __global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc*
     }
   };
   fetch(0);
-  float d[kRun], xs[kRun], ys[kRun];
-  for (int q = 0; q < 15; ++q) {
-    const int c = q / 5, pl = q % 5;
-    if (pl == 0) {
 #pragma unroll
-      for (int k = 0; k < kRun; ++k) {
-        const int gy = L.oy + r0 + k;
-        d[k] = xs[k] = ys[k] = 0.f;
-        if (gx < V.width && gy < V.height) {
-          const long long qq = (long long)gy * V.width + gx;
-          xs[k] = img[3 * qq + c];
-          ys[k] = tgt[3 * qq + c];
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < kWinPerThread; ++r) {
-      const int idx = tid + r * 256;
-      if (idx < kWinH * kWinW) s_t[idx / kWinW][idx % kWinW] = pre[r];
-    }
-    __syncthreads();
-    if (q + 1 < 15) fetch(q + 1);
-    row_pass(0, s_t, s_t, p.kernel, s_h);
-    __syncthreads();
-    float bl[kRun];
-    col_pass(s_h, p.kernel, j, r0, bl);
+  for (int c = 0; c < 3; ++c) {
+    float d[kRun], xs[kRun], ys[kRun];
 #pragma unroll
     for (int k = 0; k < kRun; ++k) {
-      if (pl == 0) d[k] += bl[k];
-      else if (pl == 1) d[k] += 2.f * xs[k] * bl[k];
-      else if (pl == 2) d[k] -= 2.f * bl[k];
-      else if (pl == 3) d[k] += ys[k] * bl[k];
-      else d[k] -= bl[k];
-    }
-    if (pl == 4) {
-#pragma unroll
-      for (int k = 0; k < kRun; ++k) {
-        const int gy = L.oy + r0 + k;
-        if (gx >= V.width || gy >= V.height) continue;
-        const float difference = xs[k] - ys[k];
-        const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
-        float g = (1.f - w) * sign * inverse;
-        g -= w * d[k];
-        dL[3 * (V.pixel_base + (long long)gy * V.width + gx) + c] = g;
+      const int gy = L.oy + r0 + k;
+      xs[k] = ys[k] = 0.f;
+      if (gx < V.width && gy < V.height) {
+        const long long qq = (long long)gy * V.width + gx;
+        xs[k] = img[3 * qq + c];
+        ys[k] = tgt[3 * qq + c];
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      const int q = 3 * c + pl, buf = q & 1;
+#pragma unroll
+      for (int r = 0; r < kWinPerThread; ++r) {
+        const int idx = tid + r * 256;
+        if (idx < kWinH * kWinW) s_t[buf][idx / kWinW][idx % kWinW] = pre[r];
+      }
+      __syncthreads();
+      if (q + 1 < kSsimPlanes) fetch(q + 1);
+      row_pass(0, s_t[buf], s_t[buf], p.kernel, s_h[buf]);
+      __syncthreads();
+      float bl[kRun];
+      col_pass(s_h[buf], p.kernel, j, r0, bl);
+#pragma unroll
+      for (int k = 0; k < kRun; ++k) {
+        if (pl == 0) d[k] = bl[k];
+        else if (pl == 1) d[k] += 2.f * xs[k] * bl[k];
+        else d[k] += ys[k] * bl[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) {
+      const int gy = L.oy + r0 + k;
+      if (gx >= V.width || gy >= V.height) continue;
+      const float difference = xs[k] - ys[k];
+      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+      float g = (1.f - w) * sign * inverse;
+      g -= w * d[k];
+      dL[3 * (V.pixel_base + (long long)gy * V.width + gx) + c] = g;
+    }
   }
 }
 

@@ -147,9 +147,11 @@ struct LossParams {
   int total_tiles;  // loss tiles (64x16 pixels) over all views
 };
 // L1 + SSIM loss terms and dL/drgb of every view of the batch (two passes
-// over 64x16-pixel loss tiles; `terms` holds 15 floats per pixel between
-// them); per-loss-tile partial sums of L1 and SSIM into block_sums (2 floats
-// per loss tile). Each view's target starts at targets + views[v].target_base.
+// over 64x16-pixel loss tiles; `terms` holds kSsimPlanes floats per pixel
+// between them); per-loss-tile partial sums of L1 and SSIM into block_sums (2
+// floats per loss tile). Each view's target starts at targets +
+// views[v].target_base.
+constexpr int kSsimPlanes = 9;  // three partial planes per channel
 int loss_tiles(int width, int height);
 hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* rgb, const float* targets,
                        float* dL, float* terms, long long total_px, float* block_sums, hipStream_t stream);

@@ -62,6 +62,27 @@ def test_raster_golden(name):
     assert rel_l2(g.screen_gradient, d["grad_screen_gradient"]) <= 1e-4
 
 
+@pytest.mark.parametrize("name", ["fd16_sh2", "c1_small_sh0", "sh3_bg"])
+def test_float64_loss_yardstick_matches_the_reference(name):
+    """The float64 loss restatement that measures dL/drgb accuracy reproduces
+    the reference build's loss and dL/drgb on its own rendered image, and the
+    oracle's float32 gradient stays within the accuracy bar the device is held
+    to."""
+    import ssim64
+    d = load(f"raster_{name}.npz")
+    w, h = (int(v) for v in d["view_size"])
+    loss64, dl64 = ssim64.photometric_loss(d["rgb"], d["target"], w, h, float(d["ssim_weight"]))
+    assert abs(loss64 - float(d["loss"])) <= 1e-5
+    assert ssim64.errors(d["dL"], dl64)[0] <= 2e-5
+    _, dl = O.photometric_loss(d["rgb"], d["target"], w, h, float(d["ssim_weight"]))
+    worst, l2 = ssim64.errors(dl, dl64)
+    assert worst <= 5e-5 and l2 <= 2e-5
+    # a gradient with any one term dropped is far outside the bar
+    for term in ("mu", "var", "var_mu", "cov", "cov_mu"):
+        _, broken = ssim64.photometric_loss(d["rgb"], d["target"], w, h, float(d["ssim_weight"]), omit=term)
+        assert ssim64.errors(broken, dl64)[0] > 1e-3, term
+
+
 @pytest.mark.parametrize("name", ["prune", "cap"])
 def test_densify_golden(name):
     d = load(f"densify_{name}.npz")
