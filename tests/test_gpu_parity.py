@@ -31,6 +31,12 @@ def scenes():
     yield "fd_scene_16x16_sh2", syn.overlapping_cloud(2), syn.test_view(16, 16), 2, (0.2, 0.3, 0.4)
     yield ("ragged_tiles_sh1", syn.random_cloud(3000, 9, 1, (-1, -1, 2), (1, 1, 6)),
            syn.pinhole(97, 61, 80.0), 1, (0.0, 0.0, 0.0))
+    # large splats: the 256-splat segments of the super-tile lists hold 3100
+    # to 7800 (tile, splat) entries, on both sides of the 4096 the list writer
+    # assembles in LDS
+    yield ("big_splats_sh1", syn.random_cloud(1500, 13, 1, (-1, -1, 3), (1, 1, 5), log_scale_mu=-1.5,
+                                              log_scale_sigma=0.3),
+           syn.pinhole(256, 256, 300.0), 1, (0.0, 0.0, 0.0))
 
 
 def flat_grads(g):
