@@ -44,6 +44,14 @@
 namespace splat {
 namespace {
 
+// det_exp_nonpos(min(power, 0)), the compositing falloff: one med3 clamps the
+// argument to [-80, 0] (the same value as the two-step clamp; the exponential
+// of -0 and of +0 are both exactly 1).
+__device__ __forceinline__ float falloff(float power) {
+  const float e = splat_det::exp_core(__builtin_amdgcn_fmed3f(power, -80.0f, 0.0f));
+  return power <= -80.0f ? 0.0f : e;
+}
+
 constexpr int kChunk = 64;    // forward: splats fetched per wave step
 constexpr int kVals = 11;  // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3, touched
 
@@ -188,7 +196,7 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
       const int pos = lane_rank(m);
       s_a[pos] = a;
       s_b[pos] = b;
-      s_c[pos] = make_float2(c.x, __int_as_float((int)(e - begin)));
+      s_c[pos] = make_float2(c.x, __int_as_float((int)(e - begin) + 1));  // list position + 1
     }
     wave_sync_lds();
     const int cnt = __popcll(m);
@@ -202,16 +210,16 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
       const float2 sc = s_c[j];
       const float dx = fx - sa.x, dy = fy - sa.y;
       const float power = -0.5f * (sa.z * dx * dx + sb.x * dy * dy) - sa.w * dx * dy;
-      const float raw = sb.y * det_exp_nonpos(power < 0.f ? power : 0.f);
+      const float raw = sb.y * falloff(power);
       const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-      const bool use = !done && !(power > 0.f) && !(alpha < p.min_alpha);
+      const bool use = !done & !(power > 0.f) & !(alpha < p.min_alpha);
       const float w = use ? T * alpha : 0.f;
       acc0 += w * sb.z;
       acc1 += w * sb.w;
       acc2 += w * sc.x;
       T = use ? T * (1.f - alpha) : T;
-      last = use ? __float_as_int(sc.y) + 1 : last;
-      done = done || (use && T < p.cutoff);
+      last = use ? __float_as_int(sc.y) : last;
+      done = done | (use & (T < p.cutoff));
       // every 4 splats: leave once all the wave's pixels have terminated
       if ((j & 3) == 3 && __ballot(!done) == 0ull) break;
     }
@@ -548,7 +556,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             // the forward's decisions, recomputed with its exact float operations
             const float dx = (k & 1 ? fx0 + 8.f : fx0) - a.x, dy = (k >> 1 ? fy0 + 8.f : fy0) - a.y;
             const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-            const float G = det_exp_nonpos(power < 0.f ? power : 0.f);
+            const float G = falloff(power);
             const float raw = b.y * G;
             const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
             const bool contrib = idx < lastp[k] && !(power > 0.f) && !(alpha < p.min_alpha);
