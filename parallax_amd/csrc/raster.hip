@@ -220,8 +220,9 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
       T = use ? T * (1.f - alpha) : T;
       last = use ? __float_as_int(sc.y) : last;
       done = done | (use & (T < p.cutoff));
-      // every 4 splats: leave once all the wave's pixels have terminated
-      if ((j & 3) == 3 && __ballot(!done) == 0ull) break;
+      // leave once all the wave's pixels have terminated (the flags are lane
+      // masks, so the test is a scalar compare after every splat)
+      if (__ballot(!done) == 0ull) break;
     }
     wave_sync_lds();
   }
