@@ -173,6 +173,20 @@ splat_status splat_download_tiles(splat_context* ctx, int32_t slot, int32_t* off
 /* Photometric loss (1-w) L1 + w (1-SSIM) of every batch view against its
  * target; writes dL/drgb for the backward and the per-view losses. */
 splat_status splat_loss(splat_context* ctx, float ssim_weight, float* losses);
+/* No counterpart in the engine: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). splat_loss of one image held in the caller's device
+ * memory: rgb and target (3*W*H floats each, interleaved linear RGB) in,
+ * dL_drgb (3*W*H) and the loss (one float) out, all device pointers. Every
+ * intermediate lives in one fixed device array of the library (about 76 MB
+ * per device, held until the process ends), so the call allocates nothing;
+ * W*H <= 2,097,152 (1920x1080 fits) and each side <= 8192, else
+ * SPLAT_ERROR_CAPACITY. Results are bit-identical to splat_loss on the same
+ * image. Enqueued on the context's stream with no host synchronisation;
+ * calls that overlap on one device (any context, any stream) race on the
+ * shared array. */
+splat_status splat_loss_noalloc(splat_context* ctx, const float* rgb, const float* target, int32_t width,
+                                int32_t height, float ssim_weight, float* dL_drgb, float* loss);
 /* Replace dL/drgb of batch slot `slot` (3*W*H) -- for objectives other than
  * the photometric loss. */
 splat_status splat_set_image_gradient(splat_context* ctx, int32_t slot, const float* dL_drgb);

@@ -854,8 +854,10 @@ splat_status splat_create(int32_t device, splat_context** out) {
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     ctx->rcfg.cus = cus > 0 ? cus : 256;
   }
+  // the fixed scratch of splat_loss_noalloc, looked up with this device current
+  if (e == hipSuccess) e = noalloc_attach(device);
   if (e != hipSuccess) {
-    std::string msg = std::string("stream creation failed: ") + hipGetErrorString(e);
+    std::string msg = std::string("context set-up failed: ") + hipGetErrorString(e);
     delete ctx;
     return fail(nullptr, SPLAT_ERROR_DEVICE, msg);
   }

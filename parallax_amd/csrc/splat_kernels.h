@@ -158,6 +158,9 @@ hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* 
 // Fixed-order reduction of the block sums into one (l1, ssim) pair per view.
 hipError_t launch_loss_reduce(const ViewDesc* views, int n_views, const float* block_sums, float* out,
                               hipStream_t stream);
+// Looks up, once per device, the device address of the fixed scratch array
+// behind splat_loss_noalloc; `device` must be current.
+hipError_t noalloc_attach(int device);
 
 // ---- per-gaussian adjoint ----
 struct AdjointParams {

@@ -88,6 +88,8 @@ SIGNATURES = {
                                             i32p, i32p]),
     "splat_download_tiles": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p]),
     "splat_loss": (C.c_int, [C.c_void_p, C.c_float, f32p]),
+    "splat_loss_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
+                                     C.c_void_p]),
     "splat_set_image_gradient": (C.c_int, [C.c_void_p, C.c_int32, f32p]),
     "splat_download_image_gradient": (C.c_int, [C.c_void_p, C.c_int32, f32p]),
     "splat_render_backward": (C.c_int, [C.c_void_p]),
@@ -311,6 +313,15 @@ class Context:
         out = np.zeros(len(self._batch), np.float32)
         self._ck(lib().splat_loss(self.h, float(ssim_weight), _p(out)))
         return out
+
+    def loss_noalloc(self, rgb_dev: int, target_dev: int, width: int, height: int, ssim_weight: float,
+                     dL_dev: int, loss_dev: int):
+        """The loss of one image in device memory (addresses as ints), with
+        the library's fixed scratch: no allocation; asynchronous on the
+        context's stream (call synchronize() before reading the outputs)."""
+        self._ck(lib().splat_loss_noalloc(self.h, C.c_void_p(rgb_dev), C.c_void_p(target_dev), int(width),
+                                          int(height), float(ssim_weight), C.c_void_p(dL_dev),
+                                          C.c_void_p(loss_dev)))
 
     def set_image_gradient(self, slot: int, dL_drgb):
         d = np.ascontiguousarray(dL_drgb, dtype=np.float32)
