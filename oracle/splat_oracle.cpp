@@ -1268,7 +1268,7 @@ float maxScale(const float* ls) {
 
 }  // namespace
 
-// densifyAndPrune, SplatDensify.cpp:87-168.
+// densifyAndPrune, SplatDensify.cpp:28-109.
 int orc_densify(orc_cloud* cloud, int capacity, const float* screen_gradient, const int* visible_count,
                 const orc_config* c, float extent, int iteration, uint32_t* rng_state, int* source_index,
                 int* counts) {
@@ -1313,7 +1313,7 @@ int orc_densify(orc_cloud* cloud, int capacity, const float* screen_gradient, co
         for (int child = 0; child < 2; ++child) {
           const int k = push(g);
           // The reference draws the three normals as the arguments of one
-          // Vec3f constructor (SplatDensify.cpp:62-63), whose evaluation order
+          // Vec3f constructor (SplatDensify.cpp:63-64), whose evaluation order
           // C++ leaves unspecified; its GCC build evaluates them last to first,
           // which the golden vectors pin.
           float sample[3];

@@ -204,7 +204,7 @@ int orc_trainer_finish(orc_trainer* t);
  * group-concatenated moments; iteration is the 0-based step index. */
 void orc_adam_apply(orc_cloud* cloud, const orc_grads* g, float* m, float* v, int iteration,
                     float extent, const orc_config* c);
-/* densifyAndPrune (SplatDensify.cpp:87-168). The cloud arrays must have room
+/* densifyAndPrune (SplatDensify.cpp:28-109). The cloud arrays must have room
  * for `capacity` gaussians; cloud->n is updated. source_index gets the new
  * count entries. counts = {cloned, split, pruned}. */
 int orc_densify(orc_cloud* cloud, int capacity, const float* screen_gradient,

@@ -2,7 +2,7 @@
 // compaction, the Adam-moment remap, opacity reset, and the final prune +
 // sort by opacity.
 //
-// densifyAndPrune (engine/native/cpp/src/splat/SplatDensify.cpp:87-168) with
+// densifyAndPrune (engine/native/cpp/src/splat/SplatDensify.cpp:28-109) with
 // the trainer's remapState (SplatTrainer.cpp:267-288):
 //   1. classify: per gaussian, clone / split / none (only while below the
 //      cap), whether the survivor is pruned, and how many of its additions

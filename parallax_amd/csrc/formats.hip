@@ -354,77 +354,99 @@ __global__ __launch_bounds__(256) void k_unpack_spz(const uint8_t* __restrict__ 
 
 int rest_per_channel(int degree) { return degree <= 0 ? 0 : (degree == 1 ? 3 : (degree == 2 ? 8 : 15)); }
 
-// A parsed 3DGS PLY header (readGaussianPly's rules, GaussianCloud.cpp:311-365).
+// A parsed 3DGS PLY header: which inputs readGaussianPly accepts and the
+// error it reports first (GaussianCloud.cpp:305-374). The header is read from
+// the first 64 KiB of the file; `index` maps a property name to its first
+// position in the row.
 struct PlyHeader {
   size_t count = 0;
   size_t body = 0;  // byte offset of the first row
   std::vector<std::string> properties;
+  std::unordered_map<std::string, int> first;
   int degree = 0;
   int rest_count = 0;
+  int index(const std::string& name) const {
+    const auto it = first.find(name);
+    return it == first.end() ? -1 : it->second;
+  }
 };
 
+// The whitespace-separated words of header bytes [b, e), as a stream reads
+// strings (the C locale's isspace).
+std::vector<std::string> header_words(const char* b, const char* e) {
+  std::vector<std::string> words;
+  const char* p = b;
+  for (;;) {
+    while (p < e && std::isspace(static_cast<unsigned char>(*p))) ++p;
+    if (p == e) return words;
+    const char* q = p;
+    while (q < e && !std::isspace(static_cast<unsigned char>(*q))) ++q;
+    words.emplace_back(p, q);
+    p = q;
+  }
+}
+
 bool parse_ply_header(const std::vector<uint8_t>& bytes, PlyHeader* h, std::string* error) {
-  const std::string text(bytes.begin(), bytes.begin() + std::min<size_t>(bytes.size(), 65536));
-  const size_t header_end = text.find("end_header\n");
-  if (text.compare(0, 3, "ply") != 0 || header_end == std::string::npos) {
-    *error = "not a PLY file";
-    return false;
-  }
-  if (text.find("binary_little_endian") == std::string::npos) {
-    *error = "only binary little-endian 3DGS PLY is supported";
-    return false;
-  }
-  std::istringstream lines(text.substr(0, header_end));
-  std::string line;
-  while (std::getline(lines, line)) {
-    std::istringstream ls(line);
-    std::string keyword;
-    ls >> keyword;
-    if (keyword == "element") {
-      std::string name;
-      ls >> name >> h->count;
-    } else if (keyword == "property") {
-      std::string type, name;
-      ls >> type >> name;
-      if (type != "float" && type != "float32") {
-        *error = "3DGS PLY properties must be float";
-        return false;
-      }
+  enum Fault { kNone, kNotPly, kNotLittleEndian, kNotFloat, kNoVertices, kNoCore, kRestNot3, kRestNoDegree, kShort };
+  static const char* const kMessage[] = {"",
+                                         "not a PLY file",
+                                         "only binary little-endian 3DGS PLY is supported",
+                                         "3DGS PLY properties must be float",
+                                         "3DGS PLY has no vertices",
+                                         "3DGS PLY is missing core properties",
+                                         "f_rest count is not a multiple of 3",
+                                         "f_rest count does not correspond to an SH degree",
+                                         "3DGS PLY body is truncated"};
+  auto verdict = [&](Fault f) {
+    *error = kMessage[f];
+    return f == kNone;
+  };
+  static const char kEnd[] = "end_header\n";
+  const size_t scanned = std::min<size_t>(bytes.size(), 65536);
+  const char* text = reinterpret_cast<const char*>(bytes.data());
+  const char* stop = text + scanned;
+  const char* end = std::search(text, stop, kEnd, kEnd + sizeof(kEnd) - 1);
+  if (scanned < 3 || std::memcmp(text, "ply", 3) != 0 || end == stop) return verdict(kNotPly);
+  static const char kLe[] = "binary_little_endian";
+  if (std::search(text, stop, kLe, kLe + sizeof(kLe) - 1) == stop) return verdict(kNotLittleEndian);
+
+  // line by line up to end_header: every `element` line sets the count (when
+  // it has a third word; a word with no leading digits reads as 0), and every
+  // `property` line of any element must be a float, in row order
+  for (const char* line = text; line < end;) {
+    const char* eol = std::find(line, end, '\n');
+    const std::vector<std::string> w = header_words(line, eol);
+    line = eol + (eol < end ? 1 : 0);
+    if (w.empty()) continue;
+    if (w[0] == "element" && w.size() >= 3) {
+      h->count = std::strtoull(w[2].c_str(), nullptr, 10);
+    } else if (w[0] == "property") {
+      const std::string type = w.size() > 1 ? w[1] : std::string();
+      if (type != "float" && type != "float32") return verdict(kNotFloat);
+      const std::string name = w.size() > 2 ? w[2] : std::string();
+      h->first.emplace(name, static_cast<int>(h->properties.size()));
       h->properties.push_back(name);
     }
   }
-  if (h->count == 0 || h->properties.empty()) {
-    *error = "3DGS PLY has no vertices";
-    return false;
-  }
-  auto index_of = [h](const std::string& name) {
-    const auto it = std::find(h->properties.begin(), h->properties.end(), name);
-    return it == h->properties.end() ? -1 : static_cast<int>(it - h->properties.begin());
-  };
-  if (index_of("x") < 0 || index_of("y") < 0 || index_of("z") < 0 || index_of("opacity") < 0) {
-    *error = "3DGS PLY is missing core properties";
-    return false;
-  }
-  while (index_of("f_rest_" + std::to_string(h->rest_count)) >= 0) ++h->rest_count;
-  if (h->rest_count % 3 != 0) {
-    *error = "f_rest count is not a multiple of 3";
-    return false;
-  }
-  const int per_channel = h->rest_count / 3;
-  h->degree = 0;
+  if (h->count == 0 || h->properties.empty()) return verdict(kNoVertices);
+  for (const char* core : {"x", "y", "z", "opacity"})
+    if (h->index(core) < 0) return verdict(kNoCore);
+  h->rest_count = 0;
+  while (h->index("f_rest_" + std::to_string(h->rest_count)) >= 0) ++h->rest_count;
+  if (h->rest_count % 3 != 0) return verdict(kRestNot3);
+  // SH degree d carries (d + 1)^2 - 1 coefficients per channel
+  h->degree = -1;
   for (int d = 0; d <= 3; ++d)
-    if (rest_per_channel(d) == per_channel) h->degree = d;
-  if (rest_per_channel(h->degree) != per_channel) {
-    *error = "f_rest count does not correspond to an SH degree";
-    return false;
+    if ((d + 1) * (d + 1) - 1 == h->rest_count / 3) h->degree = d;
+  if (h->degree < 0) {
+    h->degree = 0;
+    return verdict(kRestNoDegree);
   }
-  h->body = header_end + std::strlen("end_header\n");
-  const size_t stride = h->properties.size() * sizeof(float);
-  if (bytes.size() < h->body + stride * h->count) {
-    *error = "3DGS PLY body is truncated";
-    return false;
-  }
-  return true;
+  h->body = static_cast<size_t>(end - text) + sizeof(kEnd) - 1;
+  const size_t row = h->properties.size() * sizeof(float);
+  const size_t room = bytes.size() - h->body;  // bytes.size() >= body: end_header lies inside the file
+  if (h->count > room / row) return verdict(kShort);
+  return verdict(kNone);
 }
 
 bool read_file(const char* path, std::vector<uint8_t>* out) {
@@ -751,10 +773,7 @@ splat_status splat_read_ply(splat_context* ctx, const char* path) {
   if (h.rest_count != ctx->rs)
     return fmt_fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT,
                     "read_ply: the file's SH degree does not match the capacity's (splat_ply_info first)");
-  auto index_of = [&h](const std::string& name) {
-    const auto it = std::find(h.properties.begin(), h.properties.end(), name);
-    return it == h.properties.end() ? -1 : static_cast<int>(it - h.properties.begin());
-  };
+  auto index_of = [&h](const std::string& name) { return h.index(name); };
   PlyIndex ix{};
   ix.x = index_of("x");
   ix.y = index_of("y");

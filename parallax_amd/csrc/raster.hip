@@ -45,8 +45,13 @@ namespace splat {
 namespace {
 
 // det_exp_nonpos(min(power, 0)), the compositing falloff: one med3 clamps the
-// argument to [-80, 0] (the same value as the two-step clamp; the exponential
-// of -0 and of +0 are both exactly 1).
+// argument to [-80, 0]. For every non-NaN power (+-inf, +-0 and -80 included)
+// this is the value of the two-step clamp; the exponential of -0 and of +0 are
+// both exactly 1. A NaN power is outside that claim: med3 then returns the
+// min of the other two operands (-80, so alpha ~ 0 and the splat is skipped)
+// where the CPU form gives alpha 0.99. A NaN power needs a non-finite mean or
+// conic, i.e. non-finite parameters, for which the engine's own binning is
+// undefined (floor of NaN converted to int, SplatRaster.cpp:104-151).
 __device__ __forceinline__ float falloff(float power) {
   const float e = splat_det::exp_core(__builtin_amdgcn_fmed3f(power, -80.0f, 0.0f));
   return power <= -80.0f ? 0.0f : e;

@@ -87,6 +87,80 @@ def test_ply_info_rejects_what_the_engine_rejects(body, message):
         os.unlink(path)
 
 
+def _ply(lines, body_bytes):
+    return ("\n".join(lines) + "\n").encode() + b"\x00" * body_bytes
+
+
+_CORE = ["property float x", "property float y", "property float z", "property float opacity"]
+_LE = ["ply", "format binary_little_endian 1.0"]
+_HEADERS = {
+    "core only": _ply(_LE + ["element vertex 2"] + _CORE + ["end_header"], 32),
+    "sh1 all fields": _ply(_LE + ["element vertex 1"] + _CORE + [f"property float f_dc_{c}" for c in range(3)]
+                           + [f"property float f_rest_{k}" for k in range(9)]
+                           + [f"property float rot_{k}" for k in range(4)] + ["end_header"], 4 * 20),
+    "rest not a multiple of 3": _ply(_LE + ["element vertex 1"] + _CORE
+                                     + [f"property float f_rest_{k}" for k in range(10)] + ["end_header"], 56),
+    "rest with no degree": _ply(_LE + ["element vertex 1"] + _CORE
+                                + [f"property float f_rest_{k}" for k in range(12)] + ["end_header"], 64),
+    "rest with a gap": _ply(_LE + ["element vertex 1"] + _CORE
+                            + [f"property float f_rest_{k}" for k in (0, 1, 2, 4)] + ["end_header"], 32),
+    "later element resets the count": _ply(_LE + ["element vertex 2"] + _CORE
+                                           + ["element face 0", "end_header"], 32),
+    "element with no count keeps it": _ply(_LE + ["element vertex 2"] + _CORE + ["element extra", "end_header"], 32),
+    "count with trailing letters": _ply(_LE + ["element vertex 2abc"] + _CORE + ["end_header"], 16),
+    "count that is not a number": _ply(_LE + ["element vertex abc"] + _CORE + ["end_header"], 32),
+    "list property": _ply(_LE + ["element vertex 1"] + _CORE + ["property list uchar int idx", "end_header"], 16),
+    "float32 and an unnamed property": _ply(_LE + ["element vertex 1", "property float32 x", "property float",
+                                                   "property float y", "property float z",
+                                                   "property float opacity", "end_header"], 20),
+    "duplicate names": _ply(_LE + ["element vertex 1"] + _CORE + ["property float x", "end_header"], 20),
+    "carriage returns and tabs": ("ply\r\nformat binary_little_endian 1.0\r\n\telement  vertex\t1\r\n"
+                                  + "".join(f" {p}\r\n" for p in _CORE) + "end_header\n").encode() + b"\x00" * 16,
+    "end_header inside a line": _ply(_LE + ["element vertex 1"] + _CORE[:3] + ["property float opacityend_header"],
+                                     16),
+    "format named only in the body": _ply(["ply", "format ascii 1.0", "element vertex 1"] + _CORE
+                                          + ["end_header"], 0) + b"binary_little_endian",
+    "one byte short": _ply(_LE + ["element vertex 2"] + _CORE + ["end_header"], 31),
+    "no magic": _ply(["plx"] + _LE[1:] + ["element vertex 1"] + _CORE + ["end_header"], 16),
+    "no end_header": _ply(_LE + ["element vertex 1"] + _CORE, 16),
+}
+
+
+@pytest.mark.parametrize("case", sorted(_HEADERS))
+def test_ply_header_rules_match_the_reference_reader(case, tmp_path):
+    """splat_ply_info accepts exactly the headers the engine's own
+    readGaussianPly accepts (reference build), with the same count and SH
+    degree, and rejects the others with the engine's message."""
+    import ctypes as C
+
+    import oracle as O
+    if not O.ref_available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    path = tmp_path / "case.ply"
+    path.write_bytes(_HEADERS[case])
+    n, deg = C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(256)
+    ref_ok = O.ref().ref_read_ply(str(path).encode(), C.byref(n), C.byref(deg), None, err, 256) == 1
+    if ref_ok:
+        assert S.ply_info(str(path)) == (n.value, deg.value)
+    else:
+        with pytest.raises(S.SplatError) as e:
+            S.ply_info(str(path))
+        assert str(e.value).endswith(": " + err.value.decode())
+
+
+def test_ply_header_count_that_wraps_is_truncated(tmp_path):
+    """A count of -1 reads as 2^64 - 1; the engine's size check
+    (GaussianCloud.cpp:372-374) wraps round and its reader then aborts in
+    reserve(), so there is no reference result: the library reports a
+    truncated body."""
+    path = tmp_path / "wrap.ply"
+    path.write_bytes(_ply(_LE + ["element vertex -1"] + _CORE + ["end_header"], 32))
+    with pytest.raises(S.SplatError) as e:
+        S.ply_info(str(path))
+    assert "3DGS PLY body is truncated" in str(e.value)
+
+
 def download(ctx):
     c = ctx.download_cloud()
     return c
