@@ -280,6 +280,19 @@ struct SplatStageResult {
   bool cancelled = false;
   std::string note;
 };
+/// The stage's seed set (Pipeline.cpp:1860-1876): with more points than a
+/// quarter of the gaussian budget, every step-th point, step = ceil(points /
+/// budget), a missing colour filled with grey (128, 128, 128); otherwise the
+/// points and colours as given.
+void splatStageSeedPoints(const std::vector<std::array<float, 3>>& points,
+                          const std::vector<std::array<uint8_t, 3>>& colors, int max_gaussians,
+                          std::vector<std::array<float, 3>>* seed_points,
+                          std::vector<std::array<uint8_t, 3>>* seed_colors);
+/// The stage's training configuration (Pipeline.cpp:1921-1926): the
+/// defaults, with the iteration count (at least 1), SH degree and budget of
+/// `settings`, densify until half way, opacity reset every third and an SH
+/// band every eighth of the run (each at least 1).
+SplatTrainingConfig splatStageTrainingConfig(const SplatStageSettings& settings);
 bool runSplatStageOnGpu(const std::vector<std::array<float, 3>>& points,
                         const std::vector<std::array<uint8_t, 3>>& colors, std::vector<SplatTrainingView> views,
                         const SplatStageSettings& settings, const std::function<bool()>& cancelled,

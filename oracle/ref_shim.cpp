@@ -13,7 +13,10 @@
 #include "parallax/splat/SplatRaster.h"
 #include "parallax/splat/SplatTrainer.h"
 
+#include <algorithm>
+#include <array>
 #include <chrono>
+#include <string>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -311,6 +314,81 @@ int ref_init_from_points(int n, const float* xyz, const unsigned char* rgb, int 
   out->sh_degree = cloud.sh_degree;
   fromCloud(cloud, out);
   return out->n;
+}
+
+/* The splat stage's seeding (Pipeline.cpp:1860-1876), restated here because
+ * Pipeline.cpp builds only with the whole engine: with more than a quarter of
+ * max_gaussians points, every step-th point (step = ceil(n / budget)), grey
+ * where the point has no colour; then the engine's own
+ * initializeFromPointCloud at the stage's SH degree and default opacity.
+ * n_colors colours are given (0 <= n_colors <= n). `out` sized for n. */
+int ref_stage_seed(int n, const float* xyz, int n_colors, const unsigned char* rgb, int max_gaussians, int sh_degree,
+                   ref_cloud* out) {
+  std::vector<Vec3f> points(static_cast<size_t>(n));
+  std::vector<std::array<uint8_t, 3>> colors(static_cast<size_t>(n_colors));
+  for (int i = 0; i < n; ++i) points[static_cast<size_t>(i)] = Vec3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  for (int i = 0; i < n_colors; ++i) colors[static_cast<size_t>(i)] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
+  const size_t seed_budget = std::max<size_t>(1, max_gaussians / 4);
+  if (points.size() > seed_budget) {
+    const size_t step = (points.size() + seed_budget - 1) / seed_budget;
+    std::vector<Vec3f> kept_points;
+    std::vector<std::array<uint8_t, 3>> kept_colors;
+    for (size_t i = 0; i < points.size(); i += step) {
+      kept_points.push_back(points[i]);
+      kept_colors.push_back(i < colors.size() ? colors[i] : std::array<uint8_t, 3>{128, 128, 128});
+    }
+    points.swap(kept_points);
+    colors.swap(kept_colors);
+  }
+  const GaussianCloud cloud = initializeFromPointCloud(points, colors, sh_degree);
+  out->sh_degree = cloud.sh_degree;
+  fromCloud(cloud, out);
+  return out->n;
+}
+
+/* The splat stage's SplatTrainingConfig (Pipeline.cpp:1921-1926) on the
+ * engine's own defaults (SplatTrainer.h:25-94), with validate()'s verdict in
+ * `error` (empty when valid). Returns 1 when valid. */
+int ref_stage_config(int iterations, int sh_degree, int max_gaussians, ref_config* out, char* error, int error_len) {
+  SplatTrainingConfig c;
+  c.iterations = std::max(1, iterations);
+  c.sh_degree = sh_degree;
+  c.max_gaussians = max_gaussians;
+  c.densify_until_iteration = c.iterations / 2;
+  c.opacity_reset_interval = std::max(1, c.iterations / 3);
+  c.sh_increase_interval = std::max(1, c.iterations / 8);
+  out->iterations = c.iterations;
+  out->position_lr_initial = c.position_lr_initial;
+  out->position_lr_final = c.position_lr_final;
+  out->dc_lr = c.dc_lr;
+  out->sh_rest_lr_divider = c.sh_rest_lr_divider;
+  out->opacity_lr = c.opacity_lr;
+  out->scale_lr = c.scale_lr;
+  out->rotation_lr = c.rotation_lr;
+  out->adam_beta1 = c.adam_beta1;
+  out->adam_beta2 = c.adam_beta2;
+  out->adam_epsilon = c.adam_epsilon;
+  out->ssim_weight = c.ssim_weight;
+  out->sh_degree = c.sh_degree;
+  out->sh_increase_interval = c.sh_increase_interval;
+  out->densify_from_iteration = c.densify_from_iteration;
+  out->densify_until_iteration = c.densify_until_iteration;
+  out->densify_interval = c.densify_interval;
+  out->densify_grad_threshold = c.densify_grad_threshold;
+  out->percent_dense = c.percent_dense;
+  out->split_scale_divider = c.split_scale_divider;
+  out->opacity_reset_interval = c.opacity_reset_interval;
+  out->opacity_reset_value = c.opacity_reset_value;
+  out->prune_opacity = c.prune_opacity;
+  out->prune_scale_fraction = c.prune_scale_fraction;
+  out->max_gaussians = c.max_gaussians;
+  out->seed = c.seed;
+  const std::string invalid = c.validate();
+  if (error && error_len > 0) {
+    std::strncpy(error, invalid.c_str(), static_cast<size_t>(error_len - 1));
+    error[error_len - 1] = 0;
+  }
+  return invalid.empty() ? 1 : 0;
 }
 
 /* writeGaussianPly (GaussianCloud.cpp:259-303). Returns 1 on success. */

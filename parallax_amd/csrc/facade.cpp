@@ -633,6 +633,38 @@ bool readSpz(const std::string& path, GaussianCloud* out, std::string* error) {
   return ok;
 }
 
+void splatStageSeedPoints(const std::vector<std::array<float, 3>>& points,
+                          const std::vector<std::array<uint8_t, 3>>& colors, int max_gaussians,
+                          std::vector<std::array<float, 3>>* seed_points,
+                          std::vector<std::array<uint8_t, 3>>* seed_colors) {
+  const size_t budget = std::max<size_t>(1, max_gaussians / 4);
+  if (points.size() <= budget) {
+    *seed_points = points;
+    *seed_colors = colors;
+    return;
+  }
+  const size_t step = (points.size() + budget - 1) / budget;
+  seed_points->clear();
+  seed_colors->clear();
+  for (size_t i = 0; i < points.size(); i += step) {
+    seed_points->push_back(points[i]);
+    seed_colors->push_back(i < colors.size() ? colors[i] : std::array<uint8_t, 3>{128, 128, 128});
+  }
+}
+
+SplatTrainingConfig splatStageTrainingConfig(const SplatStageSettings& settings) {
+  SplatTrainingConfig config;
+  config.iterations = std::max(1, settings.iterations);
+  config.sh_degree = settings.sh_degree;
+  config.max_gaussians = settings.max_gaussians;
+  config.densify_until_iteration = config.iterations / 2;
+  config.opacity_reset_interval = std::max(1, config.iterations / 3);
+  config.sh_increase_interval = std::max(1, config.iterations / 8);
+  config.batch_views = std::max(1, settings.batch_views);
+  config.device = settings.device;
+  return config;
+}
+
 bool runSplatStageOnGpu(const std::vector<std::array<float, 3>>& points,
                         const std::vector<std::array<uint8_t, 3>>& colors, std::vector<SplatTrainingView> views,
                         const SplatStageSettings& settings, const std::function<bool()>& cancelled,
@@ -646,34 +678,13 @@ bool runSplatStageOnGpu(const std::vector<std::array<float, 3>>& points,
   };
   if (points.empty()) return fail("no point cloud to seed the splat field from");
   if (views.size() < 2) return fail("at least two posed views are needed to train splats");
-  // seed with a quarter of the budget (Pipeline.cpp:1860-1876)
-  const size_t seed_budget = std::max<size_t>(1, static_cast<size_t>(settings.max_gaussians) / 4);
-  std::vector<std::array<float, 3>> kept_points;
-  std::vector<std::array<uint8_t, 3>> kept_colors;
-  const std::vector<std::array<float, 3>>* seed_points = &points;
-  const std::vector<std::array<uint8_t, 3>>* seed_colors = &colors;
-  if (points.size() > seed_budget) {
-    const size_t step = (points.size() + seed_budget - 1) / seed_budget;
-    for (size_t i = 0; i < points.size(); i += step) {
-      kept_points.push_back(points[i]);
-      kept_colors.push_back(i < colors.size() ? colors[i] : std::array<uint8_t, 3>{128, 128, 128});
-    }
-    seed_points = &kept_points;
-    seed_colors = &kept_colors;
-  }
+  std::vector<std::array<float, 3>> seed_points;
+  std::vector<std::array<uint8_t, 3>> seed_colors;
+  splatStageSeedPoints(points, colors, settings.max_gaussians, &seed_points, &seed_colors);
   std::string init_error;
-  GaussianCloud cloud = initializeFromPointCloud(*seed_points, *seed_colors, settings.sh_degree, 0.1f, &init_error);
+  GaussianCloud cloud = initializeFromPointCloud(seed_points, seed_colors, settings.sh_degree, 0.1f, &init_error);
   if (cloud.empty()) return fail(init_error.empty() ? "the point cloud produced no gaussians" : init_error);
-  // the stage's schedule (Pipeline.cpp:1920-1926)
-  SplatTrainingConfig config;
-  config.iterations = std::max(1, settings.iterations);
-  config.sh_degree = settings.sh_degree;
-  config.max_gaussians = settings.max_gaussians;
-  config.densify_until_iteration = config.iterations / 2;
-  config.opacity_reset_interval = std::max(1, config.iterations / 3);
-  config.sh_increase_interval = std::max(1, config.iterations / 8);
-  config.batch_views = std::max(1, settings.batch_views);
-  config.device = settings.device;
+  const SplatTrainingConfig config = splatStageTrainingConfig(settings);
   const std::string invalid = config.validate();
   if (!invalid.empty()) return fail(invalid);
   // cancel and time budget through the per-iteration callback (:1929-1954)

@@ -11,6 +11,8 @@ Each fixture holds the inputs and the reference's outputs:
                  including one with densification on;
   init_points.npz  initializeFromPointCloud on seeded point sets (with and
                  without colours);
+  stage.npz      the splat stage's seed subset and seeded cloud, and its
+                 derived training configuration with validate()'s verdict;
   cloud_sh2.ply + ply_sh2.npz  writeGaussianPly's bytes for a seeded SH2
                  cloud, and readGaussianPly's cloud for a hand-laid PLY
                  (other property order, no normals or f_dc, unnormalised and
@@ -82,6 +84,20 @@ def ref_raster(name, cloud, view, opt_kw, target, ssim_weight=0.2):
     print(f"raster_{name}: {m} projected, loss {loss:.6f}")
 
 
+def ref_image(cloud, view, opt_kw=None):
+    """renderSplats of the reference build: the fixtures' input images come
+    from the reference itself, so regenerating them never depends on the
+    oracle restatement."""
+    R = O.ref()
+    opt = O.default_options(**(opt_kw or {}))
+    k = O.ccloud(cloud)
+    v = O.cview(view)
+    rgb = np.zeros(view.pixels * 3, np.float32)
+    T = np.zeros(view.pixels, np.float32)
+    R.ref_render(C.byref(k.struct), C.byref(v), C.byref(opt), O.ptr(rgb), O.ptr(T))
+    return rgb
+
+
 def ref_densify(name, seed, opacity_mu, max_gaussians, iteration):
     R = O.ref()
     rng = np.random.default_rng(seed)
@@ -125,7 +141,7 @@ def ref_trainer():
               densify_grad_threshold=0.0), 60),
     ):
         truth = syn.truth_cloud()
-        images = [O.render(truth, v)[0] for v in views]
+        images = [ref_image(truth, v) for v in views]
         cfg = O.default_config(iterations=steps, sh_degree=0, sh_increase_interval=0,
                                densify_from_iteration=steps + 1, densify_until_iteration=steps + 1,
                                opacity_reset_interval=0, ssim_weight=0.2)
@@ -180,6 +196,54 @@ def ref_init_points():
             data[f"{tag}_out_{k}"] = a[: a.size if k != "rest" or rs else 0]
         print(f"init_points {tag}: log sigma {out.arrays[1][0]:.6f}")
     np.savez_compressed(os.path.join(HERE, "init_points.npz"), **data)
+
+
+STAGE_SEEDS = (("subset_partial_colours", 5000, 3000, 4000, 3, 51),
+               ("no_subset", 900, 900, 8000, 1, 52),
+               ("subset_no_colours", 2001, 0, 2000, 0, 53))
+STAGE_CONFIGS = ((0, 3, 600000), (1, 3, 600000), (7, 3, 600000), (8, 2, 600000), (100, 3, 1000),
+                 (7000, 3, 600000), (30001, 1, 250000), (500, 4, 600000), (500, 3, 0))
+
+
+def ref_stage():
+    """The splat stage's seeding (subset of a quarter of the budget, grey
+    fill, then initializeFromPointCloud) and its derived training
+    configuration with validate()'s verdict (Pipeline.cpp:1860-1876,
+    1921-1928)."""
+    R = O.ref()
+    R.ref_stage_seed.argtypes = [C.c_int, O.f32p, C.c_int, C.POINTER(C.c_ubyte), C.c_int, C.c_int,
+                                 C.POINTER(O.CCloud)]
+    R.ref_stage_config.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(O.CConfig), C.c_char_p, C.c_int]
+    data = {}
+    for tag, n, n_colours, max_g, deg, seed in STAGE_SEEDS:
+        rng = np.random.default_rng(seed)
+        xyz = (rng.normal(size=(n, 3)) * np.array([2.0, 0.5, 1.5])).astype(np.float32)
+        rgb = rng.integers(0, 256, size=(max(1, n_colours), 3), dtype=np.uint8)
+        rs = 3 * [0, 3, 8, 15][deg]
+        out = empty_ccloud(n, rs)
+        m = R.ref_stage_seed(n, O.ptr(xyz), n_colours, rgb.ctypes.data_as(C.POINTER(C.c_ubyte)), max_g, deg,
+                             C.byref(out.struct))
+        data[f"seed_{tag}_xyz"] = xyz
+        data[f"seed_{tag}_rgb"] = rgb[:n_colours]
+        data[f"seed_{tag}_args"] = np.array([max_g, deg], np.int32)
+        for k, a, width in zip(("pos", "log_scale", "rot", "opacity", "dc", "rest"), out.arrays,
+                               (3, 3, 4, 1, 3, rs)):
+            data[f"seed_{tag}_out_{k}"] = a[: m * width]
+        print(f"stage seed {tag}: {n} points -> {m} gaussians")
+    fields = [f for f, _ in O.CConfig._fields_]
+    rows, verdicts = [], []
+    for it, deg, max_g in STAGE_CONFIGS:
+        cfg = O.CConfig()
+        err = C.create_string_buffer(128)
+        R.ref_stage_config(it, deg, max_g, C.byref(cfg), err, 128)
+        rows.append([float(getattr(cfg, f)) for f in fields])
+        verdicts.append(err.value.decode())
+    data["config_args"] = np.array(STAGE_CONFIGS, np.int64)
+    data["config_fields"] = np.array(fields)
+    data["config_values"] = np.array(rows, np.float64)
+    data["config_verdicts"] = np.array(verdicts)
+    np.savez_compressed(os.path.join(HERE, "stage.npz"), **data)
+    print(f"stage configs: {len(rows)}, verdicts {verdicts}")
 
 
 def ref_ply():
@@ -340,16 +404,17 @@ def main():
                dict(sh_degree=2, background=(0.2, 0.3, 0.4)), fd_target)
     c1 = syn.random_cloud(2000, 0, 0, (-1, -1, 2), (1, 1, 6))
     v1 = syn.pinhole(128, 128, 150.0)
-    t1 = O.render(syn.random_cloud(2000, 1, 0, (-1, -1, 2), (1, 1, 6)), v1)[0]
+    t1 = ref_image(syn.random_cloud(2000, 1, 0, (-1, -1, 2), (1, 1, 6)), v1)
     ref_raster("c1_small_sh0", c1, v1, dict(), t1)
     c3 = syn.random_cloud(1500, 5, 3, (-1, -1, 2), (1, 1, 6))
     v3 = syn.pinhole(128, 80, 160.0)
-    t3 = O.render(syn.random_cloud(1500, 6, 3, (-1, -1, 2), (1, 1, 6)), v3, O.default_options(sh_degree=3))[0]
+    t3 = ref_image(syn.random_cloud(1500, 6, 3, (-1, -1, 2), (1, 1, 6)), v3, dict(sh_degree=3))
     ref_raster("sh3_bg", c3, v3, dict(sh_degree=3, background=(0.1, 0.2, 0.3)), t3)
     ref_densify("prune", 17, -2.0, 560, 3500)
     ref_densify("cap", 23, 1.0, 470, 200)
     ref_trainer()
     ref_init_points()
+    ref_stage()
     ref_ply()
     ref_compact()
 
