@@ -12,24 +12,29 @@
 // planes per channel are written, read and blurred instead of five
 // (dSSIM/dx = B(A) + 2x B(t_var) + y B(t_cov)).
 //
-// Two passes over 64x16-pixel loss tiles of every view of the batch, one
-// 256-thread workgroup each, one channel at a time:
-//   stats: the image/target window (74x26, zero outside the image) goes to
-//          LDS; one row pass over the 26 rows blurs all five statistics
-//          planes (x, y, xx, yy, xy) from a single read of each run's window,
-//          then a column pass per plane down to the tile's 16 rows, every
-//          thread sliding an 11-tap window over a run of 4 outputs in
-//          registers; then
-//          per pixel the SSIM value, the three partial planes (to HBM), and
-//          the L1 terms; per-tile L1 and SSIM sums.
-//   grad:  per partial plane, its 74x26 window to LDS (two buffers, so a
-//          plane costs two workgroup barriers), the same two passes, and the
-//          blurred planes combined into dL/drgb.
-// Each blur tap is one fused multiply-add in tap order -5..5 and the loss
-// value's expressions are the engine's, as the CPU checker evaluates them;
-// dL/drgb differs from the checker's only by the rounding of the plane sum A.
+// Both passes walk down the image. A work unit is a 128-column x 78-row block
+// of one view; its waves walk the block's rows top to bottom, from 5 rows
+// above it to 5 rows below (zero outside the image):
+//   - the row blur of a row reads each lane's 11-tap window from that row,
+//     staged in the wave's own LDS slice (no workgroup barrier);
+//   - the column blur keeps, per lane, the last 11 row-blurred values of each
+//     plane in registers (an 11-slot ring, the row loop unrolled by 11 so
+//     every slot index is static) and blurs them as each new row arrives, so
+//     no row is blurred twice within a unit.
+// stats: three waves per half unit (64 columns), one per channel, one lane
+//        per column: the five statistics planes (x, y, xx, yy, xy) per pixel,
+//        then the SSIM value, the three partial planes (to HBM) and the L1
+//        terms; per-half-unit L1 and SSIM sums in a fixed order.
+// grad:  three waves per unit, one per channel, two columns per lane: the
+//        three partial planes of the channel are blurred and combined into
+//        dL/drgb; the three channels of 8 output rows are assembled in LDS and
+//        written as whole interleaved rows.
+// The blur taps run in tap order -5..5 and the per-pixel SSIM expressions
+// are the engine's, with its divisions by c d, c^2 d and c d^2 taken as
+// products of two hardware reciprocals (1 ulp each); the loss and dL/drgb
+// are held to the float64 evaluation of the engine's loss.
 // Compulsory HBM traffic per pixel: 24 B read and 36 B written by the first
-// pass, 60 B read and 12 B written by the second (halos hit in cache).
+// pass, 60 B read and 12 B written by the second, plus 10 halo rows per 78.
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -37,198 +42,238 @@ namespace splat {
 namespace {
 
 constexpr int kR = 5;
-constexpr int kLW = 64, kLH = 16;         // loss tile, pixels
-constexpr int kWinW = kLW + 2 * kR;       // 74: window columns
-constexpr int kWinH = kLH + 2 * kR;       // 26: window rows
-constexpr int kRun = 4;                   // outputs per thread along a blur
-constexpr int kTaps = kRun + 2 * kR;      // 14 window values per run
-constexpr int kPitch = kWinW + 1;         // 75: odd, conflict-free row walks
-constexpr int kHPitch = kLW + 1;          // 65
-constexpr int kWinPerThread = (kWinH * kWinW + 255) / 256;  // window values each of 256 threads fetches
+constexpr int kTaps = 2 * kR + 1;         // 11
+constexpr int kUnitW = 128;               // columns per unit
+constexpr int kUnitH = 78;                // output rows per unit: 88 input rows, eight groups of 11
+constexpr int kStatWaves = 3;             // stats: one wave per channel over half a unit (64 columns)
+constexpr int kGradWaves = 3;             // grad: 1 wave per channel, 2 columns per lane
+constexpr int kGradPitch = 192;           // 138 staged columns per grad wave (three per lane written)
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 
-__device__ __forceinline__ int find_view_lt(const ViewDesc* views, int n_views, int lt) {
-  int v = 0;
-  for (int k = 1; k < n_views; ++k)
-    if (views[k].lt_base <= lt) v = k;
-  return v;
+// Work units of a view: 128-column x 78-row blocks. The statistics pass runs
+// one workgroup per half unit (64 columns); its partial sums and the views'
+// lt_base count half units.
+__host__ __device__ __forceinline__ int loss_units(int width, int height) {
+  return ((width + kUnitW - 1) / kUnitW) * ((height + kUnitH - 1) / kUnitH);
+}
+__host__ __device__ __forceinline__ int loss_halves(int width, int height) { return 2 * loss_units(width, height); }
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// out[k] = sum_i kernel[i] * win[k + i] for k < kRun, one fused multiply-add
-// per tap in tap order.
-__device__ __forceinline__ void blur_run(const float (&win)[kTaps], const float* kernel, float (&out)[kRun]) {
-#pragma unroll
-  for (int k = 0; k < kRun; ++k) {
-    float s = 0.f;
-#pragma unroll
-    for (int i = 0; i < 2 * kR + 1; ++i) s = __builtin_fmaf(kernel[i], win[k + i], s);
-    out[k] = s;
-  }
-}
-
-struct LossTile {
-  int ox, oy;  // first pixel
+struct Unit {
+  int block;     // index into the block sums (statistics pass: half units)
+  int sx, by;    // first column, first output row
+  int rows;      // output rows (<= kUnitH)
   const ViewDesc* V;
 };
 
-__device__ __forceinline__ LossTile loss_tile_of(const LossParams& p, const ViewDesc* views) {
-  const int lt = xcd_remap(blockIdx.x, p.total_tiles);
-  const int v = find_view_lt(views, p.n_views, lt);
+// The block's unit, or with `half` its half unit (64 columns).
+__device__ __forceinline__ Unit unit_of(const LossParams& p, const ViewDesc* views, bool half) {
+  Unit u;
+  const int n = half ? p.total_tiles : p.total_tiles / 2;
+  u.block = xcd_remap(blockIdx.x, n);
+  int v = 0;
+  for (int k = 1; k < p.n_views; ++k)
+    if ((half ? views[k].lt_base : views[k].lt_base / 2) <= u.block) v = k;
   const ViewDesc& V = views[v];
-  const int local = lt - V.lt_base;
-  const int ltcols = (V.width + kLW - 1) / kLW;
-  LossTile t;
-  t.ox = (local % ltcols) * kLW;
-  t.oy = (local / ltcols) * kLH;
-  t.V = &V;
-  return t;
+  const int local = u.block - (half ? V.lt_base : V.lt_base / 2);
+  const int unit = half ? local >> 1 : local;
+  const int strips = (V.width + kUnitW - 1) / kUnitW;
+  u.sx = (unit % strips) * kUnitW + (half ? 64 * (local & 1) : 0);
+  u.by = (unit / strips) * kUnitH;
+  u.rows = min(kUnitH, V.height - u.by);
+  u.V = &V;
+  return u;
 }
 
-// Where this thread's window elements (tid + 256 r, row-major over the
-// 74 x 26 window) come from: the pixel offset within the view (0 where the
-// element lies outside the image or past the window), and a bit per element
-// that is set when it lies inside both.
-struct WindowMap {
-  int off[kWinPerThread];
-  uint32_t inside;
-};
-
-__device__ __forceinline__ WindowMap window_map(const LossTile& L, const ViewDesc& V) {
-  WindowMap m;
-  m.inside = 0u;
-#pragma unroll
-  for (int r = 0; r < kWinPerThread; ++r) {
-    const int idx = (int)threadIdx.x + r * 256;
-    const int hy = idx / kWinW, hx = idx % kWinW;
-    const int qx = L.ox + hx - kR, qy = L.oy + hy - kR;
-    const bool in = idx < kWinH * kWinW && qx >= 0 && qx < V.width && qy >= 0 && qy < V.height;
-    m.off[r] = in ? qy * V.width + qx : 0;
-    m.inside |= in ? 1u << r : 0u;
-  }
-  return m;
+// Value of channel c of an interleaved RGB image at (row, col), zero outside.
+// The load is unconditional, from the nearest pixel inside the image, so a
+// row's loads can be issued rows ahead with no branch around them.
+__device__ __forceinline__ float pixel_or_zero(const float* img, int width, int height, int row, int col, int c) {
+  const bool in = row >= 0 && row < height && col >= 0 && col < width;
+  const int rr = min(max(row, 0), height - 1), cc = min(max(col, 0), width - 1);
+  const float v = img[3 * ((long long)rr * width + cc) + c];
+  return in ? v : 0.f;
 }
 
-// Row pass of one plane over the window's 26 rows into hbuf[26][65]: plane 0
-// blurs a, 1 blurs b, 2 a*a, 3 b*b, 4 a*b (a, b: LDS windows).
-__device__ __forceinline__ void row_pass(int plane, const float (*a)[kPitch], const float (*b)[kPitch],
-                                         const float* kernel, float (*hbuf)[kHPitch]) {
-  // a wave covers 4 window rows x 16 runs: with both pitches odd mod 4 the
-  // 64 lanes of every LDS access land in 64 different banks
-  for (int r = (int)(threadIdx.x >> 4); r < kWinH; r += (int)(blockDim.x >> 4)) {
-    const int j0 = (int)(threadIdx.x & 15) * kRun;
-    float win[kTaps], o[kRun];
-#pragma unroll
-    for (int i = 0; i < kTaps; ++i) {
-      const float x = a[r][j0 + i];
-      if (plane == 0) {
-        win[i] = x;
-      } else {
-        const float y = b[r][j0 + i];
-        win[i] = plane == 1 ? y : (plane == 2 ? x * x : (plane == 3 ? y * y : x * y));
-      }
-    }
-    blur_run(win, kernel, o);
-#pragma unroll
-    for (int k = 0; k < kRun; ++k) hbuf[r][j0 + k] = o[k];
-  }
+// Rows a wave's loads run ahead of its compositing of them: enough rows in
+// flight to cover an HBM round trip.
+constexpr int kStatAhead = 4;
+constexpr int kGradAhead = 3;
+
+// A raw buffer over `bytes` bytes at `base`: stores at offsets past the end
+// are dropped by the hardware, so a lane is switched off by giving it the
+// offset kOff, with no branch around the store.
+constexpr uint32_t kOff = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void store_or_drop(__amdgpu_buffer_rsrc_t r, uint32_t offset, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)offset, 0, 0);
+}
+// A load of the float at byte `lane_offset` + `row_offset`: the hardware's
+// range check covers the per-lane offset only (a lane offset of kOff reads
+// 0), so the wave-uniform row offset must stay inside the buffer.
+__device__ __forceinline__ float load_or_zero(__amdgpu_buffer_rsrc_t r, uint32_t lane_offset, uint32_t row_offset) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_offset, (int)row_offset, 0));
 }
 
-// The row passes of all five statistics planes at once: each run's window of
-// a and b is read from LDS once.
-__device__ __forceinline__ void row_pass5(const float (*a)[kPitch], const float (*b)[kPitch], const float* kernel,
-                                          float (*hbuf)[kWinH][kHPitch]) {
-  for (int r = (int)(threadIdx.x >> 4); r < kWinH; r += (int)(blockDim.x >> 4)) {
-    const int j0 = (int)(threadIdx.x & 15) * kRun;
-    float x[kTaps], y[kTaps], win[kTaps], o[kRun];
-#pragma unroll
-    for (int i = 0; i < kTaps; ++i) {
-      x[i] = a[r][j0 + i];
-      y[i] = b[r][j0 + i];
-    }
-#pragma unroll
-    for (int pl = 0; pl < 5; ++pl) {
-#pragma unroll
-      for (int i = 0; i < kTaps; ++i)
-        win[i] = pl == 0 ? x[i] : (pl == 1 ? y[i] : (pl == 2 ? x[i] * x[i] : (pl == 3 ? y[i] * y[i] : x[i] * y[i])));
-      blur_run(win, kernel, o);
-#pragma unroll
-      for (int k = 0; k < kRun; ++k) hbuf[pl][r][j0 + k] = o[k];
-    }
-  }
-}
-
-// Column pass: this thread's 4 outputs (column j, rows r0..r0+3 of the tile).
-__device__ __forceinline__ void col_pass(const float (*hbuf)[kHPitch], const float* kernel, int j, int r0,
-                                         float (&out)[kRun]) {
-  float win[kTaps];
-#pragma unroll
-  for (int i = 0; i < kTaps; ++i) win[i] = hbuf[r0 + i][j];
-  blur_run(win, kernel, out);
-}
-
-__global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc* __restrict__ views,
-                                                    const float* __restrict__ rgb,
-                                                    const float* __restrict__ targets, float* __restrict__ dL,
-                                                    float* __restrict__ terms, long long total_px,
-                                                    float* __restrict__ block_sums) {
-  __shared__ float s_x[kWinH][kPitch];
-  __shared__ float s_y[kWinH][kPitch];
-  __shared__ float s_h[5][kWinH][kHPitch];
-  __shared__ float s_red[2][4];
-  const LossTile L = loss_tile_of(p, views);
-  const ViewDesc& V = *L.V;
-  const int tid = threadIdx.x;
-  const int j = tid & (kLW - 1), r0 = (tid / kLW) * kRun;  // this thread's column and rows
-  const long long pixels = (long long)V.width * V.height;
+// ---------------------------------------------------------------- stats --
+// The loop body of a row is free of branches (loads from clamped addresses,
+// selects, stores the hardware drops), so the compiler can keep the rows'
+// loads in flight across it.
+template <bool kSsim>
+__global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_loss_stats(LossParams p, const ViewDesc* __restrict__ views,
+                                                                const float* __restrict__ rgb,
+                                                                const float* __restrict__ targets,
+                                                                float* __restrict__ dL, float* __restrict__ terms,
+                                                                long long total_px, float* __restrict__ block_sums) {
+  // two staged rows per wave (74 columns; every lane writes two slots of each)
+  __shared__ float s_x[kStatWaves][2 * 128];
+  __shared__ float s_y[kStatWaves][2 * 128];
+  __shared__ float s_red[2][kStatWaves];
+  const Unit U = unit_of(p, views, true);
+  const ViewDesc& V = *U.V;
+  // the wave index read from lane 0 is known to be uniform: buffer
+  // descriptors and plane bases built from it stay in scalar registers
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = wave;                            // channel
+  const int x0 = U.sx;                           // first output column
+  const int col = x0 + lane;                     // this lane's column
+  // the view's fields in registers (the buffer stores below are opaque, so
+  // fields read through V would be re-read after each of them)
+  const int W = V.width, H = V.height, by = U.by, rows = U.rows;
+  const bool col_ok = col < W;
+  const long long pixels = (long long)W * H;
   const float inverse = 1.f / (float)(pixels * 3);
-  const float w = p.ssim_weight;
   const float* img = rgb + 3 * V.pixel_base;
   const float* tgt = targets + V.target_base;
-  const int gx = L.ox + j;
+  float* sx_ = s_x[wave];
+  float* sy_ = s_y[wave];
   float l1 = 0.f, ssim = 0.f;
-  // each channel's image/target window, prefetched into registers while the
-  // previous channel is blurred
-  const WindowMap win = window_map(L, V);
-  float px_[kWinPerThread], py_[kWinPerThread];
-  auto fetch = [&](int c) {
-#pragma unroll
-    for (int r = 0; r < kWinPerThread; ++r) {
-      const float x = img[3 * win.off[r] + c], y = tgt[3 * win.off[r] + c];
-      const bool in = (win.inside >> r) & 1u;
-      px_[r] = in ? x : 0.f;
-      py_[r] = in ? y : 0.f;
-    }
-  };
-  fetch(0);
-  for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int r = 0; r < kWinPerThread; ++r) {
-      const int idx = tid + r * 256;
-      if (idx < kWinH * kWinW) {
-        s_x[idx / kWinW][idx % kWinW] = px_[r];
-        s_y[idx / kWinW][idx % kWinW] = py_[r];
-      }
-    }
-    __syncthreads();
-    if (c + 1 < 3) fetch(c + 1);
-    float st[5][kRun];
-    if (w > 0.f) {
-      row_pass5(s_x, s_y, p.kernel, s_h);
-      __syncthreads();
-#pragma unroll
-      for (int pl = 0; pl < 5; ++pl) col_pass(s_h[pl], p.kernel, j, r0, st[pl]);
-    }
-#pragma unroll
-    for (int k = 0; k < kRun; ++k) {
-      const int gy = L.oy + r0 + k;
-      if (gx >= V.width || gy >= V.height) continue;
-      const long long pixel = V.pixel_base + (long long)gy * V.width + gx;
-      const float x = s_x[r0 + k + kR][j + kR], y = s_y[r0 + k + kR][j + kR];
+
+  if (!kSsim) {
+    // L1 only: dL/drgb = sign(x - y) / (3 W H) per pixel, written here
+    const __amdgpu_buffer_rsrc_t rdl = buffer_of(dL + 3 * V.pixel_base, (uint32_t)(pixels * 12));
+    for (int o = 0; o < rows; ++o) {
+      const int row = by + o;
+      const float x = pixel_or_zero(img, W, H, row, col, c);
+      const float y = pixel_or_zero(tgt, W, H, row, col, c);
       const float difference = x - y;
-      l1 += fabsf(difference);
-      if (w > 0.f) {
-        const float mu_x = st[0][k], mu_y = st[1][k], exx = st[2][k], eyy = st[3][k], exy = st[4][k];
+      l1 += col_ok ? fabsf(difference) : 0.f;
+      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+      store_or_drop(rdl, col_ok ? (uint32_t)((3 * ((long long)row * W + col) + c) * 4) : kOff,
+                    sign * inverse);
+    }
+  } else {
+    // staged columns x0 - 5 + lane and x0 + 59 + lane of an input row, loaded
+    // kStatAhead rows before the row is staged (q[0] is the next row to stage)
+    struct Row {
+      float x0, y0, x1, y1;
+    };
+    Row q[kStatAhead];
+    // lanes read columns x0 - 5 + lane and x0 + 59 + lane (kOff outside the
+    // image); rows outside it read row 0 and are zeroed by a uniform select
+    const __amdgpu_buffer_rsrc_t rimg = buffer_of(img, (uint32_t)(pixels * 12));
+    const __amdgpu_buffer_rsrc_t rtgt = buffer_of(tgt, (uint32_t)(pixels * 12));
+    const int cola = x0 - kR + lane, colb = cola + 64;
+    const uint32_t la = cola >= 0 && cola < W ? (uint32_t)((3 * cola + c) * 4) : kOff;
+    const uint32_t lb = colb >= 0 && colb < W ? (uint32_t)((3 * colb + c) * 4) : kOff;
+    auto fetch = [&](int r) {
+      const int row = by - kR + r;
+      const bool row_ok = row >= 0 && row < H;
+      const uint32_t ro = row_ok ? (uint32_t)(row * W * 12) : 0u;
+      const uint32_t a = row_ok ? la : kOff, b = row_ok ? lb : kOff;  // rows outside read 0
+      Row f;
+      f.x0 = load_or_zero(rimg, a, ro);
+      f.y0 = load_or_zero(rtgt, a, ro);
+      f.x1 = load_or_zero(rimg, b, ro);
+      f.y1 = load_or_zero(rtgt, b, ro);
+      return f;
+    };
+    const uint32_t plane_bytes = (uint32_t)(pixels * 4);
+    const __amdgpu_buffer_rsrc_t rt0 = buffer_of(terms + (long long)(3 * c) * total_px + V.pixel_base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rt1 = buffer_of(terms + (long long)(3 * c + 1) * total_px + V.pixel_base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rt2 = buffer_of(terms + (long long)(3 * c + 2) * total_px + V.pixel_base, plane_bytes);
+    float ring[5][kTaps];  // row-blurred x, y, xx, yy, xy of the last 11 rows
+
+    // one input row: stage it, take the lane's window, add the lane's own
+    // pixel to L1 (input rows 5 .. rows + 4 are the unit's own rows), and
+    // row-blur the five statistics into ring slot `slot`
+    auto stage = [&](int r, const Row& f) {
+      float* bx = sx_ + (r & 1) * 128;
+      float* by_ = sy_ + (r & 1) * 128;
+      bx[lane] = f.x0;
+      by_[lane] = f.y0;
+      bx[64 + lane] = f.x1;
+      by_[64 + lane] = f.y1;
+    };
+    auto row_in = [&](int r, int slot) {
+      // one scheduling region per row: the prefetch queue, not the
+      // scheduler, decides how far loads run ahead
+      __builtin_amdgcn_sched_barrier(0);
+      // row r was staged (in buffer r & 1) a step ago: its writes are long
+      // done, and the stage of row r + 1 below goes to the other buffer,
+      // whose row r - 1 was read a step ago
+      wave_sync_lds();
+      float xw[kTaps], yw[kTaps];
+#pragma unroll
+      for (int i = 0; i < kTaps; ++i) {
+        xw[i] = sx_[(r & 1) * 128 + lane + i];
+        yw[i] = sy_[(r & 1) * 128 + lane + i];
+      }
+      stage(r + 1, q[0]);
+#pragma unroll
+      for (int k = 0; k + 1 < kStatAhead; ++k) q[k] = q[k + 1];
+      q[kStatAhead - 1] = fetch(r + 1 + kStatAhead);  // past the band: clamped loads, never staged
+      const bool own = col_ok && r >= kR && r < rows + kR;
+      l1 += own ? fabsf(xw[kR] - yw[kR]) : 0.f;
+      // k x and k y once per tap: the x, y sums are their sums, and the
+      // xx, yy, xy sums are one fused multiply-add each per tap
+      float h[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int i = 0; i < kTaps; ++i) {
+        const float kx = p.kernel[i] * xw[i], ky = p.kernel[i] * yw[i];
+        h[0] += kx;
+        h[1] += ky;
+        h[2] = __builtin_fmaf(kx, xw[i], h[2]);
+        h[3] = __builtin_fmaf(ky, yw[i], h[3]);
+        h[4] = __builtin_fmaf(kx, yw[i], h[4]);
+      }
+#pragma unroll
+      for (int pl = 0; pl < 5; ++pl) ring[pl][slot] = h[pl];
+    };
+
+    stage(0, fetch(0));
+#pragma unroll
+    for (int k = 0; k < kStatAhead; ++k) q[k] = fetch(k + 1);
+    // groups of 11 input rows, row r into ring slot r % 11; input rows 10 ..
+    // rows + 9 complete output rows 0 .. rows - 1, rows past the unit (the
+    // last band of a view) are computed from zeros and masked
+    const int groups = (rows + 2 * kR + kTaps - 1) / kTaps;
+    for (int g = 0; g < groups; ++g) {
+#pragma unroll
+      for (int j = 0; j < kTaps; ++j) {
+        const int r = g * kTaps + j;
+        row_in(r, j);
+        const int o = r - 2 * kR;  // the output row this input row completes
+        if (o < 0) continue;       // the first group's first ten rows only fill the ring
+        const bool out_ok = col_ok && o < rows;
+        // the column blur of the ring, oldest row first
+        float st[5];
+#pragma unroll
+        for (int pl = 0; pl < 5; ++pl) {
+          float s = 0.f;
+#pragma unroll
+          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][(j + 1 + i) % kTaps], s);
+          st[pl] = s;
+        }
+        const float mu_x = st[0], mu_y = st[1], exx = st[2], eyy = st[3], exy = st[4];
         const float var_x = exx - mu_x * mu_x;
         const float var_y = eyy - mu_y * mu_y;
         const float cov = exy - mu_x * mu_y;
@@ -236,132 +281,225 @@ __global__ __launch_bounds__(256) void k_loss_stats(LossParams p, const ViewDesc
         const float b = 2.f * cov + kC2;
         const float cc = mu_x * mu_x + mu_y * mu_y + kC1;
         const float d = var_x + var_y + kC2;
-        ssim += (a * b) / (cc * d);
-        const float term_mu = inverse * (2.f * mu_y * b / (cc * d) - 2.f * mu_x * a * b / (cc * cc * d));
-        const float term_var = inverse * (-(a * b) / (cc * d * d));
-        const float term_cov = inverse * (2.f * a / (cc * d));
-        float* tp = terms + (long long)(c * 3) * total_px + pixel;
-        tp[0] = (term_mu - 2.f * term_var * mu_x) - term_cov * mu_y;
-        tp[total_px] = term_var;
-        tp[2 * total_px] = term_cov;
-      } else {
-        const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
-        dL[3 * pixel + c] = sign * inverse;
+        // cc >= C1 and d ~ C2 > 0: two hardware reciprocals (1 ulp) stand in
+        // for the engine's five divisions
+        const float rc = __builtin_amdgcn_rcpf(cc), rd = __builtin_amdgcn_rcpf(d);
+        const float rcd = rc * rd;
+        const float value = (a * b) * rcd;  // the pixel's SSIM
+        ssim += out_ok ? value : 0.f;
+        const float term_mu = inverse * (2.f * mu_y * b * rcd - 2.f * mu_x * value * rc);
+        const float term_var = inverse * (-value * rd);
+        const float term_cov = inverse * (2.f * a * rcd);
+        const uint32_t off = out_ok ? (uint32_t)(((by + o) * W + col) * 4) : kOff;
+        store_or_drop(rt0, off, (term_mu - 2.f * term_var * mu_x) - term_cov * mu_y);
+        store_or_drop(rt1, off, term_var);
+        store_or_drop(rt2, off, term_cov);
       }
+      // the three waves read the same interleaved RGB lines (each takes one
+      // channel): kept within a group of rows of each other, the lines one
+      // wave brings into the cache are still there for the others
+      __syncthreads();
     }
-    __syncthreads();
   }
-  // fixed-order workgroup sums
-  float a = l1, b = ssim;
+  // fixed-order sums: wave butterflies, then the waves in index order
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_xor(a, off, 64);
-    b += __shfl_xor(b, off, 64);
+    l1 += __shfl_xor(l1, off, 64);
+    ssim += __shfl_xor(ssim, off, 64);
   }
-  if ((tid & 63) == 0) {
-    s_red[0][tid >> 6] = a;
-    s_red[1][tid >> 6] = b;
+  if (lane == 0) {
+    s_red[0][wave] = l1;
+    s_red[1][wave] = ssim;
   }
   __syncthreads();
-  if (tid == 0) {
-    const int lt = xcd_remap(blockIdx.x, p.total_tiles);
-    block_sums[2 * lt] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
-    block_sums[2 * lt + 1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
+  if (threadIdx.x == 0) {
+    float a = 0.f, b = 0.f;
+#pragma unroll
+    for (int k = 0; k < kStatWaves; ++k) {
+      a += s_red[0][k];
+      b += s_red[1][k];
+    }
+    block_sums[2 * U.block] = a;
+    block_sums[2 * U.block + 1] = b;
   }
 }
 
-__global__ __launch_bounds__(256) void k_loss_grad(LossParams p, const ViewDesc* __restrict__ views,
-                                                   const float* __restrict__ rgb, const float* __restrict__ targets,
-                                                   const float* __restrict__ terms, long long total_px,
-                                                   float* __restrict__ dL) {
-  // plane q stages through buffer q & 1: the barrier after a plane's window
-  // store also orders the previous plane's column reads before this plane's
-  // row pass overwrites the other buffer
-  __shared__ float s_t[2][kWinH][kPitch];
-  __shared__ float s_h[2][kWinH][kHPitch];
-  const LossTile L = loss_tile_of(p, views);
-  const ViewDesc& V = *L.V;
-  const int tid = threadIdx.x;
-  const int j = tid & (kLW - 1), r0 = (tid / kLW) * kRun;
-  const long long pixels = (long long)V.width * V.height;
+// ----------------------------------------------------------------- grad --
+__global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void k_loss_grad(LossParams p, const ViewDesc* __restrict__ views,
+                                                               const float* __restrict__ rgb,
+                                                               const float* __restrict__ targets,
+                                                               const float* __restrict__ terms, long long total_px,
+                                                               float* __restrict__ dL) {
+  __shared__ __align__(16) float s_t[kGradWaves][3][kGradPitch];  // 138 staged columns; every lane writes three
+  __shared__ __align__(16) float s_dl[kTaps][kUnitW * 3];          // one group of output rows, interleaved
+  const Unit U = unit_of(p, views, false);
+  const ViewDesc& V = *U.V;
+  const int lane = threadIdx.x & 63;
+  const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave = channel, wave-uniform
+  const int ca = U.sx + 2 * lane;                          // this lane's two columns: ca, ca + 1
+  // the view's fields in registers (the buffer stores below are opaque)
+  const int W = V.width, H = V.height, by = U.by, sx = U.sx, rows = U.rows;
+  const long long pixels = (long long)W * H;
   const float inverse = 1.f / (float)(pixels * 3);
   const float w = p.ssim_weight;
   const float* img = rgb + 3 * V.pixel_base;
   const float* tgt = targets + V.target_base;
-  const int gx = L.ox + j;
-  // the 9 partial planes' windows, each prefetched into registers while the
-  // previous one is blurred; the window's element offsets (0 outside the
-  // image, then masked) are the same for every plane, so the loads need no
-  // branches
-  const WindowMap win = window_map(L, V);
-  float pre[kWinPerThread];
-  auto fetch = [&](int q) {
-    const float* plane = terms + (long long)q * total_px + V.pixel_base;
-#pragma unroll
-    for (int r = 0; r < kWinPerThread; ++r) {
-      const float x = plane[win.off[r]];
-      pre[r] = (win.inside >> r) & 1u ? x : 0.f;
-    }
+  const float* plane0 = terms + (long long)(3 * c) * total_px + V.pixel_base;
+  const int valid_floats = 3 * min(kUnitW, W - sx);
+  const __amdgpu_buffer_rsrc_t rdl = buffer_of(dL + 3 * V.pixel_base, (uint32_t)(pixels * 12));
+  float (*st)[kGradPitch] = s_t[c];
+
+  // for input row r: the three planes at staged columns sx - 5 + lane + 64 k,
+  // and x and y of this lane's two pixels in output row r - 10; loaded
+  // kGradAhead rows before they are used (clamped loads, no branches)
+  struct Row {
+    float v[3][3];
+    float ox[2], oy[2];
   };
-  fetch(0);
+  Row q[kGradAhead];
+  const uint32_t plane_bytes = (uint32_t)(pixels * 4);
+  const __amdgpu_buffer_rsrc_t rp0 = buffer_of(plane0, plane_bytes);
+  const __amdgpu_buffer_rsrc_t rp1 = buffer_of(plane0 + total_px, plane_bytes);
+  const __amdgpu_buffer_rsrc_t rp2 = buffer_of(plane0 + 2 * total_px, plane_bytes);
+  const __amdgpu_buffer_rsrc_t rimg = buffer_of(img, (uint32_t)(pixels * 12));
+  const __amdgpu_buffer_rsrc_t rtgt = buffer_of(tgt, (uint32_t)(pixels * 12));
+  uint32_t lp[3];  // plane columns sx - 5 + lane + 64 k (kOff outside the image)
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    float d[kRun], xs[kRun], ys[kRun];
+  for (int k = 0; k < 3; ++k) {
+    const int colk = sx - kR + lane + 64 * k;
+    lp[k] = colk >= 0 && colk < W ? (uint32_t)(colk * 4) : kOff;
+  }
+  uint32_t lo[2];  // this lane's two pixels, channel c
 #pragma unroll
-    for (int k = 0; k < kRun; ++k) {
-      const int gy = L.oy + r0 + k;
-      xs[k] = ys[k] = 0.f;
-      if (gx < V.width && gy < V.height) {
-        const long long qq = (long long)gy * V.width + gx;
-        xs[k] = img[3 * qq + c];
-        ys[k] = tgt[3 * qq + c];
-      }
+  for (int k = 0; k < 2; ++k) lo[k] = ca + k < W ? (uint32_t)((3 * (ca + k) + c) * 4) : kOff;
+  auto fetch = [&](int r) {
+    Row f;
+    const int row = by - kR + r;
+    const bool row_ok = row >= 0 && row < H;
+    const uint32_t ro = row_ok ? (uint32_t)(row * W * 4) : 0u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t a = row_ok ? lp[k] : kOff;  // rows outside read 0
+      f.v[0][k] = load_or_zero(rp0, a, ro);
+      f.v[1][k] = load_or_zero(rp1, a, ro);
+      f.v[2][k] = load_or_zero(rp2, a, ro);
     }
+    const int orow = by + r - 2 * kR;
+    const bool orow_ok = orow >= 0 && orow < H;
+    const uint32_t oro = orow_ok ? (uint32_t)(orow * W * 12) : 0u;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t a = orow_ok ? lo[k] : kOff;
+      f.ox[k] = load_or_zero(rimg, a, oro);
+      f.oy[k] = load_or_zero(rtgt, a, oro);
+    }
+    return f;
+  };
+  float ring[3][2][kTaps];
+  float ox[2] = {0.f, 0.f}, oy[2] = {0.f, 0.f};  // x and y of the lane's pixels in the current output row
+
+  // one input row: stage the planes, row-blur the lane's two columns into
+  // ring slot `slot`, and take the output row's x and y
+  auto row_in = [&](int r, int slot) {
+    __builtin_amdgcn_sched_barrier(0);  // one scheduling region per row
+    wave_sync_lds();
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
-      const int q = 3 * c + pl, buf = q & 1;
-#pragma unroll
-      for (int r = 0; r < kWinPerThread; ++r) {
-        const int idx = tid + r * 256;
-        if (idx < kWinH * kWinW) s_t[buf][idx / kWinW][idx % kWinW] = pre[r];
-      }
-      __syncthreads();
-      if (q + 1 < kSsimPlanes) fetch(q + 1);
-      row_pass(0, s_t[buf], s_t[buf], p.kernel, s_h[buf]);
-      __syncthreads();
-      float bl[kRun];
-      col_pass(s_h[buf], p.kernel, j, r0, bl);
-#pragma unroll
-      for (int k = 0; k < kRun; ++k) {
-        if (pl == 0) d[k] = bl[k];
-        else if (pl == 1) d[k] += 2.f * xs[k] * bl[k];
-        else d[k] += ys[k] * bl[k];
-      }
+      st[pl][lane] = q[0].v[pl][0];
+      st[pl][64 + lane] = q[0].v[pl][1];
+      st[pl][128 + lane] = q[0].v[pl][2];
     }
 #pragma unroll
-    for (int k = 0; k < kRun; ++k) {
-      const int gy = L.oy + r0 + k;
-      if (gx >= V.width || gy >= V.height) continue;
-      const float difference = xs[k] - ys[k];
-      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
-      float g = (1.f - w) * sign * inverse;
-      g -= w * d[k];
-      dL[3 * (V.pixel_base + (long long)gy * V.width + gx) + c] = g;
+    for (int k = 0; k < 2; ++k) {
+      ox[k] = q[0].ox[k];
+      oy[k] = q[0].oy[k];
     }
+    wave_sync_lds();
+#pragma unroll
+    for (int k = 0; k + 1 < kGradAhead; ++k) q[k] = q[k + 1];
+    q[kGradAhead - 1] = fetch(r + kGradAhead);  // past the band: clamped loads, never used
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      float win[kTaps + 1];
+#pragma unroll
+      for (int i = 0; i < kTaps + 1; i += 2) {
+        const float2 v2 = *reinterpret_cast<const float2*>(&st[pl][2 * lane + i]);
+        win[i] = v2.x;
+        win[i + 1] = v2.y;
+      }
+      float h0 = 0.f, h1 = 0.f;
+#pragma unroll
+      for (int i = 0; i < kTaps; ++i) {
+        h0 = __builtin_fmaf(p.kernel[i], win[i], h0);
+        h1 = __builtin_fmaf(p.kernel[i], win[i + 1], h1);
+      }
+      ring[pl][0][slot] = h0;
+      ring[pl][1][slot] = h1;
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < kGradAhead; ++k) q[k] = fetch(k);
+  // groups of 11 input rows, row r into ring slot r % 11; input row r
+  // completes output row r - 10, staged in LDS slot (r - 10) % 11
+  const int groups = (rows + 2 * kR + kTaps - 1) / kTaps;
+  for (int g = 0; g < groups; ++g) {
+#pragma unroll
+    for (int j = 0; j < kTaps; ++j) {
+      const int r = g * kTaps + j;
+      row_in(r, j);
+      const int o = r - 2 * kR;
+      if (o < 0) continue;  // the first group's first ten rows only fill the ring
+      float bl[3][2];
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          float s = 0.f;
+#pragma unroll
+          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][k][(j + 1 + i) % kTaps], s);
+          bl[pl][k] = s;
+        }
+      float* stage = s_dl[o % kTaps];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        float d = bl[0][k];
+        d += 2.f * ox[k] * bl[1][k];
+        d += oy[k] * bl[2][k];
+        const float difference = ox[k] - oy[k];
+        const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+        float g_ = (1.f - w) * sign * inverse;
+        g_ -= w * d;
+        stage[3 * (2 * lane + k) + c] = g_;
+      }
+    }
+    // all three channels of the output rows this group completed are staged:
+    // write those inside the unit as whole interleaved rows
+    __syncthreads();
+    const int o_first = max(0, g * kTaps - 2 * kR), o_end = min(rows, g * kTaps + kTaps - 2 * kR);
+    for (int o = o_first; o < o_end; ++o) {
+      const int row_float = 3 * ((by + o) * W + sx);
+      const float* stage = s_dl[o % kTaps];
+#pragma unroll
+      for (int e0 = 0; e0 < kUnitW * 3; e0 += kGradWaves * 64) {
+        const int e = e0 + (int)threadIdx.x;
+        store_or_drop(rdl, e < valid_floats ? (uint32_t)((row_float + e) * 4) : kOff, stage[e]);
+      }
+    }
+    __syncthreads();
   }
 }
 
-// One workgroup per view: each thread sums a strided run of the view's tile
-// sums in tile order, then a fixed-shape tree combines the 256 partials, so
+// One workgroup per view: each thread sums a strided run of the view's
+// half-unit sums in order, then a fixed-shape tree combines the 256 partials, so
 // the result does not depend on scheduling.
 __global__ __launch_bounds__(256) void k_loss_reduce(const ViewDesc* __restrict__ views,
                                                      const float* __restrict__ block_sums, float* __restrict__ out) {
   __shared__ float s_l1[256], s_ss[256];
   const int v = blockIdx.x;
   const ViewDesc& V = views[v];
-  const int tiles = ((V.width + kLW - 1) / kLW) * ((V.height + kLH - 1) / kLH);
+  const int units = loss_halves(V.width, V.height);
   float l1 = 0.f, ssim = 0.f;
-  for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
+  for (int t = threadIdx.x; t < units; t += blockDim.x) {
     l1 += block_sums[2 * (V.lt_base + t)];
     ssim += block_sums[2 * (V.lt_base + t) + 1];
   }
@@ -383,17 +521,21 @@ __global__ __launch_bounds__(256) void k_loss_reduce(const ViewDesc* __restrict_
 
 }  // namespace
 
-int loss_tiles(int width, int height) { return ((width + kLW - 1) / kLW) * ((height + kLH - 1) / kLH); }
+int loss_tiles(int width, int height) { return loss_halves(width, height); }
 
 hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* rgb, const float* targets,
                        float* dL, float* terms, long long total_px, float* block_sums, hipStream_t stream) {
   if (p.total_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_loss_stats, dim3(p.total_tiles), dim3(256), 0, stream, p, views, rgb, targets, dL, terms,
-                     total_px, block_sums);
+  if (!(p.ssim_weight > 0.f)) {
+    hipLaunchKernelGGL(k_loss_stats<false>, dim3(p.total_tiles), dim3(kStatWaves * 64), 0, stream, p, views, rgb,
+                       targets, dL, terms, total_px, block_sums);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_loss_stats<true>, dim3(p.total_tiles), dim3(kStatWaves * 64), 0, stream, p, views, rgb, targets,
+                     dL, terms, total_px, block_sums);
   SPLAT_HIP_CHECK(hipGetLastError());
-  if (!(p.ssim_weight > 0.f)) return hipSuccess;
-  hipLaunchKernelGGL(k_loss_grad, dim3(p.total_tiles), dim3(256), 0, stream, p, views, rgb, targets, terms, total_px,
-                     dL);
+  hipLaunchKernelGGL(k_loss_grad, dim3(p.total_tiles / 2), dim3(kGradWaves * 64), 0, stream, p, views, rgb, targets,
+                     terms, total_px, dL);
   return hipGetLastError();
 }
 

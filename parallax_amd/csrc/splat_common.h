@@ -31,7 +31,7 @@ struct ViewDesc {
   int tile_base;       // first global tile id of this view in the batch
   int st_base;         // first global super-tile id (8x8 tiles) of this view
   int scols, srows;    // super-tiles
-  int lt_base;         // first loss tile (64x16 pixels) of this view
+  int lt_base;         // first loss block (64x78 pixels) of this view
   long long pixel_base;  // first pixel of this view in the batch image buffers
   long long target_base;  // first float of this view's target image in the target store
 };

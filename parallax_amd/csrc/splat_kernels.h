@@ -144,13 +144,13 @@ struct LossParams {
   float kernel[11];
   float ssim_weight;
   int n_views;
-  int total_tiles;  // loss tiles (64x16 pixels) over all views
+  int total_tiles;  // loss blocks (64x78 pixels, half units) over all views
 };
 // L1 + SSIM loss terms and dL/drgb of every view of the batch (two passes
-// over 64x16-pixel loss tiles; `terms` holds kSsimPlanes floats per pixel
-// between them); per-loss-tile partial sums of L1 and SSIM into block_sums (2
-// floats per loss tile). Each view's target starts at targets +
-// views[v].target_base.
+// over 128x78-pixel loss units; `terms` holds kSsimPlanes floats per pixel
+// between them); per-block (64x78 pixels) partial sums of L1 and SSIM into
+// block_sums (2 floats per block; loss_tiles counts a view's blocks). Each
+// view's target starts at targets + views[v].target_base.
 constexpr int kSsimPlanes = 9;  // three partial planes per channel
 int loss_tiles(int width, int height);
 hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* rgb, const float* targets,
