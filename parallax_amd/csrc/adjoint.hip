@@ -331,6 +331,20 @@ __device__ __forceinline__ void store_screen(float* dst_f, const float (&v)[11])
   dst[2] = make_float4(v[8], v[9], v[10], 0.f);
 }
 
+// Level 0: the owner of every chunk's first record -- the sorted position
+// whose run [offsets[o], offsets[o + 1]) holds record 64 c. Each chunk start
+// lies in exactly one non-empty run, so every entry below the record total is
+// written once, by that run's thread.
+__global__ __launch_bounds__(256) void k_det_chunk_owners(const uint32_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ p_total, long long m,
+                                                          uint32_t p_cap, uint32_t* __restrict__ chunk_owner) {
+  const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= m) return;
+  const uint32_t total = min(*p_total, p_cap);
+  const uint32_t a = min(offsets[o], total), b = o + 1 < m ? min(offsets[o + 1], total) : total;
+  for (uint32_t c = (a + kDrChunk - 1) / kDrChunk; c * kDrChunk < b; ++c) chunk_owner[c] = (uint32_t)o;
+}
+
 // Level 1: a wave per 64 consecutive partial records. Records are ordered by
 // depth-sorted slot position, each slot's run contiguous, so the lanes' owners
 // are non-decreasing; a segmented inclusive scan across the wave (fixed shape)
@@ -340,21 +354,18 @@ __device__ __forceinline__ void store_screen(float* dst_f, const float (&v)[11])
 __global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__ sorted_slots,
                                                     const uint32_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ p_total, long long m, uint32_t p_cap,
-                                                    const float* __restrict__ partial, float* __restrict__ screen,
+                                                    const uint32_t* __restrict__ chunk_owner,
+                                                    const float* __restrict__ partial,
+                                                    const uint32_t* __restrict__ epoch, float* __restrict__ screen,
                                                     float* __restrict__ carry_first, float* __restrict__ carry_last) {
   const int lane = threadIdx.x & 63;
   const long long c = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const uint32_t total = min(*p_total, p_cap);  // records past p_cap exist only in a flagged step
+  const uint32_t tag = *epoch;
   const long long r0 = c * kDrChunk;
   if (r0 >= total || m <= 0) return;  // wave-uniform
   // owner of r0: the last sorted position whose run starts at or before it
-  long long lo = 0, hi = m;  // offsets[lo] <= r0 < offsets[hi] (offsets[m] = total)
-  while (hi - lo > 1) {
-    const long long mid = (lo + hi) >> 1;
-    if (offsets[mid] <= (uint32_t)r0) lo = mid;
-    else hi = mid;
-  }
-  const long long o0 = lo;  // owners of this chunk: o0 .. o0 + 63 (every run holds a record)
+  const long long o0 = chunk_owner[c];  // owners of this chunk: o0 .. o0 + 63 (every run holds a record)
   auto off_at = [&](long long k) -> uint32_t { return k < m ? min(offsets[k], total) : total; };
   const uint32_t my_off = off_at(o0 + lane);      // run start of candidate owner o0 + lane
   const uint32_t next_off = off_at(o0 + lane + 1);
@@ -372,11 +383,14 @@ __global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__
 #pragma unroll
   for (int k = 0; k < 11; ++k) v[k] = 0.f;
   if (valid) {
-    const float4* src = reinterpret_cast<const float4*>(partial + r * kScreenStride);
+    // a pair the adjoint did not visit this pass holds an older tag: zeros
+    const float4* src = reinterpret_cast<const float4*>(partial + r * kPartialStride);
     const float4 a = src[0], b = src[1], cc = src[2];
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    v[8] = cc.x; v[9] = cc.y; v[10] = cc.z;
+    if (__float_as_uint(cc.w) == tag) {
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+      v[8] = cc.x; v[9] = cc.y; v[10] = cc.z;
+    }
   }
   const int key = valid ? own : 64 + lane;  // invalid lanes: segments of their own
 #pragma unroll
@@ -438,22 +452,40 @@ __global__ __launch_bounds__(256) void k_det_owners(const uint32_t* __restrict__
   store_screen(dst, v);
 }
 
+__global__ void k_bump_epoch(uint32_t* epoch) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const uint32_t e = *epoch + 1u;
+    *epoch = e ? e : 1u;
+  }
+}
+
 }  // namespace
 
+hipError_t launch_bump_epoch(uint32_t* epoch, hipStream_t stream) {
+  hipLaunchKernelGGL(k_bump_epoch, dim3(1), dim3(64), 0, stream, epoch);
+  return hipGetLastError();
+}
+
+// carry_first, carry_last (kScreenStride floats a chunk each), then the
+// chunk owners (one u32 a chunk)
 long long det_reduce_carry_floats(unsigned long long p_cap) {
-  return 2LL * kScreenStride * (long long)((p_cap + kDrChunk - 1) / kDrChunk + 1);
+  return (2LL * kScreenStride + 1) * (long long)((p_cap + kDrChunk - 1) / kDrChunk + 1);
 }
 
 hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
-                             long long m, uint32_t p_cap, const float* partial, float* screen, float* carry,
-                             hipStream_t stream) {
+                             long long m, uint32_t p_cap, const float* partial, const uint32_t* epoch, float* screen,
+                             float* carry, hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   const long long chunks = ((long long)p_cap + kDrChunk - 1) / kDrChunk + 1;
   float* carry_first = carry;
   float* carry_last = carry + kScreenStride * chunks;
+  uint32_t* chunk_owner = reinterpret_cast<uint32_t*>(carry + 2LL * kScreenStride * chunks);
   if (p_cap > 0) {
+    hipLaunchKernelGGL(k_det_chunk_owners, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, offsets, p_total,
+                       m, p_cap, chunk_owner);
+    SPLAT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_det_chunks, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, stream, sorted_slots, offsets,
-                       p_total, m, p_cap, partial, screen, carry_first, carry_last);
+                       p_total, m, p_cap, chunk_owner, partial, epoch, screen, carry_first, carry_last);
     SPLAT_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_det_owners, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,

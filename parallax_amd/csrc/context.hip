@@ -178,6 +178,10 @@ hipError_t splat_context::collect_timing() {
 // ------------------------------------------------------------- pipeline --
 namespace {
 
+// The u32 device scalars: [0] tile pairs, [1] super-tile pairs, [3] overflow
+// flag, [4] work-queue counter, [8] the deterministic partials' epoch.
+constexpr int kEpochScalar = 8;
+
 splat_status check_ready(splat_context* ctx) {
   if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
   if (ctx->cap <= 0) return fail(ctx, SPLAT_ERROR_STATE, "no capacity reserved: call splat_set_capacity first");
@@ -276,6 +280,17 @@ splat_status upload_batch(splat_context* ctx) {
   return SPLAT_OK;
 }
 
+// The deterministic adjoint's per-pair partials. A new allocation is
+// cleared: tag 0 is no epoch, so every record reads as zero until written.
+splat_status reserve_partials(splat_context* ctx, unsigned long long p_cap) {
+  const void* before = ctx->partial.ptr;
+  CK(ctx->partial.reserve(sizeof(float) * kPartialStride * std::max<unsigned long long>(1, p_cap)), "alloc partials");
+  if (ctx->partial.ptr != before)
+    CK(launch_fill_f32(ctx->partial.f(), 0.f, (long long)(ctx->partial.bytes / sizeof(float)), ctx->stream),
+       "clear partials");
+  return SPLAT_OK;
+}
+
 // Buffers sized by the pair counts: tile pairs p_cap, super-tile pairs s_cap.
 splat_status reserve_pairs(splat_context* ctx, unsigned long long p_cap, unsigned long long s_cap) {
   p_cap = std::max(1ULL, p_cap);
@@ -297,7 +312,7 @@ splat_status reserve_pairs(splat_context* ctx, unsigned long long p_cap, unsigne
                                                     (long long)(64ull * ns_max + 1))),
      "alloc sort workspace");
   if (ctx->opts.deterministic) {
-    CK(ctx->partial.reserve(sizeof(float) * kScreenStride * p_cap), "alloc partials");
+    if (splat_status st = reserve_partials(ctx, p_cap)) return st;
     CK(ctx->carry.reserve(sizeof(float) * det_reduce_carry_floats(p_cap)), "alloc carries");
   }
   CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, ctx->M)), "alloc screen");
@@ -541,11 +556,12 @@ splat_status enqueue_backward(splat_context* ctx) {
   const long long M = ctx->M;
   const bool det = ctx->opts.deterministic != 0;
   CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, M)), "alloc screen");
+  uint32_t* epoch = ctx->scalars.u() + kEpochScalar;
   if (det) {
-    CK(ctx->partial.reserve(sizeof(float) * kScreenStride * std::max<unsigned long long>(1, ctx->P_cap)),
-       "alloc partials");
+    if (splat_status st = reserve_partials(ctx, ctx->P_cap)) return st;
     CK(ctx->carry.reserve(sizeof(float) * det_reduce_carry_floats(std::max<unsigned long long>(1, ctx->P_cap))),
        "alloc carries");
+    TIMED("epoch", launch_bump_epoch(epoch, ctx->stream));
   } else {
     TIMED("screen_clear", launch_fill_f32(ctx->screen.f(), 0.f, kScreenStride * M, ctx->stream));
   }
@@ -553,7 +569,7 @@ splat_status enqueue_backward(splat_context* ctx) {
   auto launch = [&](const RasterConfig& c) {
     return launch_backward(ctx->raster, c, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
                            ctx->dL.f(), ctx->T.f(), ctx->last.i(), ctx->screen.f(), ctx->pair_base.u(),
-                           ctx->partial.f(), p_cap, det, ctx->tile_order, ctx->scalars.u() + 4, ctx->stream);
+                           ctx->partial.f(), p_cap, det, epoch, ctx->tile_order, ctx->scalars.u() + 4, ctx->stream);
   };
   if (ctx->tuning_backward && !ctx->capturing) {
     // the adjoint's launch shape for this batch shape: each variant timed on
@@ -588,7 +604,7 @@ splat_status enqueue_backward(splat_context* ctx) {
   ap.deterministic = det ? 1 : 0;
   if (det)
     TIMED("det_reduce", launch_det_reduce(ctx->sorted_slots, ctx->offsets.u(), ctx->scalars.u(), M, p_cap,
-                                          ctx->partial.f(), ctx->screen.f(), ctx->carry.f(), ctx->stream));
+                                          ctx->partial.f(), epoch, ctx->screen.f(), ctx->carry.f(), ctx->stream));
   TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
                                   ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream));
   return SPLAT_OK;

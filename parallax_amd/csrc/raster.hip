@@ -409,7 +409,7 @@ __device__ __forceinline__ int wave_sum_slot(int lane) {
 // marked splat from the back, each marked quadrant's pixels are evaluated
 // (a wave-uniform branch per quadrant), the lane adds its pixels' terms, the
 // wave sums them, and after the chunk each lane issues its splat's result: 11
-// hardware float atomics (atomic mode) or one 48-byte partial store
+// hardware float atomics (atomic mode) or one 64-byte partial store
 // (deterministic mode, summation order fixed by the code alone).
 template <int kBB, bool kQueue>
 __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
@@ -419,6 +419,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
                                                  const float* __restrict__ T_in, const int* __restrict__ last_in,
                                                  float* __restrict__ screen, const uint32_t* __restrict__ pair_base,
                                                  float* __restrict__ partial, uint32_t p_cap, int deterministic,
+                                                 const uint32_t* __restrict__ epoch,
                                                  const uint32_t* __restrict__ order, uint32_t* __restrict__ counter) {
   static_assert(kBB == 64 || kBB == 128, "chunks of one or two entries per lane");
   constexpr int kHalves = kBB / 64;
@@ -433,6 +434,11 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   const float slot_scale = (my_slot == 7 || my_slot == 9) ? 0.5f : 1.f;
   const bool lead = (lane & 12) == 0 && (lane & 3) != 3;  // one lane per slot after wave_sum11
   const bool writer = lead && my_slot < kVals;
+  // the 16 lanes that store a deterministic partial record, one slot each
+  const bool rec_lane = (lane & 12) == 0;
+  const int rec_slot = (lane & 3) == 3 ? 12 + ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) : my_slot;
+  // slot 11 carries this pass's tag; slots 12-15 are zero padding
+  const float rec_pad = rec_slot == 11 ? __uint_as_float(deterministic ? *epoch : 0u) : 0.f;
   for (int iter = 0;; ++iter) {
     int tile;
     if (kQueue) {
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
     const int local = tile - V.tile_base;
     const int tx = local % V.cols, ty = local / V.cols;
     const TileQuads tq = tile_quads(V, tx, ty);
-    const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
+    const uint32_t begin = ranges[2 * tile];  // the walk starts at the last contributor, before the list's end
     // the lane's four pixels: state of the engine's backward walk
     // (SplatRaster.cpp:346-433), with the suffix colour kept folded (the
     // value the next contributor's step reads, updated as soon as this one is
@@ -481,21 +487,9 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
       for (int off = 32; off >= 1; off >>= 1) s = max(s, __shfl_xor(s, off, 64));
       stops[k] = __builtin_amdgcn_readfirstlane(s);
     }
+    // pairs past every pixel's last contributor are not visited: their
+    // partials keep an older tag and read as zeros
     const uint32_t stop = begin + (uint32_t)max(max(stops[0], stops[1]), max(stops[2], stops[3]));
-    // Pairs past every pixel's last contributor get zero partials.
-    if (deterministic) {
-      for (uint32_t e = stop + lane; e < end; e += 64) {
-        const uint32_t slot = pair_vals[e];
-        const SplatRec& r = rec[slot];
-        const uint32_t pid = pair_base[slot] + (uint32_t)((ty - r.y0) * (r.x1 - r.x0 + 1) + (tx - r.x0));
-        if (pid >= p_cap) continue;  // only when the step's capacity check has flagged it
-        float4* dst = reinterpret_cast<float4*>(partial + (long long)pid * kScreenStride);
-        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-        dst[0] = zero;
-        dst[1] = zero;
-        dst[2] = zero;
-      }
-    }
     for (uint32_t hi = stop; hi > begin;) {
       const uint32_t lo = hi - min((uint32_t)kBB, hi - begin);
       const int cnt = (int)(hi - lo);
@@ -526,13 +520,6 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
                                                       tq.x1[k & 1] - a.x, tq.y0[k >> 1] - a.y, tq.y1[k >> 1] - a.y)))
               m &= ~(1u << k);
           dst = deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
-          if (deterministic && m == 0u && dst < p_cap) {  // no quadrant visits it: a zero record
-            float4* out = reinterpret_cast<float4*>(partial + (long long)dst * kScreenStride);
-            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-            out[0] = zero;
-            out[1] = zero;
-            out[2] = zero;
-          }
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) qbits[h][k] = __ballot((m >> k) & 1u);
@@ -610,12 +597,13 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           float z = 0.f;
           if (any) z = wave_sum11(acc, lane) * slot_scale;
           // the 12 lead lanes hold the splat's wave totals: one instruction
-          // covers the splat's contiguous 48-byte record -- atomics (nonzero
-          // terms only) or the partial's stores (slot 11 is padding, zero)
+          // covers the splat's contiguous record -- atomics (nonzero terms
+          // only) or the partial's 64-byte store (lanes 3, 19, 35, 51 add
+          // slots 12-15, so the store fills its whole memory sector)
           const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
           if (deterministic) {
-            if (lead && my_slot < kScreenStride && dsti < p_cap)
-              partial[(long long)dsti * kScreenStride + my_slot] = my_slot < kVals ? z : 0.f;
+            if (rec_lane && dsti < p_cap)
+              partial[(long long)dsti * kPartialStride + rec_slot] = rec_slot < kVals ? z : rec_pad;
           } else if (any && writer && z != 0.f) {
             unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
           }
@@ -671,12 +659,13 @@ template <int kBB, bool kQueue>
 hipError_t backward_variant(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
                             const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
                             const int* last, float* screen, const uint32_t* pair_base, float* partial,
-                            uint32_t p_cap, bool deterministic, const uint32_t* order, uint32_t* counter,
-                            int cus, hipStream_t stream) {
+                            uint32_t p_cap, bool deterministic, const uint32_t* epoch, const uint32_t* order,
+                            uint32_t* counter, int cus, hipStream_t stream) {
   static const int per_cu = resident_blocks(k_backward<kBB, kQueue>, 64);
   const int blocks = kQueue ? std::min(p.total_tiles, per_cu * cus) : p.total_tiles;
   hipLaunchKernelGGL((k_backward<kBB, kQueue>), dim3(blocks), dim3(64), 0, stream, p, views, ranges, pair_vals, rec,
-                     dL_drgb, T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0, order, counter);
+                     dL_drgb, T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0, epoch, order,
+                     counter);
   return hipGetLastError();
 }
 
@@ -715,23 +704,23 @@ hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const
                            const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec,
                            const float* dL_drgb, const float* T, const int* last, float* screen,
                            const uint32_t* pair_base, float* partial, uint32_t p_cap, bool deterministic,
-                           const uint32_t* order, uint32_t* counter, hipStream_t stream) {
+                           const uint32_t* epoch, const uint32_t* order, uint32_t* counter, hipStream_t stream) {
   if (p.total_tiles <= 0) return hipSuccess;
   if (cfg.bwd_queue) SPLAT_HIP_CHECK(launch_fill_u32(counter, 0u, 1, stream));
   const int pb = cfg.cus;
   switch ((cfg.bwd_batch == 64 ? 0 : 2) + (cfg.bwd_queue ? 1 : 0)) {
     case 0:
       return backward_variant<64, false>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                         partial, p_cap, deterministic, order, counter, pb, stream);
+                                         partial, p_cap, deterministic, epoch, order, counter, pb, stream);
     case 1:
       return backward_variant<64, true>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                        partial, p_cap, deterministic, order, counter, pb, stream);
+                                        partial, p_cap, deterministic, epoch, order, counter, pb, stream);
     case 2:
       return backward_variant<128, false>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                          partial, p_cap, deterministic, order, counter, pb, stream);
+                                          partial, p_cap, deterministic, epoch, order, counter, pb, stream);
     default:
       return backward_variant<128, true>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                         partial, p_cap, deterministic, order, counter, pb, stream);
+                                         partial, p_cap, deterministic, epoch, order, counter, pb, stream);
   }
 }
 

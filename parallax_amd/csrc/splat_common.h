@@ -15,6 +15,9 @@ constexpr float kSlack = 1.3f;        // SplatRaster.cpp:26
 constexpr float kShC0 = 0.28209479177387814f;
 constexpr float kShC1 = 0.4886025119029199f;
 constexpr int kScreenStride = 12;     // floats per (view, splat) screen accumulator
+// floats per deterministic (tile, splat) partial: 64-byte records, each one
+// whole memory sector, so their scattered stores need no read-modify-write
+constexpr int kPartialStride = 16;
 constexpr int kMaxBatchViews = 64;
 
 // One camera of a batch, in the float form the projection uses

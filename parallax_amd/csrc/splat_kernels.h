@@ -132,12 +132,14 @@ hipError_t launch_forward(const RasterParams& p, const RasterConfig& cfg, const 
                           int* last, const uint32_t* order, uint32_t* counter, hipStream_t stream);
 // Atomic mode: screen accumulators (kScreenStride floats per slot) via
 // wave-reduced atomics. Deterministic mode: per-pair partials written to
-// partial[pair_base[slot] + local pair] (kScreenStride floats each).
+// partial[pair_base[slot] + local pair] (kPartialStride floats each), for the
+// pairs the adjoint visits only, each tagged in slot 11 with *epoch; a record
+// whose tag is not the current epoch reads as zero.
 hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
                            const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec,
                            const float* dL_drgb, const float* T, const int* last, float* screen,
                            const uint32_t* pair_base, float* partial, uint32_t p_cap, bool deterministic,
-                           const uint32_t* order, uint32_t* counter, hipStream_t stream);
+                           const uint32_t* epoch, const uint32_t* order, uint32_t* counter, hipStream_t stream);
 
 // ---- loss ----
 struct LossParams {
@@ -177,10 +179,16 @@ hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const f
 // [offsets[i], offsets[i+1]) of depth-sorted position i, the last run ending
 // at *p_total, all clamped to p_cap) summed in a fixed order. `carry` holds
 // det_reduce_carry_floats(p_cap) floats.
+// Records tagged with another epoch than *epoch count as zeros.
 long long det_reduce_carry_floats(unsigned long long p_cap);
 hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
-                             long long m, uint32_t p_cap, const float* partial, float* screen, float* carry,
-                             hipStream_t stream);
+                             long long m, uint32_t p_cap, const float* partial, const uint32_t* epoch, float* screen,
+                             float* carry, hipStream_t stream);
+// The next epoch of the deterministic partials: *epoch + 1, skipping 0 (the
+// tag of a cleared buffer). A device-side counter, so each replay of a
+// captured step takes a new one; a record can be mistaken for current only
+// after 2^32 - 1 further passes.
+hipError_t launch_bump_epoch(uint32_t* epoch, hipStream_t stream);
 
 // ---- optimiser ----
 // The per-step scalars, read from device memory so a captured step can take
