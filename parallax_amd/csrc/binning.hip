@@ -226,9 +226,21 @@ __global__ __launch_bounds__(256) void k_tile_count(const uint32_t* __restrict__
 }
 
 // Writes every splat of every segment into each covered tile's run. Same
-// traversal as k_tile_count; `scan` is the exclusive scan of its table. Per
-// chunk and covered tile, the covering lanes store at the tile's next slot
-// plus their rank among those lanes (mbcnt), so each run is in list order.
+// segments as k_tile_count; `scan` is the exclusive scan of its table. The
+// wave holds the segment's (up to four) chunks of 64 list entries in
+// registers, each as its slot and a 64-bit mask of the super-tile's tiles its
+// rect covers, and walks the tiles in the outer loop: per tile and chunk, the
+// covering lanes store at the tile's next slot plus their rank among those
+// lanes (mbcnt), so each run is in list order and each tile's whole run for
+// the segment goes out as one burst of adjacent stores. Both loops are
+// unrolled: a lane's coverage of a tile is one constant-bit test, and its
+// ballot is the store's execution mask. Entries at or past p_cap (a step the
+// capacity check flags) all land on the spare entry pair_vals[p_cap]. With
+// kNarrow (4 (p_cap + 1) < 2^32), the stores take 32-bit byte offsets from
+// the list's base.
+constexpr int kSegChunks = kBinSegment / 64;
+
+template <bool kNarrow>
 __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ svals,
                                                     const uint32_t* __restrict__ sranges,
@@ -238,27 +250,45 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ ns_total,
                                                     const uint32_t* __restrict__ scan, uint32_t p_cap,
                                                     uint32_t* __restrict__ pair_vals) {
+  static_assert(kBinSegment % 64 == 0 && kSuper == 8, "segments of whole chunks, 8x8 super-tiles");
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   const uint32_t ns = *ns_total;
   for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < ns; g += waves) {
     const Segment S = segment_of(g, sranges, nseg, seg_base, seg_st);
-    uint32_t base = scan[S.table0 + (uint64_t)lane * S.nsg + S.s];  // lane t: next slot of tile t
-    for (uint32_t c = S.b0; c < S.b1; c += 64) {
-      const uint32_t i = c + (uint32_t)lane;
-      const bool valid = i < S.b1;
-      const uint32_t slot = valid ? svals[i] : 0u;
-      const ChunkCover cv = chunk_cover(valid, valid ? skeys[i] : 0u);
+    const uint32_t base = scan[S.table0 + (uint64_t)lane * S.nsg + S.s];  // lane t: tile t's first slot
+    uint32_t slot[kSegChunks], cov[kSegChunks][2];  // cov: tiles (ty * 8 + tx) the rect covers
 #pragma unroll
-      for (int t = 0; t < 64; ++t) {
-        const uint64_t b = cv.rows[t >> 3] & cv.cols[t & 7];
-        if (b == 0ull) continue;  // wave-uniform
-        const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
-        const uint32_t to =
-            at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-        if (((b >> lane) & 1ull) && to < p_cap) pair_vals[to] = slot;  // past p_cap: the step is flagged
+    for (int c = 0; c < kSegChunks; ++c) {
+      const uint32_t i = S.b0 + 64u * c + (uint32_t)lane;
+      const bool valid = i < S.b1;
+      const uint32_t key = valid ? skeys[i] : 0u;
+      slot[c] = valid ? svals[i] : 0u;
+      const uint32_t lx0 = (key >> kStBits) & 7, lx1 = (key >> (kStBits + 3)) & 7;
+      const uint32_t ly0 = (key >> (kStBits + 6)) & 7, ly1 = (key >> (kStBits + 9)) & 7;
+      const uint64_t cols = ((2ull << lx1) - 1ull) & ~((1ull << lx0) - 1ull);                    // bits lx0..lx1
+      const uint64_t rows = ((2ull << (8 * ly1 + 7)) - 1ull) & ~((1ull << (8 * ly0)) - 1ull);  // bytes ly0..ly1
+      const uint64_t m = valid ? (cols * 0x0101010101010101ull) & rows : 0ull;
+      cov[c][0] = (uint32_t)m;
+      cov[c][1] = (uint32_t)(m >> 32);
+    }
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+      uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
+#pragma unroll
+      for (int c = 0; c < kSegChunks; ++c) {
+        const uint64_t b = __ballot((cov[c][t >> 5] >> (t & 31)) & 1u);
+        const uint32_t to = min(at + (uint32_t)__builtin_amdgcn_mbcnt_hi(
+                                         (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u)),
+                                p_cap);
+        if (__builtin_amdgcn_inverse_ballot_w64(b)) {
+          if constexpr (kNarrow)
+            *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(pair_vals) + (uint32_t)(to << 2)) = slot[c];
+          else
+            pair_vals[to] = slot[c];
+        }
+        at += (uint32_t)__popcll(b);
       }
-      base += lane_tile_count(cv, lane);
     }
   }
 }
@@ -356,8 +386,12 @@ hipError_t launch_tile_write(const uint32_t* skeys, const uint32_t* svals,
                              const uint32_t* seg_st, const uint32_t* ns_total, uint32_t ns_max, const uint32_t* scan,
                              uint32_t p_cap, uint32_t* pair_vals, hipStream_t stream) {
   if (ns_max == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tile_write, dim3(segment_grid(ns_max)), dim3(256), 0, stream, skeys, svals,
-                     sranges, nseg, seg_base, seg_st, ns_total, scan, p_cap, pair_vals);
+  if (4ull * ((unsigned long long)p_cap + 1ull) <= 0xFFFFFFFFull)
+    hipLaunchKernelGGL(k_tile_write<true>, dim3(segment_grid(ns_max)), dim3(256), 0, stream, skeys, svals, sranges,
+                       nseg, seg_base, seg_st, ns_total, scan, p_cap, pair_vals);
+  else
+    hipLaunchKernelGGL(k_tile_write<false>, dim3(segment_grid(ns_max)), dim3(256), 0, stream, skeys, svals, sranges,
+                       nseg, seg_base, seg_st, ns_total, scan, p_cap, pair_vals);
   return hipGetLastError();
 }
 

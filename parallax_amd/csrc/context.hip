@@ -301,7 +301,7 @@ splat_status reserve_pairs(splat_context* ctx, unsigned long long p_cap, unsigne
   ctx->S_cap = s_cap;
   // segments: at most ceil(len / kBinSegment) per super-tile
   const unsigned long long ns_max = s_cap / kBinSegment + (unsigned long long)ctx->n_st;
-  CK(ctx->pair_vals.reserve(4 * p_cap), "alloc pairs");
+  CK(ctx->pair_vals.reserve(4 * (p_cap + 1)), "alloc pairs");  // + the spare entry overflowing writes land on
   CK(ctx->skeys.reserve(4 * s_cap), "alloc super pairs");
   CK(ctx->svals.reserve(4 * s_cap), "alloc super pairs");
   CK(ctx->skeys_alt.reserve(4 * s_cap), "alloc super pairs");
