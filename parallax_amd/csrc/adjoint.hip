@@ -335,10 +335,17 @@ __device__ __forceinline__ void store_screen(float* dst_f, const float (&v)[11])
 // whose run [offsets[o], offsets[o + 1]) holds record 64 c. Each chunk start
 // lies in exactly one non-empty run, so every entry below the record total is
 // written once, by that run's thread.
+// The depth-sorted positions in use: min(m, *m_dev).
+__device__ __forceinline__ long long sorted_count(long long m, const uint32_t* m_dev) {
+  return m_dev ? min(m, (long long)*m_dev) : m;
+}
+
 __global__ __launch_bounds__(256) void k_det_chunk_owners(const uint32_t* __restrict__ offsets,
-                                                          const uint32_t* __restrict__ p_total, long long m,
-                                                          uint32_t p_cap, uint32_t* __restrict__ chunk_owner) {
+                                                          const uint32_t* __restrict__ p_total, long long m_cap,
+                                                          const uint32_t* __restrict__ m_dev, uint32_t p_cap,
+                                                          uint32_t* __restrict__ chunk_owner) {
   const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long m = sorted_count(m_cap, m_dev);
   if (o >= m) return;
   const uint32_t total = min(*p_total, p_cap);
   const uint32_t a = min(offsets[o], total), b = o + 1 < m ? min(offsets[o + 1], total) : total;
@@ -353,12 +360,14 @@ __global__ __launch_bounds__(256) void k_det_chunk_owners(const uint32_t* __rest
 // edge go to carry_first (the chunk's first owner) / carry_last (its last).
 __global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__ sorted_slots,
                                                     const uint32_t* __restrict__ offsets,
-                                                    const uint32_t* __restrict__ p_total, long long m, uint32_t p_cap,
+                                                    const uint32_t* __restrict__ p_total, long long m_cap,
+                                                    const uint32_t* __restrict__ m_dev, uint32_t p_cap,
                                                     const uint32_t* __restrict__ chunk_owner,
                                                     const float* __restrict__ partial,
                                                     const uint32_t* __restrict__ epoch, float* __restrict__ screen,
                                                     float* __restrict__ carry_first, float* __restrict__ carry_last) {
   const int lane = threadIdx.x & 63;
+  const long long m = sorted_count(m_cap, m_dev);
   const long long c = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const uint32_t total = min(*p_total, p_cap);  // records past p_cap exist only in a flagged step
   const uint32_t tag = *epoch;
@@ -423,10 +432,12 @@ __global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__
 // zeros.
 __global__ __launch_bounds__(256) void k_det_owners(const uint32_t* __restrict__ sorted_slots,
                                                     const uint32_t* __restrict__ offsets,
-                                                    const uint32_t* __restrict__ p_total, long long m, uint32_t p_cap,
+                                                    const uint32_t* __restrict__ p_total, long long m_cap,
+                                                    const uint32_t* __restrict__ m_dev, uint32_t p_cap,
                                                     const float* __restrict__ carry_first,
                                                     const float* __restrict__ carry_last, float* __restrict__ screen) {
   const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long m = sorted_count(m_cap, m_dev);
   if (o >= m) return;
   const uint32_t total = min(*p_total, p_cap);
   const uint32_t a = min(offsets[o], total), b = o + 1 < m ? min(offsets[o + 1], total) : total;
@@ -473,8 +484,8 @@ long long det_reduce_carry_floats(unsigned long long p_cap) {
 }
 
 hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
-                             long long m, uint32_t p_cap, const float* partial, const uint32_t* epoch, float* screen,
-                             float* carry, hipStream_t stream) {
+                             long long m, const uint32_t* m_dev, uint32_t p_cap, const float* partial,
+                             const uint32_t* epoch, float* screen, float* carry, hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   const long long chunks = ((long long)p_cap + kDrChunk - 1) / kDrChunk + 1;
   float* carry_first = carry;
@@ -482,14 +493,14 @@ hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offse
   uint32_t* chunk_owner = reinterpret_cast<uint32_t*>(carry + 2LL * kScreenStride * chunks);
   if (p_cap > 0) {
     hipLaunchKernelGGL(k_det_chunk_owners, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, offsets, p_total,
-                       m, p_cap, chunk_owner);
+                       m, m_dev, p_cap, chunk_owner);
     SPLAT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_det_chunks, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, stream, sorted_slots, offsets,
-                       p_total, m, p_cap, chunk_owner, partial, epoch, screen, carry_first, carry_last);
+                       p_total, m, m_dev, p_cap, chunk_owner, partial, epoch, screen, carry_first, carry_last);
     SPLAT_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_det_owners, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
-                     p_total, m, p_cap, carry_first, carry_last, screen);
+                     p_total, m, m_dev, p_cap, carry_first, carry_last, screen);
   return hipGetLastError();
 }
 

@@ -55,23 +55,34 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ soffsets,
                                                     const uint32_t* __restrict__ s_total,
                                                     const uint2* __restrict__ rects,
+                                                    const uint32_t* __restrict__ sorted_prects,
                                                     const ViewDesc* __restrict__ views, long long per_view,
-                                                    long long m, uint32_t s_cap, uint32_t* __restrict__ skeys,
+                                                    long long m_cap, const uint32_t* __restrict__ m_dev,
+                                                    uint32_t s_cap, uint32_t* __restrict__ skeys,
                                                     uint32_t* __restrict__ svals, uint32_t* __restrict__ pair_base) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  const long long m = m_dev ? min(m_cap, (long long)*m_dev) : m_cap;
   uint32_t slot = 0, c = 0, off = 0xFFFFFFFFu;
   int x0 = 0, x1 = 0, y0 = 0, y1 = 0, w = 1, scols = 0, st_base = 0;
   if (s < m) {
     slot = sorted_slots[s];
     off = soffsets[s];
     c = (s + 1 < m ? soffsets[s + 1] : *s_total) - off;
-    if (c) {
+    if (c && sorted_prects) {  // the packed rect rode along with the depth sort
+      const uint32_t r = sorted_prects[s];
+      x0 = rect_x0(r);
+      x1 = rect_x1(r);
+      y0 = rect_y0(r);
+      y1 = rect_y1(r);
+    } else if (c) {
       const uint2 packed = rects[slot];  // x0, x1, y0, y1 as int16
       x0 = (int)(int16_t)(packed.x & 0xFFFF);
       x1 = (int)(int16_t)(packed.x >> 16);
       y0 = (int)(int16_t)(packed.y & 0xFFFF);
       y1 = (int)(int16_t)(packed.y >> 16);
+    }
+    if (c) {
       const int v = (int)(slot / per_view);
       w = x1 / kSuper - x0 / kSuper + 1;
       scols = views[v].scols;
@@ -115,16 +126,33 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
   }
 }
 
+// Four sorted keys a thread (one 16-byte load, the neighbours from the lanes
+// beside it): a super-tile's list starts where its id first appears and
+// ends after its last.
 __global__ __launch_bounds__(256) void k_super_ranges(const uint32_t* __restrict__ keys, long long n_cap,
                                                       const uint32_t* __restrict__ n_dev,
                                                       uint32_t* __restrict__ ranges, int n_st) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // keys 4q .. 4q + 3
   const long long n = n_dev ? min((long long)*n_dev, n_cap) : n_cap;
-  if (i >= n) return;
-  const uint32_t k = keys[i] & kStMask;
-  if (k >= (uint32_t)n_st) return;  // cannot happen for keys the emitter wrote
-  if (i == 0 || (keys[i - 1] & kStMask) != k) ranges[2 * (long long)k] = (uint32_t)i;
-  if (i == n - 1 || (keys[i + 1] & kStMask) != k) ranges[2 * (long long)k + 1] = (uint32_t)(i + 1);
+  const long long i0 = 4 * q;
+  if (i0 >= n) return;
+  uint32_t k[6];  // keys i0 - 1 .. i0 + 4 (ids; past the ends: none)
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  if (i0 + 4 <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+    k[1] = v.x & kStMask; k[2] = v.y & kStMask; k[3] = v.z & kStMask; k[4] = v.w & kStMask;
+  } else {
+    for (int a = 0; a < 4; ++a) k[1 + a] = i0 + a < n ? keys[i0 + a] & kStMask : kNone;
+  }
+  k[0] = i0 > 0 ? keys[i0 - 1] & kStMask : kNone;
+  k[5] = i0 + 4 < n ? keys[i0 + 4] & kStMask : kNone;
+#pragma unroll
+  for (int a = 1; a <= 4; ++a) {
+    const uint32_t id = k[a];
+    if (id >= (uint32_t)n_st) continue;  // past the end (or an id the emitter cannot write)
+    if (k[a - 1] != id) ranges[2 * (long long)id] = (uint32_t)(i0 + a - 1);
+    if (k[a + 1] != id) ranges[2 * (long long)id + 1] = (uint32_t)(i0 + a);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_segment_counts(const uint32_t* __restrict__ sranges, int n_st,
@@ -240,6 +268,30 @@ __global__ __launch_bounds__(256) void k_tile_count(const uint32_t* __restrict__
 // the list's base.
 constexpr int kSegChunks = kBinSegment / 64;
 
+template <bool kNarrow, bool kBound>
+__device__ __forceinline__ void write_segment_runs(uint32_t base, const uint32_t (&slot)[kSegChunks],
+                                                   const uint32_t (&cov)[kSegChunks][2], uint32_t p_cap,
+                                                   uint32_t* __restrict__ pair_vals) {
+#pragma unroll
+  for (int t = 0; t < 64; ++t) {
+    uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
+#pragma unroll
+    for (int c = 0; c < kSegChunks; ++c) {
+      const uint64_t b = __ballot((cov[c][t >> 5] >> (t & 31)) & 1u);
+      uint32_t to = at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+      if constexpr (kBound) to = min(to, p_cap);
+      if (__builtin_amdgcn_inverse_ballot_w64(b)) {
+        if constexpr (kNarrow)
+          *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(pair_vals) + (uint32_t)(to << 2)) = slot[c];
+        else
+          pair_vals[to] = slot[c];
+      }
+      at += (uint32_t)__popcll(b);
+    }
+  }
+}
+
 template <bool kNarrow>
 __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ svals,
@@ -272,24 +324,14 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
       cov[c][0] = (uint32_t)m;
       cov[c][1] = (uint32_t)(m >> 32);
     }
-#pragma unroll
-    for (int t = 0; t < 64; ++t) {
-      uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
-#pragma unroll
-      for (int c = 0; c < kSegChunks; ++c) {
-        const uint64_t b = __ballot((cov[c][t >> 5] >> (t & 31)) & 1u);
-        const uint32_t to = min(at + (uint32_t)__builtin_amdgcn_mbcnt_hi(
-                                         (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u)),
-                                p_cap);
-        if (__builtin_amdgcn_inverse_ballot_w64(b)) {
-          if constexpr (kNarrow)
-            *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(pair_vals) + (uint32_t)(to << 2)) = slot[c];
-          else
-            pair_vals[to] = slot[c];
-        }
-        at += (uint32_t)__popcll(b);
-      }
-    }
+    // the segment's runs all end by the end of tile 63's run: when that is
+    // within p_cap (every step the capacity check passes), no store needs a
+    // bound
+    const uint32_t seg_end = scan[S.table0 + 63ull * S.nsg + S.s + 1];
+    if (seg_end <= p_cap)
+      write_segment_runs<kNarrow, false>(base, slot, cov, p_cap, pair_vals);
+    else
+      write_segment_runs<kNarrow, true>(base, slot, cov, p_cap, pair_vals);
   }
 }
 
@@ -328,12 +370,14 @@ hipError_t launch_check_caps(uint32_t* totals, uint32_t p_cap, uint32_t s_cap, h
 }
 
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* s_total, const uint2* rects, const ViewDesc* views, long long per_view,
-                             long long m, uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
+                             const uint32_t* s_total, const uint2* rects, const uint32_t* sorted_prects,
+                             const ViewDesc* views, long long per_view, long long m, const uint32_t* m_dev,
+                             uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
                              hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
-                     soffsets, s_total, rects, views, per_view, m, s_cap, skeys, svals, pair_base);
+                     soffsets, s_total, rects, sorted_prects, views, per_view, m, m_dev, s_cap, skeys, svals,
+                     pair_base);
   return hipGetLastError();
 }
 
@@ -341,8 +385,8 @@ hipError_t launch_super_ranges(const uint32_t* sorted_keys, long long n, const u
                                int n_st, hipStream_t stream) {
   SPLAT_HIP_CHECK(launch_fill_u32(sranges, 0u, 2LL * n_st, stream));
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_super_ranges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sorted_keys, n, n_dev,
-                     sranges, n_st);
+  hipLaunchKernelGGL(k_super_ranges, dim3((unsigned)(((n + 3) / 4 + 255) / 256)), dim3(256), 0, stream, sorted_keys, n,
+                     n_dev, sranges, n_st);
   return hipGetLastError();
 }
 

@@ -179,7 +179,9 @@ hipError_t splat_context::collect_timing() {
 namespace {
 
 // The u32 device scalars: [0] tile pairs, [1] super-tile pairs, [3] overflow
-// flag, [4] work-queue counter, [8] the deterministic partials' epoch.
+// flag, [4] work-queue counter, [5] projected slots (the depth-sorted ones),
+// [8] the deterministic partials' epoch.
+constexpr int kSortedScalar = 5;
 constexpr int kEpochScalar = 8;
 
 splat_status check_ready(splat_context* ctx) {
@@ -215,6 +217,9 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
     ctx->batch_desc.push_back(d);
   }
   if (supers > (int)kStMask) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: batch too large");
+  ctx->packed = true;
+  for (const ViewDesc& d : ctx->batch_desc)
+    if (d.cols > kPackedTiles || d.rows > kPackedTiles) ctx->packed = false;
   ctx->shape_key.clear();
   for (const ViewDesc& d : ctx->batch_desc) {
     ctx->shape_key += std::to_string(d.width) + "x" + std::to_string(d.height) + ",";
@@ -327,28 +332,43 @@ splat_status enqueue_front(splat_context* ctx) {
   const splat_raster_options* o = &ctx->opts;
   uint32_t* d_total = ctx->scalars.u();
   CK(launch_fill_u32(d_total, 0u, 4, ctx->stream), "clear counters");
+  const bool packed = ctx->packed;
   TIMED("project", launch_project(ctx->params.f(), ctx->cap, (int)n, ctx->rs, ctx->sh_use, ctx->views_dev(), n_views,
-                                  o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(), ctx->vals.u(),
-                                  ctx->counts.u(), ctx->scounts.u(), reinterpret_cast<uint2*>(ctx->rects.u()),
-                                  ctx->stream));
+                                  o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(),
+                                  packed ? nullptr : ctx->counts.u(), packed ? nullptr : ctx->scounts.u(),
+                                  packed ? nullptr : reinterpret_cast<uint2*>(ctx->rects.u()),
+                                  packed ? ctx->scounts.u() : nullptr, ctx->stream));
   bool alt = false;
-  // the super-tile counts ride along with the slots, so their scan reads them
-  // in depth order without a gather
+  // the packed rects (or the super-tile counts) ride along with the slots, so
+  // what reads them in depth order needs no gather; the slots start as the
+  // item indices, and culled slots (key 0xFFFFFFFF) leave in the first pass:
+  // the sorted arrays hold the scalars[kSortedScalar] projected slots
+  uint32_t* n_sorted = d_total + kSortedScalar;
   TIMED("depth_sort", launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
                                         ctx->sort_ws, &alt, ctx->stream, nullptr, ctx->scounts.u(),
-                                        ctx->scounts_alt.u()));
+                                        ctx->scounts_alt.u(), true, n_sorted));
   ctx->sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
-  const uint32_t* sorted_scounts = alt ? ctx->scounts_alt.u() : ctx->scounts.u();
-  // The tile-pair offsets in depth order are the deterministic adjoint's pair
-  // ids; otherwise only their total is needed.
+  const uint32_t* sorted_v2 = alt ? ctx->scounts_alt.u() : ctx->scounts.u();
+  ctx->sorted_prects = packed ? const_cast<uint32_t*>(sorted_v2) : nullptr;
+  if (packed) {
+    // the super-tile offsets in depth order are the emitter's places (their
+    // total S, and on the side the tile-pair total P); the tile-pair offsets
+    // are the deterministic adjoint's pair ids
+    TIMED("pair_scan", launch_exclusive_scan(sorted_v2, nullptr, ctx->soffsets.u(), M, d_total + 1, ctx->sort_ws.scan,
+                                             ctx->stream, kScanRectSupers, n_sorted, d_total, kScanRectTiles));
+    if (o->deterministic)
+      TIMED("pair_scan", launch_exclusive_scan(sorted_v2, nullptr, ctx->offsets.u(), M, nullptr, ctx->sort_ws.scan,
+                                               ctx->stream, kScanRectTiles, n_sorted));
+    return SPLAT_OK;
+  }
   if (o->deterministic) {
     TIMED("pair_scan", launch_exclusive_scan(ctx->counts.u(), ctx->sorted_slots, ctx->offsets.u(), M, d_total,
-                                             ctx->sort_ws.scan, ctx->stream));
+                                             ctx->sort_ws.scan, ctx->stream, kScanPlain, n_sorted));
   } else {
     TIMED("pair_scan", launch_sum_u32(ctx->counts.u(), M, d_total, ctx->stream));
   }
-  TIMED("pair_scan", launch_exclusive_scan(sorted_scounts, nullptr, ctx->soffsets.u(), M, d_total + 1,
-                                           ctx->sort_ws.scan, ctx->stream));
+  TIMED("pair_scan", launch_exclusive_scan(sorted_v2, nullptr, ctx->soffsets.u(), M, d_total + 1, ctx->sort_ws.scan,
+                                           ctx->stream, kScanPlain, n_sorted));
   return SPLAT_OK;
 }
 
@@ -364,10 +384,11 @@ splat_status enqueue_back(splat_context* ctx) {
   const uint32_t ns_max = (uint32_t)(ctx->S_cap / kBinSegment + (unsigned long long)supers);
   uint32_t* d_total = ctx->scalars.u();
   CK(launch_check_caps(d_total, p_cap, s_cap, ctx->stream), "capacity check");
+  uint32_t* pair_base = ctx->opts.deterministic ? ctx->pair_base.u() : nullptr;
   TIMED("emit_super", launch_emit_super(ctx->sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), d_total + 1,
-                                        reinterpret_cast<const uint2*>(ctx->rects.u()), ctx->views_dev(), n, M, s_cap,
-                                        ctx->skeys.u(), ctx->svals.u(),
-                                        ctx->opts.deterministic ? ctx->pair_base.u() : nullptr, ctx->stream));
+                                        reinterpret_cast<const uint2*>(ctx->rects.u()), ctx->sorted_prects,
+                                        ctx->views_dev(), n, M, d_total + kSortedScalar, s_cap, ctx->skeys.u(),
+                                        ctx->svals.u(), pair_base, ctx->stream));
   const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
   bool salt = false;
   TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(), s_cap,
@@ -603,8 +624,9 @@ splat_status enqueue_backward(splat_context* ctx) {
   ap.n_views = (int)ctx->batch_ids.size();
   ap.deterministic = det ? 1 : 0;
   if (det)
-    TIMED("det_reduce", launch_det_reduce(ctx->sorted_slots, ctx->offsets.u(), ctx->scalars.u(), M, p_cap,
-                                          ctx->partial.f(), epoch, ctx->screen.f(), ctx->carry.f(), ctx->stream));
+    TIMED("det_reduce", launch_det_reduce(ctx->sorted_slots, ctx->offsets.u(), ctx->scalars.u(), M,
+                                          ctx->scalars.u() + kSortedScalar, p_cap, ctx->partial.f(), epoch,
+                                          ctx->screen.f(), ctx->carry.f(), ctx->stream));
   TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
                                   ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream));
   return SPLAT_OK;
@@ -1058,15 +1080,17 @@ splat_status splat_download_projection(splat_context* ctx, int32_t slot, int32_t
   const long long n = ctx->n, M = ctx->M;
   std::vector<SplatRec> recs((size_t)n);
   std::vector<uint32_t> order((size_t)M);
+  uint32_t sorted = 0;  // the depth-sorted (projected) slots of the batch
   if (n > 0) {
     CK(hipMemcpyAsync(recs.data(), ctx->recs() + (long long)slot * n, sizeof(SplatRec) * n, hipMemcpyDeviceToHost,
                       ctx->stream),
        "download");
+    CK(hipMemcpyAsync(&sorted, ctx->scalars.u() + kSortedScalar, 4, hipMemcpyDeviceToHost, ctx->stream), "download");
     CK(hipMemcpyAsync(order.data(), ctx->sorted_slots, 4 * M, hipMemcpyDeviceToHost, ctx->stream), "download");
   }
   CK(hipStreamSynchronize(ctx->stream), "sync");
   int k = 0;
-  for (long long s = 0; s < M; ++s) {
+  for (long long s = 0; s < std::min<long long>(M, sorted); ++s) {
     const long long sl = order[(size_t)s];
     if (sl / n != slot) continue;
     const int g = (int)(sl % n);

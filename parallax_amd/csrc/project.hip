@@ -8,7 +8,10 @@
 // (:91-102) and tile rect (:121-126), in IEEE float with no contraction so that
 // culling, the radius and the tile rect -- the inputs of the binning -- match
 // the CPU checker bit for bit. HBM-bound: compulsory traffic is the parameter
-// read (56 + 4*rs bytes per gaussian) plus 80 bytes per (view, gaussian).
+// read (56 + 4*rs bytes per gaussian) plus, per (view, gaussian), the record
+// and the depth key and -- where every view has at most 256 x 256 tiles -- the
+// packed tile rect (72 bytes), else the tile count, the super-tile count and
+// the 8-byte rect (84 bytes).
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -86,8 +89,8 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
                                                  int rest_stride, const ViewDesc* __restrict__ views,
                                                  int n_views, float dilation, float min_alpha,
                                                  SplatRec* __restrict__ rec, uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals, uint32_t* __restrict__ counts,
-                                                 uint32_t* __restrict__ scounts, uint2* __restrict__ rects) {
+                                                 uint32_t* __restrict__ counts, uint32_t* __restrict__ scounts,
+                                                 uint2* __restrict__ rects, uint32_t* __restrict__ prects) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const float* P = params;
@@ -247,11 +250,14 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
     }
     rec[slot] = o;
     keys[slot] = key;
-    vals[slot] = (uint32_t)slot;
-    counts[slot] = count;
-    scounts[slot] = scount;
-    rects[slot] = make_uint2((uint32_t)(uint16_t)o.x0 | (uint32_t)(uint16_t)o.x1 << 16,
-                             (uint32_t)(uint16_t)o.y0 | (uint32_t)(uint16_t)o.y1 << 16);
+    if (prects) {  // wave-uniform
+      prects[slot] = count ? pack_rect(o.x0, o.x1, o.y0, o.y1) : kNoRect;
+    } else {
+      counts[slot] = count;
+      scounts[slot] = scount;
+      rects[slot] = make_uint2((uint32_t)(uint16_t)o.x0 | (uint32_t)(uint16_t)o.x1 << 16,
+                               (uint32_t)(uint16_t)o.y0 | (uint32_t)(uint16_t)o.y1 << 16);
+    }
   }
 }
 
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
 
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
-                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, uint2* rects,
+                          uint32_t* keys, uint32_t* counts, uint32_t* scounts, uint2* rects, uint32_t* prects,
                           hipStream_t stream) {
   if (n <= 0 || n_views <= 0) return hipSuccess;
   const int blocks = (n + 255) / 256;
@@ -268,19 +274,19 @@ hipError_t launch_project(const float* params, long long cap, int n, int rest_st
   switch (sh_use) {
     case 0:
       hipLaunchKernelGGL(k_project<0>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
       break;
     case 1:
       hipLaunchKernelGGL(k_project<1>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
       break;
     case 2:
       hipLaunchKernelGGL(k_project<2>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
       break;
     default:
       hipLaunchKernelGGL(k_project<3>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, vals, counts, scounts, rects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
       break;
   }
   return hipGetLastError();

@@ -50,16 +50,41 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 // barrier, so the scan may run in place.)
 __device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
 
+// n_dev (nullable): the item count, read on the device and clamped to n
+// (the capacity the grid was sized for).
+__device__ __forceinline__ long long item_count(long long n, const uint32_t* n_dev) {
+  if (!n_dev) return n;
+  const long long d = (long long)*n_dev;
+  return d < n ? d : n;
+}
+
+// The value a scan or sum takes of an input word.
+__device__ __forceinline__ uint32_t scan_value(uint32_t x, int xform) {
+  return xform == kScanRectTiles ? rect_tiles(x) : (xform == kScanRectSupers ? rect_supers(x) : x);
+}
+
+// With side_total, each block also adds the sum of side_xform of its inputs
+// to *side_total (one atomic per block).
 __global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* in, const uint32_t* idx, uint32_t* out,
-                                                           long long n, uint32_t* block_sums, uint32_t* total) {
+                                                           long long n_cap, const uint32_t* n_dev, int xform,
+                                                           uint32_t* block_sums, uint32_t* total,
+                                                           uint32_t* side_total, int side_xform) {
   __shared__ uint32_t lds[kScanThreads / 64];
   __shared__ uint32_t tile[kScanTile + kScanTile / 32];
+  const long long n = item_count(n_cap, n_dev);
   const long long base = (long long)blockIdx.x * kScanTile;
   const int tid = threadIdx.x;
+  uint32_t side = 0;
 #pragma unroll
   for (int r = 0; r < kScanItems; ++r) {
     const long long j = base + (long long)r * kScanThreads + tid;
-    tile[scan_pad(r * kScanThreads + tid)] = j < n ? in[idx ? idx[j] : j] : 0u;
+    const uint32_t x = j < n ? in[idx ? idx[j] : j] : 0u;
+    tile[scan_pad(r * kScanThreads + tid)] = j < n ? scan_value(x, xform) : 0u;
+    if (side_total && j < n) side += scan_value(x, side_xform);
+  }
+  if (side_total) {  // block-uniform
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) side += (uint32_t)__shfl_xor((int)side, d, 64);
   }
   __syncthreads();
   uint32_t v[kScanItems];
@@ -82,13 +107,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* in,
     const long long j = base + (long long)r * kScanThreads + tid;
     if (j < n) out[j] = tile[scan_pad(r * kScanThreads + tid)];
   }
+  if (side_total) {  // after every store: one atomic per block, nothing waits on it
+    if ((tid & 63) == 0) lds[tid >> 6] = side;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (int w = 0; w < kScanThreads / 64; ++w) sum += lds[w];
+      if (sum) atomicAdd(side_total, sum);
+    }
+  }
   if (tid == 0) {
     if (block_sums) block_sums[blockIdx.x] = tot;
     if (total && gridDim.x == 1) *total = tot;
   }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* out, long long n, const uint32_t* offsets) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* out, long long n_cap, const uint32_t* n_dev,
+                                                         const uint32_t* offsets) {
+  const long long n = item_count(n_cap, n_dev);
   const uint32_t add = offsets[blockIdx.x];
   const long long base = (long long)blockIdx.x * kScanTile;
   for (int r = 0; r < kScanItems; ++r) {
@@ -97,42 +134,71 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* out, long l
   }
 }
 
-hipError_t scan_level(const uint32_t* in, const uint32_t* idx, uint32_t* out, long long n, uint32_t* total,
-                      uint32_t* ws, hipStream_t stream) {
+// The first level reads n_dev and the transform; the levels above scan plain
+// block sums (zero for blocks past the items).
+hipError_t scan_level(const uint32_t* in, const uint32_t* idx, uint32_t* out, long long n, const uint32_t* n_dev,
+                      int xform, uint32_t* total, uint32_t* side_total, int side_xform, uint32_t* ws,
+                      hipStream_t stream) {
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (nb <= 1) {
-    hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, in, idx, out, n, nullptr, total);
+    hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, in, idx, out, n, n_dev, xform, nullptr,
+                       total, side_total, side_xform);
     return hipGetLastError();
   }
   uint32_t* sums = ws;
-  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, in, idx, out, n, sums,
-                     nullptr);
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, in, idx, out, n, n_dev, xform,
+                     sums, nullptr, side_total, side_xform);
   SPLAT_HIP_CHECK(hipGetLastError());
-  SPLAT_HIP_CHECK(scan_level(sums, nullptr, sums, nb, total, ws + nb, stream));
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, out, n, sums);
+  SPLAT_HIP_CHECK(scan_level(sums, nullptr, sums, nb, nullptr, kScanPlain, total, nullptr, kScanPlain, ws + nb,
+                             stream));
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, out, n, n_dev, sums);
   return hipGetLastError();
 }
 
-// n_dev (nullable): the item count, read on the device and clamped to n
-// (the capacity the grid was sized for).
-__device__ __forceinline__ long long item_count(long long n, const uint32_t* n_dev) {
-  if (!n_dev) return n;
-  const long long d = (long long)*n_dev;
-  return d < n ? d : n;
-}
+// With drop, keys equal to kDropKey are not counted (the scatter drops them).
+constexpr uint32_t kDropKey = 0xFFFFFFFFu;
 
+// Per-block digit counts, into per-wave LDS copies. With kAggregate, each
+// round of 64 items finds the lanes sharing a digit (one ballot per digit
+// bit) and only the group's lowest lane adds the group's size: a skewed
+// digit (a narrow last digit, the few exponents in the top byte of positive
+// float keys) then costs one LDS atomic per round, not up to 64; for spread
+// digits the plain atomics are cheaper.
+template <bool kAggregate>
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys, long long n_cap,
-                                                    const uint32_t* __restrict__ n_dev, int shift, uint32_t mask,
-                                                    uint32_t* __restrict__ hist, int nblocks) {
+                                                    const uint32_t* __restrict__ n_dev, int shift, int bits,
+                                                    int drop, uint32_t* __restrict__ hist, int nblocks) {
   __shared__ uint32_t h[4][256];
   const long long n = item_count(n_cap, n_dev);
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t mask = (1u << bits) - 1u;
   for (int w = 0; w < 4; ++w) h[w][tid] = 0;
   __syncthreads();
-  const long long base = (long long)blockIdx.x * kScanTile;
+  // the wave's own 1024 consecutive items
+  const long long wbase = (long long)blockIdx.x * kScanTile + (long long)wave * (kScanTile / 4);
+  uint32_t k[kScanItems];
+#pragma unroll
   for (int r = 0; r < kScanItems; ++r) {
-    const long long j = base + (long long)r * 256 + tid;
-    if (j < n) atomicAdd(&h[wave][(keys[j] >> shift) & mask], 1u);
+    const long long j = wbase + (long long)r * 64 + lane;
+    k[r] = j < n ? keys[j] : kDropKey;
+  }
+  const uint64_t below = __lanemask_lt();
+#pragma unroll
+  for (int r = 0; r < kScanItems; ++r) {
+    const long long j = wbase + (long long)r * 64 + lane;
+    const bool valid = j < n && !(drop && k[r] == kDropKey);
+    const uint32_t d = (k[r] >> shift) & mask;
+    if constexpr (kAggregate) {
+      uint64_t peers = __ballot(valid);
+      for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      if (valid && (peers & below) == 0ull) atomicAdd(&h[wave][d], (uint32_t)__popcll(peers));
+    } else {
+      if (valid) atomicAdd(&h[wave][d], 1u);
+    }
   }
   __syncthreads();
   hist[(long long)tid * nblocks + blockIdx.x] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
@@ -149,7 +215,9 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
 constexpr int kRsWaveItems = kScanTile / 4;  // 1024 items per wave
 constexpr int kRsRounds = kRsWaveItems / 64;  // 16
 
-// With kV2, a second value array rides along with the first.
+// With kV2, a second value array rides along with the first (staged through
+// LDS after the keys and values, so its stores are runs too). Without vals
+// (null), item j's value is j.
 template <bool kV2>
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ vals,
@@ -158,12 +226,14 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
                                                        uint32_t* __restrict__ vals_out,
                                                        uint32_t* __restrict__ vals2_out, long long n_cap,
                                                        const uint32_t* __restrict__ n_dev, int shift, int bits,
-                                                       const uint32_t* __restrict__ hist, int nblocks) {
+                                                       int drop, const uint32_t* __restrict__ hist,
+                                                       const uint32_t* __restrict__ hist_prefix, int nblocks) {
   const long long n = item_count(n_cap, n_dev);
   if ((long long)blockIdx.x * kScanTile >= n) return;  // whole block past the items (uniform)
   __shared__ uint32_t s_cnt[4][256];   // per-wave digit counts, then per-wave digit offsets
   __shared__ uint32_t s_start[256];    // block-local start of each digit
   __shared__ uint32_t s_global[256];   // global start of each digit for this block
+  __shared__ uint32_t s_count;         // items of the block kept
   __shared__ uint32_t s_keys[kScanTile];
   __shared__ uint32_t s_vals[kScanTile];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -172,7 +242,12 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   const long long wbase = base + (long long)wave * kRsWaveItems;
 #pragma unroll
   for (int w = 0; w < 4; ++w) s_cnt[w][tid] = 0;
-  s_global[tid] = hist[(long long)tid * nblocks + blockIdx.x];
+  {
+    // the block-level scan of the digit table, plus its second level's
+    // prefix when the table spans several scan blocks
+    const long long e = (long long)tid * nblocks + blockIdx.x;
+    s_global[tid] = hist[e] + (hist_prefix ? hist_prefix[e / kScanTile] : 0u);
+  }
   __syncthreads();
   const uint64_t below = __lanemask_lt();
   uint32_t k[kRsRounds], v[kRsRounds], v2[kV2 ? kRsRounds : 1], rank[kRsRounds];
@@ -181,9 +256,9 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
 #pragma unroll
   for (int r = 0; r < kRsRounds; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
-    const bool valid = j < n;
-    k[r] = valid ? keys[j] : 0xFFFFFFFFu;
-    v[r] = valid ? vals[j] : 0u;
+    k[r] = j < n ? keys[j] : kDropKey;
+    const bool valid = j < n && !(drop && k[r] == kDropKey);
+    v[r] = valid ? (vals ? vals[j] : (uint32_t)j) : 0u;
     if constexpr (kV2) v2[r] = valid ? vals2[j] : 0u;
     const uint32_t d = (k[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
@@ -229,6 +304,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     uint32_t add = 0;
     for (int w = 0; w < wave; ++w) add += s_wsum[w];
     s_start[tid] = x - total + add;
+    if (tid == 255) s_count = x + add;
   }
   __syncthreads();
 #pragma unroll
@@ -238,18 +314,29 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     const uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
     s_keys[pos] = k[r];
     s_vals[pos] = v[r];
-    // the second value goes straight to its final place (no LDS staging)
-    if constexpr (kV2) vals2_out[s_global[d] + (pos - s_start[d])] = v2[r];
   }
   __syncthreads();
-  const long long remaining = n - base;
-  const int count = remaining < kScanTile ? (int)remaining : kScanTile;
+  const int count = (int)s_count;
   for (int p = tid; p < count; p += 256) {
     const uint32_t kk = s_keys[p];
     const uint32_t d = (kk >> shift) & mask;
     const uint32_t dst = s_global[d] + (uint32_t)p - s_start[d];
     keys_out[dst] = kk;
     vals_out[dst] = s_vals[p];
+  }
+  if constexpr (kV2) {
+    __syncthreads();  // every value is out of s_vals: it takes the second values
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+      if (rank[r] == 0xFFFFFFFFu) continue;
+      const uint32_t d = (k[r] >> shift) & mask;
+      s_vals[s_start[d] + s_cnt[wave][d] + rank[r]] = v2[r];
+    }
+    __syncthreads();
+    for (int p = tid; p < count; p += 256) {
+      const uint32_t d = (s_keys[p] >> shift) & mask;
+      vals2_out[s_global[d] + (uint32_t)p - s_start[d]] = s_vals[p];
+    }
   }
 }
 
@@ -308,33 +395,64 @@ long long scan_workspace_elems(long long max_n) {
 long long sort_hist_elems(long long max_n) { return 256LL * ((max_n + kScanTile - 1) / kScanTile + 1); }
 
 hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx, uint32_t* out, long long n,
-                                 uint32_t* total, const ScanWorkspace& ws, hipStream_t stream) {
+                                 uint32_t* total, const ScanWorkspace& ws, hipStream_t stream, int xform,
+                                 const uint32_t* n_dev, uint32_t* side_total, int side_xform) {
   if (n <= 0) return total ? launch_fill_u32(total, 0u, 1, stream) : hipSuccess;
   if (n > ws.max_n) return hipErrorInvalidValue;
-  return scan_level(in, gather_idx, out, n, total, ws.buf, stream);
+  return scan_level(in, gather_idx, out, n, n_dev, xform, total, side_total, side_xform, ws.buf, stream);
+}
+
+// An in-place exclusive scan of data[0..n) whose readers add the second
+// level themselves: afterwards the scan of element i is data[i] +
+// (*prefix ? (*prefix)[i / kScanTile] : 0). Two levels when n fits them (the
+// last level's add is skipped); otherwise the full scan, and *prefix = null.
+static hipError_t scan_no_add(uint32_t* data, long long n, uint32_t* total, const ScanWorkspace& ws,
+                              const uint32_t** prefix, hipStream_t stream) {
+  *prefix = nullptr;
+  const long long nb = (n + kScanTile - 1) / kScanTile;
+  if (nb <= 1 || nb > kScanTile || n > ws.max_n)
+    return launch_exclusive_scan(data, nullptr, data, n, total, ws, stream);
+  uint32_t* sums = ws.buf;
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, data, nullptr, data, n, nullptr,
+                     (int)kScanPlain, sums, nullptr, nullptr, (int)kScanPlain);
+  SPLAT_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, sums, nullptr, sums, nb, nullptr,
+                     (int)kScanPlain, nullptr, total, nullptr, (int)kScanPlain);
+  *prefix = sums;
+  return hipGetLastError();
 }
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
-                             hipStream_t stream, const uint32_t* n_dev, uint32_t* vals2, uint32_t* vals2_alt) {
+                             hipStream_t stream, const uint32_t* n_dev, uint32_t* vals2, uint32_t* vals2_alt,
+                             bool identity_vals, uint32_t* n_kept) {
   *result_in_alt = false;
-  if (end_bit <= begin_bit || n <= (n_dev ? 0 : 1)) return hipSuccess;
+  if (end_bit <= begin_bit || n <= 0) return hipSuccess;
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (256 * nb > ws.hist_capacity) return hipErrorInvalidValue;
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt, *wi = vals2, *wo = vals2_alt;
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     const int bits = (end_bit - shift) < 8 ? (end_bit - shift) : 8;
-    const uint32_t mask = (1u << bits) - 1u;
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, n_dev, shift, mask, ws.hist,
-                       (int)nb);
-    SPLAT_HIP_CHECK(hipGetLastError());
-    SPLAT_HIP_CHECK(launch_exclusive_scan(ws.hist, nullptr, ws.hist, 256 * nb, nullptr, ws.scan, stream));
-    if (vals2)
-      hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, wi, ko, vo, wo, n,
-                         n_dev, shift, bits, ws.hist, (int)nb);
+    const bool first = shift == begin_bit;
+    // with n_kept, the first pass drops kDropKey and the later ones sort what it kept
+    const int drop = n_kept && first ? 1 : 0;
+    const uint32_t* count = n_kept && !first ? n_kept : n_dev;
+    if (bits <= 4 || shift + bits == 32)
+      hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, count, shift, bits, drop,
+                         ws.hist, (int)nb);
     else
-      hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vi, nullptr, ko, vo,
-                         nullptr, n, n_dev, shift, bits, ws.hist, (int)nb);
+      hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, count, shift, bits,
+                         drop, ws.hist, (int)nb);
+    SPLAT_HIP_CHECK(hipGetLastError());
+    const uint32_t* prefix = nullptr;
+    SPLAT_HIP_CHECK(scan_no_add(ws.hist, 256 * nb, drop ? n_kept : nullptr, ws.scan, &prefix, stream));
+    const uint32_t* vin = (identity_vals && first) ? nullptr : vi;
+    if (vals2)
+      hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vin, wi, ko, vo, wo, n,
+                         count, shift, bits, drop, ws.hist, prefix, (int)nb);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vin, nullptr, ko, vo,
+                         nullptr, n, count, shift, bits, drop, ws.hist, prefix, (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     uint32_t* t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;

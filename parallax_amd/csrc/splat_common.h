@@ -76,6 +76,30 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 // Depth in an unsigned key that sorts like the float (depth >= 0.01 > 0).
 __device__ __forceinline__ uint32_t depth_key(float d) { return __float_as_uint(d); }
 
+// A slot's tile rect packed into 32 bits for the binning, where every view of
+// the batch has at most kPackedTiles tile columns and rows (4096 pixels):
+// x0 | x1 << 8 | y0 << 16 | y1 << 24, inclusive; kNoRect (x0 > x1) for a
+// slot that enters no tile.
+constexpr int kPackedTiles = 256;
+constexpr uint32_t kNoRect = 0x000000FFu;
+__host__ __device__ __forceinline__ uint32_t pack_rect(int x0, int x1, int y0, int y1) {
+  return (uint32_t)x0 | (uint32_t)x1 << 8 | (uint32_t)y0 << 16 | (uint32_t)y1 << 24;
+}
+__host__ __device__ __forceinline__ int rect_x0(uint32_t r) { return (int)(r & 255u); }
+__host__ __device__ __forceinline__ int rect_x1(uint32_t r) { return (int)((r >> 8) & 255u); }
+__host__ __device__ __forceinline__ int rect_y0(uint32_t r) { return (int)((r >> 16) & 255u); }
+__host__ __device__ __forceinline__ int rect_y1(uint32_t r) { return (int)(r >> 24); }
+// tiles of a packed rect
+__host__ __device__ __forceinline__ uint32_t rect_tiles(uint32_t r) {
+  if (rect_x0(r) > rect_x1(r)) return 0u;
+  return (uint32_t)((rect_x1(r) - rect_x0(r) + 1) * (rect_y1(r) - rect_y0(r) + 1));
+}
+// super-tiles (kSuper x kSuper tiles) of a packed rect
+__host__ __device__ __forceinline__ uint32_t rect_supers(uint32_t r) {
+  if (rect_x0(r) > rect_x1(r)) return 0u;
+  return (uint32_t)((rect_x1(r) / kSuper - rect_x0(r) / kSuper + 1) * (rect_y1(r) / kSuper - rect_y0(r) / kSuper + 1));
+}
+
 }  // namespace splat
 
 #define SPLAT_HIP_CHECK(expr)                                   \

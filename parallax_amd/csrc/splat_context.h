@@ -79,6 +79,10 @@ struct splat_context {
   DeviceBuffer pair_vals, ranges;                   // per-tile lists (slots), [begin, end) per tile
   unsigned long long S = 0;                         // super-tile pairs of the batch
   int n_st = 0;                                     // super-tiles of the batch
+  // every view at most kPackedTiles tiles a side: packed rects ride along with
+  // the depth sort (in scounts), and the emitter reads them in depth order
+  bool packed = false;
+  uint32_t* sorted_prects = nullptr;
   uint32_t* sorted_slots = nullptr;
   uint32_t* sorted_pair_vals = nullptr;
   // capacities the pair-count-sized buffers were reserved for (kernel bounds)

@@ -9,12 +9,15 @@
 namespace splat {
 
 // ---- projection ----
-// counts[slot] = tiles of the slot's rect, scounts[slot] = super-tiles of it;
-// rects[slot] = the record's tile rect again ((x0 | x1 << 16, y0 | y1 << 16),
-// 8 bytes a slot) for the binning's gathers in depth order.
+// rec[slot] and keys[slot] (depth bits; 0xFFFFFFFF for a culled slot). With
+// prects (every view of the batch at most kPackedTiles tiles a side):
+// prects[slot] = the packed tile rect (kNoRect for a culled slot), and counts,
+// scounts, rects are not written. Otherwise counts[slot] = tiles of the
+// slot's rect, scounts[slot] = super-tiles of it, rects[slot] = the rect
+// ((x0 | x1 << 16, y0 | y1 << 16), 8 bytes) for the binning's gathers.
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
-                          uint32_t* keys, uint32_t* vals, uint32_t* counts, uint32_t* scounts, uint2* rects,
+                          uint32_t* keys, uint32_t* counts, uint32_t* scounts, uint2* rects, uint32_t* prects,
                           hipStream_t stream);
 
 // ---- scan / sort ----
@@ -30,9 +33,17 @@ struct ScanWorkspace {
 };
 long long scan_workspace_elems(long long max_n);
 
-// out[i] = sum_{j<i} in[idx ? idx[j] : j]; *total (device, nullable) = sum of all.
+// What a scan takes of each input word: the word, or the tiles / super-tiles
+// of a packed rect.
+enum { kScanPlain = 0, kScanRectTiles = 1, kScanRectSupers = 2 };
+// out[i] = sum_{j<i} f(in[idx ? idx[j] : j]); *total (device, nullable) = sum
+// of all; f per xform. With n_dev the item count is min(*n_dev, n) (n sizes
+// the launches); out past it is not written. With side_total (device),
+// *side_total += the sum of side_xform of the same inputs.
 hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx, uint32_t* out, long long n,
-                                 uint32_t* total, const ScanWorkspace& ws, hipStream_t stream);
+                                 uint32_t* total, const ScanWorkspace& ws, hipStream_t stream,
+                                 int xform = kScanPlain, const uint32_t* n_dev = nullptr,
+                                 uint32_t* side_total = nullptr, int side_xform = kScanPlain);
 
 struct SortWorkspace {
   uint32_t* hist = nullptr;  // 256 * blocks
@@ -48,11 +59,14 @@ long long sort_hist_elems(long long max_n);
 // device (clamped to n, which then only sizes the launches), so the sort can
 // sit in a captured graph whose item count changes from replay to replay.
 // With vals2 (and vals2_alt), a second value array is permuted alongside and
-// ends in the same buffer of the pair as the first.
+// ends in the same buffer of the pair as the first. With identity_vals, the
+// values start as the item indices (vals is not read, only written). With
+// n_kept (device), items whose key is 0xFFFFFFFF are dropped, *n_kept is set
+// to the count of the rest, and the sorted output holds those only.
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
                              hipStream_t stream, const uint32_t* n_dev = nullptr, uint32_t* vals2 = nullptr,
-                             uint32_t* vals2_alt = nullptr);
+                             uint32_t* vals2_alt = nullptr, bool identity_vals = false, uint32_t* n_kept = nullptr);
 // *out (device) += the sum of in[0..n).
 hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream);
 
@@ -61,14 +75,18 @@ constexpr int kBinSegment = 256;   // splats per segment of a super-tile list
 constexpr int kStBits = 20;        // super-tile id bits of a super-tile key
 constexpr uint32_t kStMask = (1u << kStBits) - 1u;
 
-// For each depth-sorted slot s: writes one (super-tile key, slot) pair per
-// super-tile of its rect at soffsets[s] (*s_total: all pairs); a key holds the
-// global super-tile id in its low kStBits bits and the splat's rect inside
-// that super-tile above them. With pair_base: pair_base[slot] = offsets[s].
-// Pairs at or past s_cap are dropped (the capacity check flags the step).
+// For each depth-sorted slot s < min(m, *m_dev): writes one (super-tile key,
+// slot) pair per super-tile of its rect at soffsets[s] (*s_total: all pairs);
+// a key holds the global super-tile id in its low kStBits bits and the
+// splat's rect inside that super-tile above them. With pair_base:
+// pair_base[slot] = offsets[s]. Pairs at or past s_cap are dropped (the
+// capacity check flags the step).
+// The rects are read from sorted_prects (the packed rects in depth order)
+// when given, else gathered from rects[slot].
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
-                             const uint32_t* s_total, const uint2* rects, const ViewDesc* views, long long per_view,
-                             long long m, uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
+                             const uint32_t* s_total, const uint2* rects, const uint32_t* sorted_prects,
+                             const ViewDesc* views, long long per_view, long long m, const uint32_t* m_dev,
+                             uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
                              hipStream_t stream);
 // sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted
 // pairs; the pair count is min(*n_dev, n) with n_dev, else n.
@@ -176,14 +194,14 @@ struct AdjointParams {
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
                           const float* screen, float* grads, float* sg, int* vc, hipStream_t stream);
 // Deterministic mode: screen[slot] = the slot's per-tile partials (its run
-// [offsets[i], offsets[i+1]) of depth-sorted position i, the last run ending
-// at *p_total, all clamped to p_cap) summed in a fixed order. `carry` holds
-// det_reduce_carry_floats(p_cap) floats.
+// [offsets[i], offsets[i+1]) of depth-sorted position i < min(m, *m_dev),
+// the last run ending at *p_total, all clamped to p_cap) summed in a fixed
+// order. `carry` holds det_reduce_carry_floats(p_cap) floats.
 // Records tagged with another epoch than *epoch count as zeros.
 long long det_reduce_carry_floats(unsigned long long p_cap);
 hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
-                             long long m, uint32_t p_cap, const float* partial, const uint32_t* epoch, float* screen,
-                             float* carry, hipStream_t stream);
+                             long long m, const uint32_t* m_dev, uint32_t p_cap, const float* partial,
+                             const uint32_t* epoch, float* screen, float* carry, hipStream_t stream);
 // The next epoch of the deterministic partials: *epoch + 1, skipping 0 (the
 // tag of a cleared buffer). A device-side counter, so each replay of a
 // captured step takes a new one; a record can be mistaken for current only
