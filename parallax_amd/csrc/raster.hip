@@ -539,9 +539,18 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           const float4 a = s_a[j];
           const float4 b = s_b[j];
           const float cblue = s_c[j];
+          // the accumulators start as zeros held in registers (the empty asm
+          // keeps the compiler from folding them into the first quadrant's
+          // adds, which would cost a copy of all eleven on every path that
+          // skips that quadrant); term 10, the pixels that shaped the splat,
+          // is counted in a scalar register from ballots
           float acc[kVals];
 #pragma unroll
-          for (int i = 0; i < kVals; ++i) acc[i] = 0.f;
+          for (int i = 0; i < kVals - 1; ++i) {
+            acc[i] = 0.f;
+            asm volatile("" : "+v"(acc[i]));
+          }
+          uint32_t shaped = 0u;
           bool any = false;
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
@@ -587,7 +596,7 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             acc[7] -= px * dx;  // halved after the wave sum
             acc[8] -= px * dy;
             acc[9] -= py * dy;  // halved after the wave sum
-            acc[10] += shape ? 1.f : 0.f;
+            shaped += (uint32_t)__popcll(__ballot(shape));
             T[k] = contrib ? Tn : T[k];
             suf0[k] += alpha_c * d0;
             suf1[k] += alpha_c * d1;
@@ -595,7 +604,10 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
             }
           }
           float z = 0.f;
-          if (any) z = wave_sum11(acc, lane) * slot_scale;
+          if (any) {
+            acc[10] = lane == 0 ? (float)shaped : 0.f;  // an exact integer through the wave sum
+            z = wave_sum11(acc, lane) * slot_scale;
+          }
           // the 12 lead lanes hold the splat's wave totals: one instruction
           // covers the splat's contiguous record -- atomics (nonzero terms
           // only) or the partial's 64-byte store (lanes 3, 19, 35, 51 add
