@@ -228,7 +228,8 @@ def main():
     }
     design = {
         "adam": 32.0 * n * (14 + rs),  # + the gradient clear
-        "project": n * param_bytes + n * B * 84.0,  # 64 B record + key, value, count, super-tile count, 8 B rect
+        # 64 B record + depth key + packed tile rect (views up to 4096 px; 84 B a slot beyond)
+        "project": n * param_bytes + n * B * 72.0,
         "adjoint": 3.0 * n * param_bytes + n * B * (64 + 48),  # gradients read and written; 64 B record, 48 B screen
         "loss": px_batch * LOSS_DESIGN_BYTES_PER_PX,
         "tile_write": 4.0 * pairs,
