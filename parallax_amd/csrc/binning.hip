@@ -268,27 +268,41 @@ __global__ __launch_bounds__(256) void k_tile_count(const uint32_t* __restrict__
 // the list's base.
 constexpr int kSegChunks = kBinSegment / 64;
 
+// One segment's runs, tile by tile: the covering lanes of each chunk place
+// their slots at their rank in the wave's LDS slice (the chunk's ballot is
+// the store's execution mask), then the tile's whole run for the segment goes
+// to its place in one coalesced store per 64 entries. Tiles alternate between
+// two LDS buffers, so a tile's placement needs no wait for the previous
+// tile's read; the LDS keeps one wave's accesses in order, and the wave
+// barriers keep the compiler from moving them across each other.
 template <bool kNarrow, bool kBound>
 __device__ __forceinline__ void write_segment_runs(uint32_t base, const uint32_t (&slot)[kSegChunks],
                                                    const uint32_t (&cov)[kSegChunks][2], uint32_t p_cap,
-                                                   uint32_t* __restrict__ pair_vals) {
+                                                   uint32_t* __restrict__ pair_vals, uint32_t* __restrict__ s_runs) {
+  const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
   for (int t = 0; t < 64; ++t) {
-    uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
+    uint32_t* s_run = s_runs + (t & 1) * kBinSegment;
+    const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)base, t);
+    uint32_t n = 0;  // entries of the tile's run so far
 #pragma unroll
     for (int c = 0; c < kSegChunks; ++c) {
       const uint64_t b = __ballot((cov[c][t >> 5] >> (t & 31)) & 1u);
-      uint32_t to = at + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-      if constexpr (kBound) to = min(to, p_cap);
-      if (__builtin_amdgcn_inverse_ballot_w64(b)) {
-        if constexpr (kNarrow)
-          *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(pair_vals) + (uint32_t)(to << 2)) = slot[c];
-        else
-          pair_vals[to] = slot[c];
-      }
-      at += (uint32_t)__popcll(b);
+      const uint32_t k = n + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+      if (__builtin_amdgcn_inverse_ballot_w64(b)) s_run[k] = slot[c];
+      n += (uint32_t)__popcll(b);
     }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < n; k += 64) {
+      uint32_t to = at + k;
+      if constexpr (kBound) to = min(to, p_cap);
+      if constexpr (kNarrow)
+        *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(pair_vals) + (uint32_t)(to << 2)) = s_run[k];
+      else
+        pair_vals[to] = s_run[k];
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -303,7 +317,9 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ scan, uint32_t p_cap,
                                                     uint32_t* __restrict__ pair_vals) {
   static_assert(kBinSegment % 64 == 0 && kSuper == 8, "segments of whole chunks, 8x8 super-tiles");
+  __shared__ uint32_t s_runs[4][2 * kBinSegment];  // per wave: two tiles' runs of the segment
   const int lane = threadIdx.x & 63;
+  uint32_t* s_run = s_runs[threadIdx.x >> 6];
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   const uint32_t ns = *ns_total;
   for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < ns; g += waves) {
@@ -329,9 +345,9 @@ __global__ __launch_bounds__(256) void k_tile_write(const uint32_t* __restrict__
     // bound
     const uint32_t seg_end = scan[S.table0 + 63ull * S.nsg + S.s + 1];
     if (seg_end <= p_cap)
-      write_segment_runs<kNarrow, false>(base, slot, cov, p_cap, pair_vals);
+      write_segment_runs<kNarrow, false>(base, slot, cov, p_cap, pair_vals, s_run);
     else
-      write_segment_runs<kNarrow, true>(base, slot, cov, p_cap, pair_vals);
+      write_segment_runs<kNarrow, true>(base, slot, cov, p_cap, pair_vals, s_run);
   }
 }
 
