@@ -220,17 +220,17 @@ def main():
     param_bytes = 56 + 4 * rs  # one SH3 gaussian's parameters
     px_batch = sum(views[i].pixels for i in batches[0])
     alg = {
-        "adam": 28.0 * n * (14 + rs),  # read p, g, m, v; write p, m, v
         "project": n * param_bytes + n * B * 56.0,  # parameters; 52 B record + 4 B tile count per slot
-        "adjoint": n * (2.0 * param_bytes + B * (52 + 40)),  # parameters, gradients; record + 10 screen terms
+        # the per-gaussian adjoint with Adam fused (the training step's form): record + 10 screen terms per
+        # slot; read p, g, m, v and write p, m, v per parameter (the gradient stays in registers)
+        "adjoint_adam": n * B * (52 + 40) + 28.0 * n * (14 + rs),
         "loss": px_batch * 36.0,  # rgb and target in, dL/drgb out
         "tile_write": 4.0 * pairs,  # the per-tile lists
     }
     design = {
-        "adam": 32.0 * n * (14 + rs),  # + the gradient clear
         # 64 B record + depth key + packed tile rect (views up to 4096 px; 84 B a slot beyond)
         "project": n * param_bytes + n * B * 72.0,
-        "adjoint": 3.0 * n * param_bytes + n * B * (64 + 48),  # gradients read and written; 64 B record, 48 B screen
+        "adjoint_adam": n * B * (64 + 48) + 32.0 * n * (14 + rs),  # 64 B record, 48 B screen; + the gradient clear
         "loss": px_batch * LOSS_DESIGN_BYTES_PER_PX,
         "tile_write": 4.0 * pairs,
     }

@@ -84,11 +84,61 @@ __device__ __forceinline__ void sh_jacobian(int count, const float d[3], float j
   set(14, kShC3[6], 3.f * xx - 3.f * yy, -6.f * x * y, 0.f);
 }
 
-template <int kDeg>
+// The Adam step of one parameter, as k_adam takes it (SplatTrainer.cpp:56-65).
+__device__ __forceinline__ float adam_delta(float& m, float& v, float g, float lr, const AdamStep& p) {
+  m = p.beta1 * m + (1.f - p.beta1) * g;
+  v = p.beta2 * v + (1.f - p.beta2) * g * g;
+  const float m_hat = m / p.c1;
+  const float v_hat = v / p.c2;
+  return -lr * m_hat / (__builtin_sqrtf(v_hat) + p.eps);
+}
+
+// Adam on kN consecutive parameters at o: every operand loaded first (the
+// four arrays are distinct, so the loads go out together), then the update,
+// then the stores. `touched`: this adjoint added gradient (the plain adjoint
+// leaves an untouched gaussian's stored gradient as it is).
+template <int kN>
+__device__ __forceinline__ void adam_block(long long o, const float* gnew, bool touched, float lr, const AdamStep& P,
+                                           float* __restrict__ pw, float* __restrict__ gw, float* __restrict__ mw,
+                                           float* __restrict__ vw) {
+  float pp[kN], gg[kN], mm[kN], vv[kN];
+#pragma unroll
+  for (int i = 0; i < kN; ++i) {
+    pp[i] = pw[o + i];
+    gg[i] = gw[o + i];
+    mm[i] = mw[o + i];
+    vv[i] = vw[o + i];
+  }
+#pragma unroll
+  for (int i = 0; i < kN; ++i) {
+    const float g = touched && gnew ? gg[i] + gnew[i] : gg[i];
+    pp[i] = pp[i] + adam_delta(mm[i], vv[i], g, lr, P);
+  }
+#pragma unroll
+  for (int i = 0; i < kN; ++i) {
+    pw[o + i] = pp[i];
+    mw[o + i] = mm[i];
+    vw[o + i] = vv[i];
+    gw[o + i] = 0.f;
+  }
+}
+
+// With kAdam (a training step): unless *skip is set, the gaussian's gradient
+// -- the stored one plus this adjoint's, added exactly as the plain adjoint
+// adds it -- goes straight into the Adam update of all its parameters (the
+// quaternion renormalised after), the stored gradient is cleared and the
+// step's densify statistics are folded, exactly as k_adam would do after the
+// plain adjoint; the parameters are read once and the gradients never
+// written back.
+template <int kDeg, bool kAdam>
 __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc* __restrict__ views,
-                                                 const float* __restrict__ params, const SplatRec* __restrict__ rec,
+                                                 const float* params_in, const SplatRec* __restrict__ rec,
                                                  const float* __restrict__ screen, float* __restrict__ grads,
-                                                 float* __restrict__ sg, int* __restrict__ vc) {
+                                                 float* __restrict__ sg, int* __restrict__ vc, AdamFuse fuse) {
+  const float* params = params_in;  // not restrict: the fused Adam writes the same buffer
+  // the fused Adam's rows of SH rest gradients, one slice per wave
+  __shared__ float s_rest[kAdam ? 4 : 1][kAdam ? 64 * 45 : 1];
+  __shared__ int s_used[kAdam ? 4 : 1][kAdam ? 64 : 1];
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.n) return;
   const long long cap = p.cap;
@@ -298,6 +348,92 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
     const float ud = unit[0] * dq[0] + unit[1] * dq[1] + unit[2] * dq[2] + unit[3] * dq[3];
     for (int a = 0; a < 4; ++a) grot[a] += (dq[a] - unit[a] * ud) / raw_norm;
   }
+  if (kAdam && !(fuse.skip && *fuse.skip)) {
+    const AdamStep P = *fuse.step;
+    float* __restrict__ pw = fuse.params;  // the same buffer as params, written here
+    float* __restrict__ gw = grads;
+    float* __restrict__ mw = fuse.m;
+    float* __restrict__ vw = fuse.v;
+    const bool touched = vc_sum != 0;
+    adam_block<3>(3 * (long long)g, gpos, touched, P.lr[0], P, pw, gw, mw, vw);
+    adam_block<3>(3 * cap + 3 * (long long)g, gls, touched, P.lr[1], P, pw, gw, mw, vw);
+    {
+      const long long o = 6 * cap + 4 * (long long)g;
+      float q[4], gg[4], mm[4], vv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        q[a] = pw[o + a];
+        gg[a] = gw[o + a];
+        mm[a] = mw[o + a];
+        vv[a] = vw[o + a];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) q[a] = q[a] + adam_delta(mm[a], vv[a], touched ? gg[a] + grot[a] : gg[a], P.lr[2], P);
+      // Eigen norm() / normalize() over the (x, y, z, w) coefficients
+      const float sq = q[1] * q[1] + q[2] * q[2] + q[3] * q[3] + q[0] * q[0];
+      const float nrm = __builtin_sqrtf(sq);
+      if (nrm > 1e-6f) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) q[a] = q[a] / nrm;
+      } else {
+        q[0] = 1.f;
+        q[1] = q[2] = q[3] = 0.f;
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        pw[o + a] = q[a];
+        mw[o + a] = mm[a];
+        vw[o + a] = vv[a];
+        gw[o + a] = 0.f;
+      }
+    }
+    adam_block<1>(10 * cap + g, &gop, touched, P.lr[3], P, pw, gw, mw, vw);
+    adam_block<3>(11 * cap + 3 * (long long)g, gdc, touched, P.lr[4], P, pw, gw, mw, vw);
+    // The SH rest coefficients, every stored band: the wave's gaussians hold
+    // one contiguous block (rs floats each), so each lane puts its new
+    // gradient row in the wave's LDS slice and the wave then updates the block
+    // with coalesced accesses. Active lanes are a prefix of the wave (threads
+    // past n have left), and they stride the block together.
+    if (rs > 0) {
+      const int lane = threadIdx.x & 63;
+      float* row = s_rest[threadIdx.x >> 6];
+      int* used = s_used[threadIdx.x >> 6];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int k = 0; k < kCount; ++k) row[lane * rs + c * per_channel + k] = grc[c][k];
+      }
+      used[lane] = touched ? 1 : 0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const int nv = __popcll(__ballot(true));
+      const long long base = 14 * cap + (long long)rs * (g - lane);
+      const float inv_rs = 1.f / (float)rs;
+      for (int e = lane; e < nv * rs; e += nv) {
+        int j = (int)((float)e * inv_rs);  // gaussian of element e (exact after the correction)
+        if (j * rs > e) --j;
+        else if ((j + 1) * rs <= e) ++j;
+        int k = e - j * rs;  // coefficient within the gaussian's row, then within its channel
+        k = k >= 2 * per_channel ? k - 2 * per_channel : (k >= per_channel ? k - per_channel : k);
+        const bool add = used[j] != 0 && k < kCount;  // as the plain adjoint adds (used bands, touched gaussians)
+        const long long o = base + e;
+        const float pp = pw[o], gg = gw[o];
+        float mm = mw[o], vv = vw[o];
+        const float gt = add ? gg + row[e] : gg;
+        pw[o] = pp + adam_delta(mm, vv, gt, P.lr[5], P);
+        mw[o] = mm;
+        vw[o] = vv;
+        gw[o] = 0.f;
+      }
+    }
+    const float sg_step = touched ? sg[g] + sg_sum : sg[g];
+    const int vc_step = vc[g] + vc_sum;
+    fuse.stat_sg[g] += sg_step;
+    fuse.stat_vc[g] += vc_step;
+    sg[g] = 0.f;
+    vc[g] = 0;
+    return;
+  }
   if (vc_sum == 0) return;
   for (int i = 0; i < 3; ++i) {
     grads[3 * g + i] += gpos[i];
@@ -504,25 +640,32 @@ hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offse
   return hipGetLastError();
 }
 
+namespace {
+template <int kDeg>
+void adjoint_variant(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
+                     const float* screen, float* grads, float* sg, int* vc, const AdamFuse* fuse,
+                     hipStream_t stream) {
+  const dim3 grid((p.n + 255) / 256);
+  if (fuse)
+    hipLaunchKernelGGL((k_adjoint<kDeg, true>), grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg,
+                       vc, *fuse);
+  else
+    hipLaunchKernelGGL((k_adjoint<kDeg, false>), grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg,
+                       vc, AdamFuse{});
+}
+}  // namespace
+
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
-                          const float* screen, float* grads, float* sg, int* vc, hipStream_t stream) {
+                          const float* screen, float* grads, float* sg, int* vc, hipStream_t stream,
+                          const AdamFuse* fuse) {
   if (p.n <= 0) return hipSuccess;
   if (3 * (p.sh_use == 0 ? 0 : (p.sh_use == 1 ? 3 : (p.sh_use == 2 ? 8 : 15))) > p.rest_stride)
     return hipErrorInvalidValue;
-  const dim3 grid((p.n + 255) / 256);
   switch (p.sh_use) {
-    case 0:
-      hipLaunchKernelGGL(k_adjoint<0>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
-      break;
-    case 1:
-      hipLaunchKernelGGL(k_adjoint<1>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_adjoint<2>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
-      break;
-    default:
-      hipLaunchKernelGGL(k_adjoint<3>, grid, dim3(256), 0, stream, p, views, params, rec, screen, grads, sg, vc);
-      break;
+    case 0: adjoint_variant<0>(p, views, params, rec, screen, grads, sg, vc, fuse, stream); break;
+    case 1: adjoint_variant<1>(p, views, params, rec, screen, grads, sg, vc, fuse, stream); break;
+    case 2: adjoint_variant<2>(p, views, params, rec, screen, grads, sg, vc, fuse, stream); break;
+    default: adjoint_variant<3>(p, views, params, rec, screen, grads, sg, vc, fuse, stream); break;
   }
   return hipGetLastError();
 }

@@ -573,7 +573,9 @@ splat_status loss_impl(splat_context* ctx, float w, float* losses) {
   return SPLAT_OK;
 }
 
-splat_status enqueue_backward(splat_context* ctx) {
+// With fuse_adam (a training step), the per-gaussian adjoint also applies
+// the Adam step staged in d_adam, unless *adam_skip is set.
+splat_status enqueue_backward(splat_context* ctx, bool fuse_adam = false, const uint32_t* adam_skip = nullptr) {
   const long long M = ctx->M;
   const bool det = ctx->opts.deterministic != 0;
   CK(ctx->screen.reserve(sizeof(float) * kScreenStride * std::max(1LL, M)), "alloc screen");
@@ -627,8 +629,21 @@ splat_status enqueue_backward(splat_context* ctx) {
     TIMED("det_reduce", launch_det_reduce(ctx->sorted_slots, ctx->offsets.u(), ctx->scalars.u(), M,
                                           ctx->scalars.u() + kSortedScalar, p_cap, ctx->partial.f(), epoch,
                                           ctx->screen.f(), ctx->carry.f(), ctx->stream));
-  TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
-                                  ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream));
+  if (fuse_adam) {
+    AdamFuse fuse;
+    fuse.step = static_cast<const AdamStep*>(ctx->d_adam.ptr);
+    fuse.skip = adam_skip;
+    fuse.params = ctx->params.f();
+    fuse.m = ctx->m.f();
+    fuse.v = ctx->v.f();
+    fuse.stat_sg = ctx->stat_sg.f();
+    fuse.stat_vc = ctx->stat_vc.i();
+    TIMED("adjoint_adam", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
+                                         ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream, &fuse));
+  } else {
+    TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
+                                    ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream));
+  }
   return SPLAT_OK;
 }
 
@@ -680,8 +695,8 @@ splat_status train_step_plain(splat_context* ctx, int n_views, const int32_t* vi
                               const splat_raster_options* o, float w, const splat_adam_params* adam, float* losses) {
   if (splat_status s = forward_impl(ctx, n_views, view_ids, o)) return s;
   if (splat_status s = loss_impl(ctx, w, nullptr)) return s;
-  if (splat_status s = backward_impl(ctx)) return s;
-  if (splat_status s = adam_impl(ctx, adam)) return s;
+  if (splat_status s = stage_adam(ctx, adam)) return s;
+  if (splat_status s = enqueue_backward(ctx, true)) return s;  // the adjoint applies Adam
   if (losses) {
     std::vector<float> sums(2 * n_views);
     CK(hipMemcpyAsync(sums.data(), ctx->loss_sums.f(), sizeof(float) * 2 * n_views, hipMemcpyDeviceToHost,
@@ -766,8 +781,7 @@ splat_status train_step_graph(splat_context* ctx, int n_views, const int32_t* vi
     splat_status s = enqueue_front(ctx);
     if (!s) s = enqueue_back(ctx);
     if (!s) s = enqueue_loss(ctx, w);
-    if (!s) s = enqueue_backward(ctx);
-    if (!s) s = enqueue_adam(ctx, ctx->scalars.u() + 3);
+    if (!s) s = enqueue_backward(ctx, true, ctx->scalars.u() + 3);  // Adam fused; skipped on overflow
     ctx->capturing = false;
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);

@@ -191,8 +191,20 @@ struct AdjointParams {
   int n_views;
   int deterministic;
 };
+struct AdamStep;
+// Adam fused into the per-gaussian adjoint (a training step): see k_adjoint.
+struct AdamFuse {
+  const AdamStep* step = nullptr;
+  const uint32_t* skip = nullptr;  // device flag: when set, the plain adjoint's accumulation only
+  float* params = nullptr;         // the same buffer as the adjoint's params
+  float* m = nullptr;
+  float* v = nullptr;
+  float* stat_sg = nullptr;
+  int* stat_vc = nullptr;
+};
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
-                          const float* screen, float* grads, float* sg, int* vc, hipStream_t stream);
+                          const float* screen, float* grads, float* sg, int* vc, hipStream_t stream,
+                          const AdamFuse* fuse = nullptr);
 // Deterministic mode: screen[slot] = the slot's per-tile partials (its run
 // [offsets[i], offsets[i+1]) of depth-sorted position i < min(m, *m_dev),
 // the last run ending at *p_total, all clamped to p_cap) summed in a fixed

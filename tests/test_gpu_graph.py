@@ -8,6 +8,12 @@ against the plain run, with downloads and synchronisations between replays,
 including a batch whose pair count outgrows the capacities the graphs run
 with (detected on the device, the update skipped, the step redone) and
 changes of batch shape (more graphs).
+
+A training step applies Adam inside the per-gaussian adjoint. The separate
+public calls (splat_render_forward, splat_loss, splat_render_backward,
+splat_adam_step: the plain adjoint, then k_adam) must give the same
+parameters, moments and statistics bit for bit in deterministic mode, with
+every SH band in use and with fewer bands than the cloud stores.
 """
 import numpy as np
 import pytest
@@ -79,3 +85,49 @@ def test_graph_steps_match_plain_steps(ctx, deterministic):
         else:
             assert np.allclose(a[0], b[0], rtol=1e-6), f"step {it}: losses"
             assert rel_l2(b[2], a[2]) <= 1e-4, f"step {it}: screen gradients"
+
+
+def run_separate(c, cloud, views, targets, batches, sh):
+    c.set_graph_mode(False)
+    c.set_capacity(cloud.n, 3)
+    c.upload_cloud(cloud)
+    c.upload_views(views, targets)
+    out = []
+    for it, b in enumerate(batches):
+        c.render_forward(b, S.options(sh_degree=sh, deterministic=True))
+        c.loss(0.2)
+        c.render_backward()
+        c.adam_step(S.adam_params(it, 1e-4))
+        sg, vc = c.stats()
+        m, v = c.moments()
+        out.append((c.download_cloud(), sg, vc, m, v))
+    return out
+
+
+def run_steps(c, cloud, views, targets, batches, sh):
+    c.set_graph_mode(False)
+    c.set_capacity(cloud.n, 3)
+    c.upload_cloud(cloud)
+    c.upload_views(views, targets)
+    out = []
+    for it, b in enumerate(batches):
+        c.train_step(b, S.options(sh_degree=sh, deterministic=True), 0.2, S.adam_params(it, 1e-4))
+        sg, vc = c.stats()
+        m, v = c.moments()
+        out.append((c.download_cloud(), sg, vc, m, v))
+    return out
+
+
+@pytest.mark.parametrize("sh", [3, 1], ids=["sh3", "sh1_of_3"])
+def test_fused_adam_matches_separate_calls(ctx, sh):
+    cloud, views, targets, batches = scene()
+    batches = batches[:5]
+    print(f"[fused] separate calls, sh {sh}", flush=True)
+    sep = run_separate(ctx, cloud, views, targets, batches, sh)
+    print(f"[fused] training steps, sh {sh}", flush=True)
+    fused = run_steps(ctx, cloud, views, targets, batches, sh)
+    for it, (a, b) in enumerate(zip(sep, fused)):
+        for k in ("pos", "log_scale", "rot", "opacity", "dc", "rest"):
+            assert np.array_equal(getattr(a[0], k), getattr(b[0], k)), f"step {it}: {k}"
+        for j, name in ((1, "screen gradient"), (2, "visible count"), (3, "m"), (4, "v")):
+            assert np.array_equal(np.asarray(a[j]), np.asarray(b[j])), f"step {it}: {name}"
