@@ -127,7 +127,7 @@ splat_context::~splat_context() {
 
 std::vector<DeviceBuffer*> splat_context::all_buffers() {
   return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
-          &vals_alt, &counts, &offsets, &pair_base, &scounts, &scounts_alt, &soffsets, &rects, &skeys, &svals, &skeys_alt, &svals_alt,
+          &vals_alt, &counts, &offsets, &pair_base, &scounts, &scounts_alt, &soffsets, &keys_c, &vals_c, &scounts_c, &key_ranges, &rects, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
           &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &carry, &scan_ws, &hist_ws, &scalars,
           &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam,
@@ -180,9 +180,11 @@ namespace {
 
 // The u32 device scalars: [0] tile pairs, [1] super-tile pairs, [3] overflow
 // flag, [4] work-queue counter, [5] projected slots (the depth-sorted ones),
-// [8] the deterministic partials' epoch.
+// [8] the deterministic partials' epoch, [9..14] the depth sort's plan.
 constexpr int kSortedScalar = 5;
 constexpr int kEpochScalar = 8;
+constexpr int kSortPlanScalar = 9;
+constexpr int kScalarWords = kSortPlanScalar + kSortPlanWords;
 
 splat_status check_ready(splat_context* ctx) {
   if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
@@ -257,7 +259,8 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
   CK(ctx->order_vals.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
   CK(ctx->order_keys_alt.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
   CK(ctx->order_vals_alt.reserve(sizeof(uint32_t) * std::max(1, tiles)), "alloc tile order");
-  CK(ctx->scalars.reserve(64), "alloc scalars");
+  CK(ctx->scalars.reserve(4 * 32), "alloc scalars");
+  static_assert(kScalarWords <= 32, "device scalars");
   const long long Mc = std::max(1LL, M);
   CK(ctx->rec.reserve(sizeof(SplatRec) * Mc), "alloc records");
   CK(ctx->keys.reserve(4 * Mc), "alloc keys");
@@ -269,6 +272,10 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
   CK(ctx->pair_base.reserve(4 * Mc), "alloc pair base");
   CK(ctx->scounts.reserve(4 * Mc), "alloc counts");
   CK(ctx->scounts_alt.reserve(4 * Mc), "alloc counts");
+  CK(ctx->keys_c.reserve(4 * Mc), "alloc keys");
+  CK(ctx->vals_c.reserve(4 * Mc), "alloc keys");
+  CK(ctx->scounts_c.reserve(4 * Mc), "alloc counts");
+  CK(ctx->key_ranges.reserve(8 * std::max(1LL, ((long long)ctx->n + 63) / 64)), "alloc key ranges");
   CK(ctx->rects.reserve(8 * Mc), "alloc rects");
   CK(ctx->soffsets.reserve(4 * Mc), "alloc offsets");
   CK(ctx->sranges.reserve(sizeof(uint32_t) * 2 * std::max(1, supers)), "alloc ranges");
@@ -337,16 +344,25 @@ splat_status enqueue_front(splat_context* ctx) {
                                   o->screen_dilation, o->min_alpha, ctx->recs(), ctx->keys.u(),
                                   packed ? nullptr : ctx->counts.u(), packed ? nullptr : ctx->scounts.u(),
                                   packed ? nullptr : reinterpret_cast<uint2*>(ctx->rects.u()),
-                                  packed ? ctx->scounts.u() : nullptr, ctx->stream));
+                                  packed ? ctx->scounts.u() : nullptr, ctx->stream, ctx->key_ranges.u()));
   bool alt = false;
   // the packed rects (or the super-tile counts) ride along with the slots, so
   // what reads them in depth order needs no gather; the slots start as the
   // item indices, and culled slots (key 0xFFFFFFFF) leave in the first pass:
-  // the sorted arrays hold the scalars[kSortedScalar] projected slots
+  // the sorted arrays hold the scalars[kSortedScalar] projected slots. The
+  // keys are sorted less their minimum, in as many 8-bit passes as the
+  // kept keys' range needs (the sort's plan, set on the device).
   uint32_t* n_sorted = d_total + kSortedScalar;
-  TIMED("depth_sort", launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
-                                        ctx->sort_ws, &alt, ctx->stream, nullptr, ctx->scounts.u(),
-                                        ctx->scounts_alt.u(), true, n_sorted));
+  uint32_t* plan_words = d_total + kSortPlanScalar;
+  const SortPlan plan{plan_words, plan_words + 1, plan_words + 5, ctx->keys_c.u(), ctx->vals_c.u(),
+                      ctx->scounts_c.u()};
+  TIMED("depth_sort", [&]() -> hipError_t {
+    const hipError_t e = launch_sort_plan(ctx->key_ranges.u(), (int)((n + 63) / 64), plan_words, ctx->stream);
+    if (e != hipSuccess) return e;
+    return launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
+                             ctx->sort_ws, &alt, ctx->stream, nullptr, ctx->scounts.u(), ctx->scounts_alt.u(), true,
+                             n_sorted, &plan);
+  }());
   ctx->sorted_slots = alt ? ctx->vals_alt.u() : ctx->vals.u();
   const uint32_t* sorted_v2 = alt ? ctx->scounts_alt.u() : ctx->scounts.u();
   ctx->sorted_prects = packed ? const_cast<uint32_t*>(sorted_v2) : nullptr;

@@ -12,29 +12,29 @@
 // planes per channel are written, read and blurred instead of five
 // (dSSIM/dx = B(A) + 2x B(t_var) + y B(t_cov)).
 //
-// Both passes walk down the image. A work unit is a 128-column x 78-row block
+// Both passes walk down the image. A work unit is a 128-column x 86-row block
 // of one view; its waves walk the block's rows top to bottom, from 5 rows
 // above it to 5 rows below (zero outside the image):
 //   - the row blur of a row reads each lane's 11-tap window from that row,
 //     staged in the wave's own LDS slice (no workgroup barrier);
-//   - the column blur keeps, per lane, the last 11 row-blurred values of each
-//     plane in registers (an 11-slot ring, the row loop unrolled by 11 so
-//     every slot index is static) and blurs them as each new row arrives, so
-//     no row is blurred twice within a unit.
+//   - the column blur keeps, per lane, the last 12 row-blurred values of each
+//     plane in registers (a 12-slot ring, the row loop unrolled by 12 so
+//     every slot index is static) and blurs the last 11 as each new row
+//     arrives, so no row is blurred twice within a unit.
 // stats: three waves per half unit (64 columns), one per channel, one lane
 //        per column: the five statistics planes (x, y, xx, yy, xy) per pixel,
 //        then the SSIM value, the three partial planes (to HBM) and the L1
 //        terms; per-half-unit L1 and SSIM sums in a fixed order.
 // grad:  three waves per unit, one per channel, two columns per lane: the
 //        three partial planes of the channel are blurred and combined into
-//        dL/drgb; the three channels of 8 output rows are assembled in LDS and
+//        dL/drgb; the three channels of 12 output rows are assembled in LDS and
 //        written as whole interleaved rows.
 // The blur taps run in tap order -5..5 and the per-pixel SSIM expressions
 // are the engine's, with its divisions by c d, c^2 d and c d^2 taken as
 // products of two hardware reciprocals (1 ulp each); the loss and dL/drgb
 // are held to the float64 evaluation of the engine's loss.
 // Compulsory HBM traffic per pixel: 24 B read and 36 B written by the first
-// pass, 60 B read and 12 B written by the second, plus 10 halo rows per 78.
+// pass, 60 B read and 12 B written by the second, plus 10 halo rows per 86.
 #include "splat_common.h"
 #include "splat_kernels.h"
 
@@ -43,8 +43,12 @@ namespace {
 
 constexpr int kR = 5;
 constexpr int kTaps = 2 * kR + 1;         // 11
+// Ring slots and rows per unrolled group: 12, a multiple of both prefetch
+// depths below, so the queue's registers line up at the group loop's back
+// edge and no load in flight has to be waited for there.
+constexpr int kRing = 12;
 constexpr int kUnitW = 128;               // columns per unit
-constexpr int kUnitH = 78;                // output rows per unit: 88 input rows, eight groups of 11
+constexpr int kUnitH = 86;                // output rows per unit: 96 input rows, eight groups of 12
 constexpr int kStatWaves = 3;             // stats: one wave per channel over half a unit (64 columns)
 constexpr int kGradWaves = 3;             // grad: 1 wave per channel, 2 columns per lane
 constexpr int kGradPitch = 192;           // 138 staged columns per grad wave (three per lane written)
@@ -72,11 +76,15 @@ struct Unit {
   const ViewDesc* V;
 };
 
-// The block's unit, or with `half` its half unit (64 columns).
+// The block's unit, or with `half` its half unit (64 columns). The gradient
+// pass takes the units in the reverse of the statistics pass's order: the
+// partial planes written last are then read first, while the memory-side
+// cache may still hold them.
 __device__ __forceinline__ Unit unit_of(const LossParams& p, const ViewDesc* views, bool half) {
   Unit u;
   const int n = half ? p.total_tiles : p.total_tiles / 2;
   u.block = xcd_remap(blockIdx.x, n);
+  if (!half) u.block = n - 1 - u.block;
   int v = 0;
   for (int k = 1; k < p.n_views; ++k)
     if ((half ? views[k].lt_base : views[k].lt_base / 2) <= u.block) v = k;
@@ -200,7 +208,7 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
     const __amdgpu_buffer_rsrc_t rt0 = buffer_of(terms + (long long)(3 * c) * total_px + V.pixel_base, plane_bytes);
     const __amdgpu_buffer_rsrc_t rt1 = buffer_of(terms + (long long)(3 * c + 1) * total_px + V.pixel_base, plane_bytes);
     const __amdgpu_buffer_rsrc_t rt2 = buffer_of(terms + (long long)(3 * c + 2) * total_px + V.pixel_base, plane_bytes);
-    float ring[5][kTaps];  // row-blurred x, y, xx, yy, xy of the last 11 rows
+    float ring[5][kRing];  // row-blurred x, y, xx, yy, xy of the last 12 rows
 
     // one input row: stage it, take the lane's window, add the lane's own
     // pixel to L1 (input rows 5 .. rows + 4 are the unit's own rows), and
@@ -252,14 +260,14 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
     stage(0, fetch(0));
 #pragma unroll
     for (int k = 0; k < kStatAhead; ++k) q[k] = fetch(k + 1);
-    // groups of 11 input rows, row r into ring slot r % 11; input rows 10 ..
+    // groups of 12 input rows, row r into ring slot r % 12; input rows 10 ..
     // rows + 9 complete output rows 0 .. rows - 1, rows past the unit (the
     // last band of a view) are computed from zeros and masked
-    const int groups = (rows + 2 * kR + kTaps - 1) / kTaps;
+    const int groups = (rows + 2 * kR + kRing - 1) / kRing;
     for (int g = 0; g < groups; ++g) {
 #pragma unroll
-      for (int j = 0; j < kTaps; ++j) {
-        const int r = g * kTaps + j;
+      for (int j = 0; j < kRing; ++j) {
+        const int r = g * kRing + j;
         row_in(r, j);
         const int o = r - 2 * kR;  // the output row this input row completes
         if (o < 0) continue;       // the first group's first ten rows only fill the ring
@@ -270,7 +278,7 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
         for (int pl = 0; pl < 5; ++pl) {
           float s = 0.f;
 #pragma unroll
-          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][(j + 1 + i) % kTaps], s);
+          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][(j + kRing - 2 * kR + i) % kRing], s);
           st[pl] = s;
         }
         const float mu_x = st[0], mu_y = st[1], exx = st[2], eyy = st[3], exy = st[4];
@@ -297,8 +305,9 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
       }
       // the three waves read the same interleaved RGB lines (each takes one
       // channel): kept within a group of rows of each other, the lines one
-      // wave brings into the cache are still there for the others
-      __syncthreads();
+      // wave brings into the cache are still there for the others. A bare
+      // barrier: nothing is exchanged through memory here.
+      __builtin_amdgcn_s_barrier();
     }
   }
   // fixed-order sums: wave butterflies, then the waves in index order
@@ -331,7 +340,7 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
                                                                const float* __restrict__ terms, long long total_px,
                                                                float* __restrict__ dL) {
   __shared__ __align__(16) float s_t[kGradWaves][3][kGradPitch];  // 138 staged columns; every lane writes three
-  __shared__ __align__(16) float s_dl[kTaps][kUnitW * 3];          // one group of output rows, interleaved
+  __shared__ __align__(16) float s_dl[kRing][kUnitW * 3];          // one group of output rows, interleaved
   const Unit U = unit_of(p, views, false);
   const ViewDesc& V = *U.V;
   const int lane = threadIdx.x & 63;
@@ -395,7 +404,7 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
     }
     return f;
   };
-  float ring[3][2][kTaps];
+  float ring[3][2][kRing];
   float ox[2] = {0.f, 0.f}, oy[2] = {0.f, 0.f};  // x and y of the lane's pixels in the current output row
 
   // one input row: stage the planes, row-blur the lane's two columns into
@@ -439,13 +448,13 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
   };
 #pragma unroll
   for (int k = 0; k < kGradAhead; ++k) q[k] = fetch(k);
-  // groups of 11 input rows, row r into ring slot r % 11; input row r
-  // completes output row r - 10, staged in LDS slot (r - 10) % 11
-  const int groups = (rows + 2 * kR + kTaps - 1) / kTaps;
+  // groups of 12 input rows, row r into ring slot r % 12; input row r
+  // completes output row r - 10, staged in LDS slot (r - 10) % 12
+  const int groups = (rows + 2 * kR + kRing - 1) / kRing;
   for (int g = 0; g < groups; ++g) {
 #pragma unroll
-    for (int j = 0; j < kTaps; ++j) {
-      const int r = g * kTaps + j;
+    for (int j = 0; j < kRing; ++j) {
+      const int r = g * kRing + j;
       row_in(r, j);
       const int o = r - 2 * kR;
       if (o < 0) continue;  // the first group's first ten rows only fill the ring
@@ -456,10 +465,10 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
         for (int k = 0; k < 2; ++k) {
           float s = 0.f;
 #pragma unroll
-          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][k][(j + 1 + i) % kTaps], s);
+          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][k][(j + kRing - 2 * kR + i) % kRing], s);
           bl[pl][k] = s;
         }
-      float* stage = s_dl[o % kTaps];
+      float* stage = s_dl[o % kRing];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         float d = bl[0][k];
@@ -475,10 +484,10 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
     // all three channels of the output rows this group completed are staged:
     // write those inside the unit as whole interleaved rows
     __syncthreads();
-    const int o_first = max(0, g * kTaps - 2 * kR), o_end = min(rows, g * kTaps + kTaps - 2 * kR);
+    const int o_first = max(0, g * kRing - 2 * kR), o_end = min(rows, g * kRing + kRing - 2 * kR);
     for (int o = o_first; o < o_end; ++o) {
       const int row_float = 3 * ((by + o) * W + sx);
-      const float* stage = s_dl[o % kTaps];
+      const float* stage = s_dl[o % kRing];
 #pragma unroll
       for (int e0 = 0; e0 < kUnitW * 3; e0 += kGradWaves * 64) {
         const int e = e0 + (int)threadIdx.x;

@@ -90,9 +90,10 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
                                                  int n_views, float dilation, float min_alpha,
                                                  SplatRec* __restrict__ rec, uint32_t* __restrict__ keys,
                                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ scounts,
-                                                 uint2* __restrict__ rects, uint32_t* __restrict__ prects) {
+                                                 uint2* __restrict__ rects, uint32_t* __restrict__ prects,
+                                                 uint32_t* __restrict__ key_ranges) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
+  if (g >= n) return;  // the lanes left are a prefix of the wave
   const float* P = params;
   const float p0 = P[3 * g], p1 = P[3 * g + 1], p2 = P[3 * g + 2];
   const float* lsp = P + 3 * cap + 3 * g;
@@ -138,6 +139,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
 #pragma unroll
     for (int k = 0; k < kCount; ++k) rc[c][k] = rest[c * per_channel + k];
 
+  uint32_t key_min = 0xFFFFFFFFu, key_max = 0u;  // over this gaussian's kept slots
   for (int v = 0; v < n_views; ++v) {
     const ViewDesc& V = views[v];
     const long long slot = (long long)v * n + g;
@@ -250,6 +252,10 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
     }
     rec[slot] = o;
     keys[slot] = key;
+    if (key != 0xFFFFFFFFu) {
+      key_min = min(key_min, key);
+      key_max = max(key_max, key);
+    }
     if (prects) {  // wave-uniform
       prects[slot] = count ? pack_rect(o.x0, o.x1, o.y0, o.y1) : kNoRect;
     } else {
@@ -259,6 +265,15 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
                                (uint32_t)(uint16_t)o.y0 | (uint32_t)(uint16_t)o.y1 << 16);
     }
   }
+  if (key_ranges) {  // uniform: the wave's extremes over its active lanes, into the wave's own pair
+    key_min = __ockl_wfred_min_u32(key_min);
+    key_max = __ockl_wfred_max_u32(key_max);
+    if ((threadIdx.x & 63) == 0) {
+      const int w = g >> 6;
+      key_ranges[2 * w] = key_min;
+      key_ranges[2 * w + 1] = key_max;
+    }
+  }
 }
 
 }  // namespace
@@ -266,7 +281,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
                           uint32_t* keys, uint32_t* counts, uint32_t* scounts, uint2* rects, uint32_t* prects,
-                          hipStream_t stream) {
+                          hipStream_t stream, uint32_t* key_ranges) {
   if (n <= 0 || n_views <= 0) return hipSuccess;
   const int blocks = (n + 255) / 256;
   // the cloud's coefficients must cover the degree in use
@@ -274,19 +289,19 @@ hipError_t launch_project(const float* params, long long cap, int n, int rest_st
   switch (sh_use) {
     case 0:
       hipLaunchKernelGGL(k_project<0>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects, key_ranges);
       break;
     case 1:
       hipLaunchKernelGGL(k_project<1>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects, key_ranges);
       break;
     case 2:
       hipLaunchKernelGGL(k_project<2>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects, key_ranges);
       break;
     default:
       hipLaunchKernelGGL(k_project<3>, dim3(blocks), dim3(256), 0, stream, params, cap, n, rest_stride, views,
-                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects);
+                         n_views, dilation, min_alpha, rec, keys, counts, scounts, rects, prects, key_ranges);
       break;
   }
   return hipGetLastError();

@@ -63,12 +63,15 @@ __device__ __forceinline__ uint32_t scan_value(uint32_t x, int xform) {
   return xform == kScanRectTiles ? rect_tiles(x) : (xform == kScanRectSupers ? rect_supers(x) : x);
 }
 
-// With side_total, each block also adds the sum of side_xform of its inputs
-// to *side_total (one atomic per block).
+// With side_total, each block also sums side_xform of its inputs: into
+// side_parts[block] when given (a later sum adds them to *side_total: one
+// word's atomics from thousands of blocks would queue behind each other),
+// else with one atomic add to *side_total.
 __global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* in, const uint32_t* idx, uint32_t* out,
                                                            long long n_cap, const uint32_t* n_dev, int xform,
                                                            uint32_t* block_sums, uint32_t* total,
-                                                           uint32_t* side_total, int side_xform) {
+                                                           uint32_t* side_total, int side_xform,
+                                                           uint32_t* side_parts) {
   __shared__ uint32_t lds[kScanThreads / 64];
   __shared__ uint32_t tile[kScanTile + kScanTile / 32];
   const long long n = item_count(n_cap, n_dev);
@@ -107,14 +110,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* in,
     const long long j = base + (long long)r * kScanThreads + tid;
     if (j < n) out[j] = tile[scan_pad(r * kScanThreads + tid)];
   }
-  if (side_total) {  // after every store: one atomic per block, nothing waits on it
+  if (side_total) {  // after every store
     if ((tid & 63) == 0) lds[tid >> 6] = side;
     __syncthreads();
     if (tid == 0) {
       uint32_t sum = 0;
 #pragma unroll
       for (int w = 0; w < kScanThreads / 64; ++w) sum += lds[w];
-      if (sum) atomicAdd(side_total, sum);
+      if (side_parts) side_parts[blockIdx.x] = sum;
+      else if (sum) atomicAdd(side_total, sum);
     }
   }
   if (tid == 0) {
@@ -142,15 +146,19 @@ hipError_t scan_level(const uint32_t* in, const uint32_t* idx, uint32_t* out, lo
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (nb <= 1) {
     hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, in, idx, out, n, n_dev, xform, nullptr,
-                       total, side_total, side_xform);
+                       total, side_total, side_xform, nullptr);
     return hipGetLastError();
   }
+  // the level's block sums, then (with side_total) the blocks' side sums
   uint32_t* sums = ws;
+  uint32_t* side_parts = side_total ? ws + nb : nullptr;
+  uint32_t* above = ws + (side_total ? 2 * nb : nb);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, in, idx, out, n, n_dev, xform,
-                     sums, nullptr, side_total, side_xform);
+                     sums, nullptr, side_total, side_xform, side_parts);
   SPLAT_HIP_CHECK(hipGetLastError());
-  SPLAT_HIP_CHECK(scan_level(sums, nullptr, sums, nb, nullptr, kScanPlain, total, nullptr, kScanPlain, ws + nb,
+  SPLAT_HIP_CHECK(scan_level(sums, nullptr, sums, nb, nullptr, kScanPlain, total, nullptr, kScanPlain, above,
                              stream));
+  if (side_total) SPLAT_HIP_CHECK(launch_sum_u32(side_parts, nb, side_total, stream));
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, out, n, n_dev, sums);
   return hipGetLastError();
 }
@@ -158,17 +166,58 @@ hipError_t scan_level(const uint32_t* in, const uint32_t* idx, uint32_t* out, lo
 // With drop, keys equal to kDropKey are not counted (the scatter drops them).
 constexpr uint32_t kDropKey = 0xFFFFFFFFu;
 
+// One radix pass's buffers: set k of keys, values and second values is
+// (k[i], v[i], w[i]). Without a plan the pass reads set 0 and writes set 1.
+// With a plan (*passes on the device, 2 to 4) a pass at or past *passes does
+// nothing, and the sets are chosen so the last pass run writes set 0: two or
+// four passes go 0 -> 1 -> 0 (-> 1 -> 0), three go 0 -> 1 -> 2 -> 0, and the
+// first pass takes every kept key less *key_base.
+struct SortRoute {
+  uint32_t* k[3];
+  uint32_t* v[3];
+  uint32_t* w[3];
+  const uint32_t* passes;
+  const uint32_t* key_base;
+  int pass;
+};
+
+// kPlan: the route is read from the plan (the depth sort); otherwise set 0
+// to set 1 with keys as they are, known at compile time.
+template <bool kPlan>
+__device__ __forceinline__ bool route_sets(const SortRoute& R, int& src, int& dst) {
+  if (!kPlan) {
+    src = 0;
+    dst = 1;
+    return true;
+  }
+  const int p = (int)*R.passes, i = R.pass;
+  if (i >= p) return false;
+  src = i == 0 ? 0 : (p == 3 ? (i == 1 ? 1 : 2) : (i & 1));
+  dst = i == p - 1 ? 0 : (p == 3 ? (i == 0 ? 1 : 2) : ((i & 1) ? 0 : 1));
+  return true;
+}
+
+// Set k's buffer of one array, chosen by selects (no indexed access).
+__device__ __forceinline__ uint32_t* route_pick(uint32_t* const (&a)[3], int k) {
+  return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]);
+}
+
 // Per-block digit counts, into per-wave LDS copies. With kAggregate, each
 // round of 64 items finds the lanes sharing a digit (one ballot per digit
 // bit) and only the group's lowest lane adds the group's size: a skewed
 // digit (a narrow last digit, the few exponents in the top byte of positive
 // float keys) then costs one LDS atomic per round, not up to 64; for spread
 // digits the plain atomics are cheaper.
-template <bool kAggregate>
-__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys, long long n_cap,
+template <bool kAggregate, bool kPlan>
+__global__ __launch_bounds__(256) void k_radix_hist(SortRoute R, long long n_cap,
                                                     const uint32_t* __restrict__ n_dev, int shift, int bits,
                                                     int drop, uint32_t* __restrict__ hist, int nblocks) {
   __shared__ uint32_t h[4][256];
+  int src, dst;
+  if (!route_sets<kPlan>(R, src, dst)) return;  // a pass the plan skips (uniform)
+  const uint32_t* __restrict__ keys = route_pick(R.k, src);
+  // the first pass of a plan takes the keys less the base
+  const uint32_t key_base = (kPlan && R.pass == 0) ? *R.key_base : 0u;
   const long long n = item_count(n_cap, n_dev);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t mask = (1u << bits) - 1u;
@@ -181,6 +230,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
   for (int r = 0; r < kScanItems; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
     k[r] = j < n ? keys[j] : kDropKey;
+    if (kPlan && k[r] != kDropKey) k[r] -= key_base;
   }
   const uint64_t below = __lanemask_lt();
 #pragma unroll
@@ -218,16 +268,20 @@ constexpr int kRsRounds = kRsWaveItems / 64;  // 16
 // With kV2, a second value array rides along with the first (staged through
 // LDS after the keys and values, so its stores are runs too). Without vals
 // (null), item j's value is j.
-template <bool kV2>
-__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ keys,
-                                                       const uint32_t* __restrict__ vals,
-                                                       const uint32_t* __restrict__ vals2,
-                                                       uint32_t* __restrict__ keys_out,
-                                                       uint32_t* __restrict__ vals_out,
-                                                       uint32_t* __restrict__ vals2_out, long long n_cap,
+template <bool kV2, bool kPlan>
+__global__ __launch_bounds__(256) void k_radix_scatter(SortRoute R, int identity, long long n_cap,
                                                        const uint32_t* __restrict__ n_dev, int shift, int bits,
                                                        int drop, const uint32_t* __restrict__ hist,
                                                        const uint32_t* __restrict__ hist_prefix, int nblocks) {
+  int src, dst;
+  if (!route_sets<kPlan>(R, src, dst)) return;  // a pass the plan skips (uniform)
+  const uint32_t* __restrict__ keys = route_pick(R.k, src);
+  const uint32_t* __restrict__ vals = identity ? nullptr : route_pick(R.v, src);
+  const uint32_t* __restrict__ vals2 = route_pick(R.w, src);
+  uint32_t* __restrict__ keys_out = route_pick(R.k, dst);
+  uint32_t* __restrict__ vals_out = route_pick(R.v, dst);
+  uint32_t* __restrict__ vals2_out = route_pick(R.w, dst);
+  const uint32_t key_base = (kPlan && R.pass == 0) ? *R.key_base : 0u;
   const long long n = item_count(n_cap, n_dev);
   if ((long long)blockIdx.x * kScanTile >= n) return;  // whole block past the items (uniform)
   __shared__ uint32_t s_cnt[4][256];   // per-wave digit counts, then per-wave digit offsets
@@ -257,6 +311,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   for (int r = 0; r < kRsRounds; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
     k[r] = j < n ? keys[j] : kDropKey;
+    if (kPlan && k[r] != kDropKey) k[r] -= key_base;
     const bool valid = j < n && !(drop && k[r] == kDropKey);
     v[r] = valid ? (vals ? vals[j] : (uint32_t)j) : 0u;
     if constexpr (kV2) v2[r] = valid ? vals2[j] : 0u;
@@ -389,7 +444,8 @@ long long scan_workspace_elems(long long max_n) {
     n = (n + kScanTile - 1) / kScanTile;
     total += n;
   }
-  return total + 16;
+  // + the first level's side sums
+  return total + (max_n + kScanTile - 1) / kScanTile + 16;
 }
 
 long long sort_hist_elems(long long max_n) { return 256LL * ((max_n + kScanTile - 1) / kScanTile + 1); }
@@ -406,18 +462,20 @@ hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx,
 // level themselves: afterwards the scan of element i is data[i] +
 // (*prefix ? (*prefix)[i / kScanTile] : 0). Two levels when n fits them (the
 // last level's add is skipped); otherwise the full scan, and *prefix = null.
+// With n_dev (device), the scan covers min(n, *n_dev) items (a pass a sort
+// plan skips gives 0: nothing is read or written).
 static hipError_t scan_no_add(uint32_t* data, long long n, uint32_t* total, const ScanWorkspace& ws,
-                              const uint32_t** prefix, hipStream_t stream) {
+                              const uint32_t** prefix, hipStream_t stream, const uint32_t* n_dev = nullptr) {
   *prefix = nullptr;
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (nb <= 1 || nb > kScanTile || n > ws.max_n)
-    return launch_exclusive_scan(data, nullptr, data, n, total, ws, stream);
+    return launch_exclusive_scan(data, nullptr, data, n, total, ws, stream, kScanPlain, n_dev);
   uint32_t* sums = ws.buf;
-  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, data, nullptr, data, n, nullptr,
-                     (int)kScanPlain, sums, nullptr, nullptr, (int)kScanPlain);
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, data, nullptr, data, n, n_dev,
+                     (int)kScanPlain, sums, nullptr, nullptr, (int)kScanPlain, nullptr);
   SPLAT_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, sums, nullptr, sums, nb, nullptr,
-                     (int)kScanPlain, nullptr, total, nullptr, (int)kScanPlain);
+  hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, sums, nullptr, sums, nb, n_dev,
+                     (int)kScanPlain, nullptr, total, nullptr, (int)kScanPlain, nullptr);
   *prefix = sums;
   return hipGetLastError();
 }
@@ -425,41 +483,95 @@ static hipError_t scan_no_add(uint32_t* data, long long n, uint32_t* total, cons
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
                              hipStream_t stream, const uint32_t* n_dev, uint32_t* vals2, uint32_t* vals2_alt,
-                             bool identity_vals, uint32_t* n_kept) {
+                             bool identity_vals, uint32_t* n_kept, const SortPlan* plan) {
   *result_in_alt = false;
   if (end_bit <= begin_bit || n <= 0) return hipSuccess;
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (256 * nb > ws.hist_capacity) return hipErrorInvalidValue;
+  if (plan && (begin_bit != 0 || end_bit != 32 || (plan->vals2_c == nullptr) != (vals2 == nullptr)))
+    return hipErrorInvalidValue;  // a plan covers a full 32-bit key
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt, *wi = vals2, *wo = vals2_alt;
-  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+  int pass = 0;
+  for (int shift = begin_bit; shift < end_bit; shift += 8, ++pass) {
     const int bits = (end_bit - shift) < 8 ? (end_bit - shift) : 8;
     const bool first = shift == begin_bit;
     // with n_kept, the first pass drops kDropKey and the later ones sort what it kept
     const int drop = n_kept && first ? 1 : 0;
     const uint32_t* count = n_kept && !first ? n_kept : n_dev;
-    if (bits <= 4 || shift + bits == 32)
-      hipLaunchKernelGGL(k_radix_hist<true>, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, count, shift, bits, drop,
-                         ws.hist, (int)nb);
-    else
-      hipLaunchKernelGGL(k_radix_hist<false>, dim3((unsigned)nb), dim3(256), 0, stream, ki, n, count, shift, bits,
-                         drop, ws.hist, (int)nb);
+    SortRoute R{};
+    if (plan) {
+      R = SortRoute{{keys, keys_alt, plan->keys_c}, {vals, vals_alt, plan->vals_c}, {vals2, vals2_alt, plan->vals2_c},
+                    plan->passes, plan->key_base, pass};
+    } else {
+      R = SortRoute{{ki, ko, nullptr}, {vi, vo, nullptr}, {wi, wo, nullptr}, nullptr, nullptr, pass};
+    }
+    const bool aggregate = bits <= 4 || shift + bits == 32;
+    auto hist_kernel = plan ? (aggregate ? k_radix_hist<true, true> : k_radix_hist<false, true>)
+                            : (aggregate ? k_radix_hist<true, false> : k_radix_hist<false, false>);
+    hipLaunchKernelGGL(hist_kernel, dim3((unsigned)nb), dim3(256), 0, stream, R, n, count, shift, bits, drop, ws.hist,
+                       (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     const uint32_t* prefix = nullptr;
-    SPLAT_HIP_CHECK(scan_no_add(ws.hist, 256 * nb, drop ? n_kept : nullptr, ws.scan, &prefix, stream));
-    const uint32_t* vin = (identity_vals && first) ? nullptr : vi;
-    if (vals2)
-      hipLaunchKernelGGL(k_radix_scatter<true>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vin, wi, ko, vo, wo, n,
-                         count, shift, bits, drop, ws.hist, prefix, (int)nb);
-    else
-      hipLaunchKernelGGL(k_radix_scatter<false>, dim3((unsigned)nb), dim3(256), 0, stream, ki, vin, nullptr, ko, vo,
-                         nullptr, n, count, shift, bits, drop, ws.hist, prefix, (int)nb);
+    SPLAT_HIP_CHECK(scan_no_add(ws.hist, 256 * nb, drop ? n_kept : nullptr, ws.scan, &prefix, stream,
+                                plan ? plan->active + pass : nullptr));
+    const int identity = (identity_vals && first) ? 1 : 0;
+    auto scatter_kernel = plan ? (vals2 ? k_radix_scatter<true, true> : k_radix_scatter<false, true>)
+                               : (vals2 ? k_radix_scatter<true, false> : k_radix_scatter<false, false>);
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)nb), dim3(256), 0, stream, R, identity, n, count, shift, bits,
+                       drop, ws.hist, prefix, (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
-    uint32_t* t = ki; ki = ko; ko = t;
-    t = vi; vi = vo; vo = t;
-    t = wi; wi = wo; wo = t;
-    *result_in_alt = !*result_in_alt;
+    if (!plan) {
+      uint32_t* t = ki; ki = ko; ko = t;
+      t = vi; vi = vo; vo = t;
+      t = wi; wi = wo; wo = t;
+      *result_in_alt = !*result_in_alt;
+    }
   }
   return hipSuccess;
+}
+
+namespace {
+
+// The depth sort's plan from the projection's per-wave ranges of kept keys:
+// every kept key less the minimum fits the low 8 * passes bits, so the
+// passes above them would only copy.
+__global__ __launch_bounds__(1024) void k_sort_plan(const uint32_t* __restrict__ key_ranges, int n_waves,
+                                                    uint32_t* __restrict__ plan) {
+  __shared__ uint32_t s_lo[16], s_hi[16];
+  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  for (int w = threadIdx.x; w < n_waves; w += 1024) {
+    lo = min(lo, key_ranges[2 * w]);
+    hi = max(hi, key_ranges[2 * w + 1]);
+  }
+  lo = __ockl_wfred_min_u32(lo);
+  hi = __ockl_wfred_max_u32(hi);
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 16; ++k) {
+    lo = min(lo, s_lo[k]);
+    hi = max(hi, s_hi[k]);
+  }
+  uint32_t passes = 2, base = 0;
+  if (hi >= lo) {  // some key was kept
+    base = lo;
+    const uint32_t span = hi - lo;
+    const int bits = span ? 32 - __clz((int)span) : 0;
+    passes = (uint32_t)max(2, (bits + 7) / 8);
+  }
+  plan[0] = passes;
+  for (int i = 0; i < 4; ++i) plan[1 + i] = (uint32_t)i < passes ? 0xFFFFFFFFu : 0u;
+  plan[5] = base;
+}
+
+}  // namespace
+
+hipError_t launch_sort_plan(const uint32_t* key_ranges, int n_waves, uint32_t* plan, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(1024), 0, stream, key_ranges, n_waves, plan);
+  return hipGetLastError();
 }
 
 hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream) {

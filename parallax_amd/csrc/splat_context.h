@@ -73,6 +73,8 @@ struct splat_context {
 
   DeviceBuffer rec, keys, vals, keys_alt, vals_alt, counts, offsets, pair_base;
   DeviceBuffer scounts, scounts_alt, soffsets;      // super-tile pairs per slot (sorted along), their scan
+  DeviceBuffer keys_c, vals_c, scounts_c;           // the depth sort's third buffers (three passes)
+  DeviceBuffer key_ranges;                          // per projection wave: min, max of its kept keys
   DeviceBuffer rects;                                // each slot's tile rect, 8 bytes
   DeviceBuffer skeys, svals, skeys_alt, svals_alt;  // (super-tile, slot) pairs
   DeviceBuffer sranges, nseg, seg_base, seg_st, bin_table;

@@ -15,10 +15,13 @@ namespace splat {
 // scounts, rects are not written. Otherwise counts[slot] = tiles of the
 // slot's rect, scounts[slot] = super-tiles of it, rects[slot] = the rect
 // ((x0 | x1 << 16, y0 | y1 << 16), 8 bytes) for the binning's gathers.
+// With key_ranges (device, 2 * ceil(n / 64) words), wave w of gaussians
+// [64 w, 64 w + 64) writes the minimum and maximum of its kept keys to
+// key_ranges[2 w], key_ranges[2 w + 1] (0xFFFFFFFF, 0 when it kept none).
 hipError_t launch_project(const float* params, long long cap, int n, int rest_stride, int sh_use,
                           const ViewDesc* views, int n_views, float dilation, float min_alpha, SplatRec* rec,
                           uint32_t* keys, uint32_t* counts, uint32_t* scounts, uint2* rects, uint32_t* prects,
-                          hipStream_t stream);
+                          hipStream_t stream, uint32_t* key_ranges = nullptr);
 
 // ---- scan / sort ----
 constexpr int kScanThreads = 256;
@@ -63,10 +66,30 @@ long long sort_hist_elems(long long max_n);
 // values start as the item indices (vals is not read, only written). With
 // n_kept (device), items whose key is 0xFFFFFFFF are dropped, *n_kept is set
 // to the count of the rest, and the sorted output holds those only.
+//
+// With a plan (a full 32-bit key only), the number of 8-bit passes is read on
+// the device: the first pass takes each kept key less *key_base, and the
+// passes at or past *passes do nothing (each of their kernels returns at
+// once). The output then always ends in (keys, vals, vals2), through the
+// third buffers when three passes run; *result_in_alt is false.
+struct SortPlan {
+  const uint32_t* passes;    // device: the passes to run, 2 to 4
+  const uint32_t* active;    // device, per pass: 0 when it is skipped (caps its scans' item count)
+  const uint32_t* key_base;  // device: subtracted from every kept key
+  uint32_t* keys_c;          // third buffers, each of n elements
+  uint32_t* vals_c;
+  uint32_t* vals2_c;         // with vals2 only
+};
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
                              hipStream_t stream, const uint32_t* n_dev = nullptr, uint32_t* vals2 = nullptr,
-                             uint32_t* vals2_alt = nullptr, bool identity_vals = false, uint32_t* n_kept = nullptr);
+                             uint32_t* vals2_alt = nullptr, bool identity_vals = false, uint32_t* n_kept = nullptr,
+                             const SortPlan* plan = nullptr);
+// The plan of a depth sort (plan[0] passes, plan[1..4] active, plan[5] key
+// base, as SortPlan reads them) from the projection's per-wave key ranges
+// (n_waves (min, max) pairs).
+constexpr int kSortPlanWords = 6;
+hipError_t launch_sort_plan(const uint32_t* key_ranges, int n_waves, uint32_t* plan, hipStream_t stream);
 // *out (device) += the sum of in[0..n).
 hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream);
 

@@ -1,5 +1,6 @@
 """The photometric loss kernels on image shapes that do not fill their work
-units (128 x 64 pixels) and in uneven batches.
+units (128 x 86 pixels, statistics in 64-column halves) and in uneven
+batches.
 
 * Through splat_loss_noalloc, on seeded images of ragged sizes (a single
   pixel, thin strips, one pixel either side of a unit edge, 1080p): the loss
@@ -26,7 +27,8 @@ pytestmark = pytest.mark.gpu
 
 DL_MAX = 5e-5
 DL_REL_L2 = 2e-5
-SHAPES = [(1, 1), (5, 3), (127, 63), (128, 64), (129, 65), (200, 150), (7, 300), (300, 7), (1920, 1080)]
+SHAPES = [(1, 1), (5, 3), (127, 63), (128, 64), (129, 65), (96, 85), (96, 86), (96, 87), (131, 173), (200, 150),
+          (7, 300), (300, 7), (1920, 1080)]
 
 
 def images(w, h, seed):
