@@ -58,8 +58,17 @@ __global__ __launch_bounds__(256) void k_emit_super(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ sorted_prects,
                                                     const ViewDesc* __restrict__ views, long long per_view,
                                                     long long m_cap, const uint32_t* __restrict__ m_dev,
-                                                    uint32_t s_cap, uint32_t* __restrict__ skeys,
-                                                    uint32_t* __restrict__ svals, uint32_t* __restrict__ pair_base) {
+                                                    uint32_t s_cap, uint32_t* __restrict__ skeys_even,
+                                                    uint32_t* __restrict__ svals_even,
+                                                    uint32_t* __restrict__ skeys_odd,
+                                                    uint32_t* __restrict__ svals_odd,
+                                                    const uint32_t* __restrict__ sort_passes,
+                                                    uint32_t* __restrict__ pair_base) {
+  // the sort after this one ends in the even set whatever its pass count:
+  // its input goes to the odd set when that count is odd
+  const bool odd = sort_passes && (*sort_passes & 1u);
+  uint32_t* __restrict__ skeys = odd ? skeys_odd : skeys_even;
+  uint32_t* __restrict__ svals = odd ? svals_odd : svals_even;
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const long long m = m_dev ? min(m_cap, (long long)*m_dev) : m_cap;
@@ -388,12 +397,13 @@ hipError_t launch_check_caps(uint32_t* totals, uint32_t p_cap, uint32_t s_cap, h
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
                              const uint32_t* s_total, const uint2* rects, const uint32_t* sorted_prects,
                              const ViewDesc* views, long long per_view, long long m, const uint32_t* m_dev,
-                             uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
+                             uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* skeys_odd,
+                             uint32_t* svals_odd, const uint32_t* sort_passes, uint32_t* pair_base,
                              hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_emit_super, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,
                      soffsets, s_total, rects, sorted_prects, views, per_view, m, m_dev, s_cap, skeys, svals,
-                     pair_base);
+                     skeys_odd, svals_odd, sort_passes, pair_base);
   return hipGetLastError();
 }
 

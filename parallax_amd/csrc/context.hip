@@ -184,7 +184,7 @@ namespace {
 constexpr int kSortedScalar = 5;
 constexpr int kEpochScalar = 8;
 constexpr int kSortPlanScalar = 9;
-constexpr int kScalarWords = kSortPlanScalar + kSortPlanWords;
+constexpr int kScalarWords = kSortPlanScalar + kPlanWords;
 
 splat_status check_ready(splat_context* ctx) {
   if (!ctx) return SPLAT_ERROR_INVALID_ARGUMENT;
@@ -205,19 +205,28 @@ splat_status batch_setup(splat_context* ctx, int n_views, const int32_t* view_id
   ctx->batch_desc.clear();
   int tiles = 0, supers = 0, ltiles = 0;
   long long px = 0;
+  // super-tiles numbered view * 256 + super-tile within the view when every
+  // view has at most 256 of them: the super-tile sort's digit is then the
+  // id within the view (the sort plan's view grouping does the rest)
+  bool st_by_view = true;
+  for (int k = 0; k < n_views; ++k) {
+    const ViewDesc d = make_desc(ctx->views[view_ids[k]]);
+    if (d.scols * d.srows > 256) st_by_view = false;
+  }
   for (int k = 0; k < n_views; ++k) {
     ViewDesc d = make_desc(ctx->views[view_ids[k]]);
     d.tile_base = tiles;
-    d.st_base = supers;
+    d.st_base = st_by_view ? 256 * k : supers;
     d.pixel_base = px;
     d.target_base = ctx->has_target[view_ids[k]] ? ctx->target_offset[view_ids[k]] : 0;
     d.lt_base = ltiles;
     ltiles += loss_tiles(d.width, d.height);
     tiles += d.cols * d.rows;
-    supers += d.scols * d.srows;
+    supers = st_by_view ? 256 * (k + 1) : supers + d.scols * d.srows;
     px += (long long)d.width * d.height;
     ctx->batch_desc.push_back(d);
   }
+  ctx->st_view_bits = st_by_view ? bit_width((unsigned long long)(n_views - 1)) : -1;
   if (supers > (int)kStMask) return fail(ctx, SPLAT_ERROR_INVALID_ARGUMENT, "render_forward: batch too large");
   ctx->packed = true;
   for (const ViewDesc& d : ctx->batch_desc)
@@ -354,10 +363,12 @@ splat_status enqueue_front(splat_context* ctx) {
   // kept keys' range needs (the sort's plan, set on the device).
   uint32_t* n_sorted = d_total + kSortedScalar;
   uint32_t* plan_words = d_total + kSortPlanScalar;
-  const SortPlan plan{plan_words, plan_words + 1, plan_words + 5, ctx->keys_c.u(), ctx->vals_c.u(),
+  const SortPlan plan{plan_words + kPlanPasses, plan_words + kPlanActive, plan_words + kPlanKeyBase,
+                      plan_words + kPlanViewShift, (uint32_t)n, kDepthDigitBits, ctx->keys_c.u(), ctx->vals_c.u(),
                       ctx->scounts_c.u()};
   TIMED("depth_sort", [&]() -> hipError_t {
-    const hipError_t e = launch_sort_plan(ctx->key_ranges.u(), (int)((n + 63) / 64), plan_words, ctx->stream);
+    const hipError_t e = launch_sort_plan(ctx->key_ranges.u(), (int)((n + 63) / 64), ctx->st_view_bits, plan_words,
+                                          ctx->stream);
     if (e != hipSuccess) return e;
     return launch_radix_sort(ctx->keys.u(), ctx->vals.u(), ctx->keys_alt.u(), ctx->vals_alt.u(), M, 0, 32,
                              ctx->sort_ws, &alt, ctx->stream, nullptr, ctx->scounts.u(), ctx->scounts_alt.u(), true,
@@ -401,14 +412,29 @@ splat_status enqueue_back(splat_context* ctx) {
   uint32_t* d_total = ctx->scalars.u();
   CK(launch_check_caps(d_total, p_cap, s_cap, ctx->stream), "capacity check");
   uint32_t* pair_base = ctx->opts.deterministic ? ctx->pair_base.u() : nullptr;
+  // with super-tiles numbered by view, the super-tile sort runs under the
+  // plan the depth sort's plan kernel wrote: one pass over the id within a
+  // view when the depth sort grouped the slots by view, else a second by view
+  const bool st_by_view = ctx->st_view_bits >= 0;
+  const uint32_t* plan_words = d_total + kSortPlanScalar;
   TIMED("emit_super", launch_emit_super(ctx->sorted_slots, ctx->offsets.u(), ctx->soffsets.u(), d_total + 1,
                                         reinterpret_cast<const uint2*>(ctx->rects.u()), ctx->sorted_prects,
                                         ctx->views_dev(), n, M, d_total + kSortedScalar, s_cap, ctx->skeys.u(),
-                                        ctx->svals.u(), pair_base, ctx->stream));
-  const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
+                                        ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(),
+                                        st_by_view ? plan_words + kPlanSuperPasses : nullptr, pair_base,
+                                        ctx->stream));
   bool salt = false;
-  TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(), s_cap,
-                                        0, st_bits, ctx->sort_ws, &salt, ctx->stream, d_total + 1));
+  if (st_by_view) {
+    const SortPlan splan{plan_words + kPlanSuperPasses, plan_words + kPlanSuperActive, nullptr, nullptr, 1u, 8,
+                         nullptr, nullptr, nullptr};
+    TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(),
+                                          s_cap, 0, 8 + ctx->st_view_bits, ctx->sort_ws, &salt, ctx->stream,
+                                          d_total + 1, nullptr, nullptr, false, nullptr, &splan));
+  } else {
+    const int st_bits = std::max(1, bit_width((unsigned long long)std::max(0, supers - 1)));
+    TIMED("super_sort", launch_radix_sort(ctx->skeys.u(), ctx->svals.u(), ctx->skeys_alt.u(), ctx->svals_alt.u(),
+                                          s_cap, 0, st_bits, ctx->sort_ws, &salt, ctx->stream, d_total + 1));
+  }
   const uint32_t* skeys = salt ? ctx->skeys_alt.u() : ctx->skeys.u();
   const uint32_t* svals = salt ? ctx->svals_alt.u() : ctx->svals.u();
   TIMED("segments", launch_super_ranges(skeys, s_cap, d_total + 1, ctx->sranges.u(), supers, ctx->stream));

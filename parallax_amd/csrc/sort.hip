@@ -168,21 +168,27 @@ constexpr uint32_t kDropKey = 0xFFFFFFFFu;
 
 // One radix pass's buffers: set k of keys, values and second values is
 // (k[i], v[i], w[i]). Without a plan the pass reads set 0 and writes set 1.
-// With a plan (*passes on the device, 2 to 4) a pass at or past *passes does
-// nothing, and the sets are chosen so the last pass run writes set 0: two or
-// four passes go 0 -> 1 -> 0 (-> 1 -> 0), three go 0 -> 1 -> 2 -> 0, and the
-// first pass takes every kept key less *key_base.
+// With a plan (*passes on the device) a pass at or past *passes does nothing,
+// and the last pass run always writes set 0: with a third set (k[2]) the
+// input is in set 0 and two or four passes go 0 -> 1 -> 0 (-> 1 -> 0), three
+// go 0 -> 1 -> 2 -> 0; without one, the input is in set 1 for an odd number
+// of passes and in set 0 for an even one, and the passes alternate. A plan's
+// first pass may also rewrite every kept key: less *key_base, and, when
+// *view_shift < 32, with its item's view (index / view_size) at that bit.
 struct SortRoute {
   uint32_t* k[3];
   uint32_t* v[3];
   uint32_t* w[3];
   const uint32_t* passes;
   const uint32_t* key_base;
+  const uint32_t* view_shift;
+  uint32_t view_size;
+  float view_inv;  // 1 / view_size, rounded
   int pass;
 };
 
-// kPlan: the route is read from the plan (the depth sort); otherwise set 0
-// to set 1 with keys as they are, known at compile time.
+// kPlan: the route is read from the plan; otherwise set 0 to set 1 with keys
+// as they are, known at compile time.
 template <bool kPlan>
 __device__ __forceinline__ bool route_sets(const SortRoute& R, int& src, int& dst) {
   if (!kPlan) {
@@ -192,8 +198,13 @@ __device__ __forceinline__ bool route_sets(const SortRoute& R, int& src, int& ds
   }
   const int p = (int)*R.passes, i = R.pass;
   if (i >= p) return false;
-  src = i == 0 ? 0 : (p == 3 ? (i == 1 ? 1 : 2) : (i & 1));
-  dst = i == p - 1 ? 0 : (p == 3 ? (i == 0 ? 1 : 2) : ((i & 1) ? 0 : 1));
+  if (R.k[2]) {
+    src = i == 0 ? 0 : (p == 3 ? (i == 1 ? 1 : 2) : (i & 1));
+    dst = i == p - 1 ? 0 : (p == 3 ? (i == 0 ? 1 : 2) : ((i & 1) ? 0 : 1));
+  } else {
+    src = (p - i) & 1;
+    dst = (p - i - 1) & 1;
+  }
   return true;
 }
 
@@ -202,35 +213,78 @@ __device__ __forceinline__ uint32_t* route_pick(uint32_t* const (&a)[3], int k) 
   return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]);
 }
 
-// Per-block digit counts, into per-wave LDS copies. With kAggregate, each
-// round of 64 items finds the lanes sharing a digit (one ballot per digit
-// bit) and only the group's lowest lane adds the group's size: a skewed
-// digit (a narrow last digit, the few exponents in the top byte of positive
-// float keys) then costs one LDS atomic per round, not up to 64; for spread
-// digits the plain atomics are cheaper.
-template <bool kAggregate, bool kPlan>
-__global__ __launch_bounds__(256) void k_radix_hist(SortRoute R, long long n_cap,
+// The first pass of a plan: a kept key's sort key (see SortRoute).
+struct KeyRewrite {
+  uint32_t base = 0u, shift = 32u, size = 1u;
+  float inv = 1.f;
+  bool on = false;
+  __device__ __forceinline__ uint32_t operator()(uint32_t k, long long j) const {
+    if (!on || k == kDropKey) return k;
+    k -= base;
+    if (shift < 32u) {
+      // j / size: the float quotient is within one of it (j < 2^32 and at
+      // most 64 views), and one step corrects it
+      uint32_t v = (uint32_t)((float)j * inv);
+      if ((unsigned long long)v * size > (unsigned long long)j) --v;
+      else if ((unsigned long long)(v + 1) * size <= (unsigned long long)j) ++v;
+      k |= v << shift;
+    }
+    return k;
+  }
+};
+template <bool kPlan>
+__device__ __forceinline__ KeyRewrite key_rewrite(const SortRoute& R) {
+  KeyRewrite w;
+  if (kPlan && R.pass == 0 && R.key_base) {
+    w.on = true;
+    w.base = *R.key_base;
+    w.shift = R.view_shift ? *R.view_shift : 32u;
+    w.size = R.view_size;
+    w.inv = R.view_inv;
+  }
+  return w;
+}
+
+// Per-block digit counts (1 << kDigitBits digits), into per-wave LDS copies.
+// With kAggregate, each round of 64 items finds the lanes sharing a digit
+// (one ballot per digit bit) and only the group's lowest lane adds the
+// group's size: a skewed digit (a narrow last digit, the few exponents in the
+// top byte of positive float keys) then costs one LDS atomic per round, not
+// up to 64; for spread digits the plain atomics are cheaper.
+// A pass's workgroup: four waves over 4096 items for 8-bit digits, eight
+// over 8192 for 9-bit ones (each wave 1024 items), so a digit's run in a
+// workgroup averages 16 items either way.
+template <int kDigitBits>
+struct RadixShape {
+  static constexpr int kWaves = kDigitBits == 9 ? 8 : 4;
+  static constexpr int kThreads = 64 * kWaves;
+  static constexpr int kItems = 1024 * kWaves;
+};
+
+template <bool kAggregate, bool kPlan, int kDigitBits>
+__global__ __launch_bounds__(RadixShape<kDigitBits>::kThreads) void k_radix_hist(SortRoute R, long long n_cap,
                                                     const uint32_t* __restrict__ n_dev, int shift, int bits,
                                                     int drop, uint32_t* __restrict__ hist, int nblocks) {
-  __shared__ uint32_t h[4][256];
+  constexpr int kRadix = 1 << kDigitBits;
+  constexpr int kWaves = RadixShape<kDigitBits>::kWaves, kThreads = RadixShape<kDigitBits>::kThreads;
+  __shared__ uint32_t h[kWaves][kRadix];
   int src, dst;
   if (!route_sets<kPlan>(R, src, dst)) return;  // a pass the plan skips (uniform)
   const uint32_t* __restrict__ keys = route_pick(R.k, src);
-  // the first pass of a plan takes the keys less the base
-  const uint32_t key_base = (kPlan && R.pass == 0) ? *R.key_base : 0u;
+  const KeyRewrite rewrite = key_rewrite<kPlan>(R);
   const long long n = item_count(n_cap, n_dev);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t mask = (1u << bits) - 1u;
-  for (int w = 0; w < 4; ++w) h[w][tid] = 0;
+  for (int d = tid; d < kRadix; d += kThreads)
+    for (int w = 0; w < kWaves; ++w) h[w][d] = 0;
   __syncthreads();
   // the wave's own 1024 consecutive items
-  const long long wbase = (long long)blockIdx.x * kScanTile + (long long)wave * (kScanTile / 4);
+  const long long wbase = (long long)blockIdx.x * RadixShape<kDigitBits>::kItems + (long long)wave * 1024;
   uint32_t k[kScanItems];
 #pragma unroll
   for (int r = 0; r < kScanItems; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
-    k[r] = j < n ? keys[j] : kDropKey;
-    if (kPlan && k[r] != kDropKey) k[r] -= key_base;
+    k[r] = j < n ? rewrite(keys[j], j) : kDropKey;
   }
   const uint64_t below = __lanemask_lt();
 #pragma unroll
@@ -251,28 +305,45 @@ __global__ __launch_bounds__(256) void k_radix_hist(SortRoute R, long long n_cap
     }
   }
   __syncthreads();
-  hist[(long long)tid * nblocks + blockIdx.x] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+  for (int d = tid; d < kRadix; d += kThreads) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) c += h[w][d];
+    hist[(long long)d * nblocks + blockIdx.x] = c;
+  }
 }
 
 // Stable scatter of one 4096-item block for one digit. Each wave ranks its
 // own contiguous 1024 items with no block barriers: per round of 64, the
 // lanes sharing a digit are found with one ballot per digit bit, and the
 // group's lowest lane adds the group's size to a wave-private LDS counter per
-// digit with a returning atomic, whose old value is the group's base. Then
-// per-digit wave offsets and block digit starts place every item at its
-// stable position in an LDS copy of the block, sorted by digit; the block is
-// written out from LDS so each digit's run goes to global memory contiguously.
+// digit with a returning atomic, whose old value is the group's base. The
+// counters are 16-bit halves of 32-bit words (a wave counts at most 1024
+// items), digits 2c and 2c + 1 in word c, so a 9-bit digit's block still
+// fits four workgroups per CU. Then per-digit wave offsets and block digit
+// starts place every item at its stable position in an LDS copy of the
+// block, sorted by digit; the block is written out from LDS so each digit's
+// run goes to global memory contiguously.
 constexpr int kRsWaveItems = kScanTile / 4;  // 1024 items per wave
 constexpr int kRsRounds = kRsWaveItems / 64;  // 16
 
+__device__ __forceinline__ uint32_t half_of(uint32_t word, uint32_t d) { return (word >> (16u * (d & 1u))) & 0xFFFFu; }
+
 // With kV2, a second value array rides along with the first (staged through
 // LDS after the keys and values, so its stores are runs too). Without vals
-// (null), item j's value is j.
-template <bool kV2, bool kPlan>
-__global__ __launch_bounds__(256) void k_radix_scatter(SortRoute R, int identity, long long n_cap,
+// (identity), item j's value is j.
+template <bool kV2, bool kPlan, int kDigitBits>
+__global__ __launch_bounds__(RadixShape<kDigitBits>::kThreads) void k_radix_scatter(SortRoute R, int identity,
+                                                       long long n_cap,
                                                        const uint32_t* __restrict__ n_dev, int shift, int bits,
                                                        int drop, const uint32_t* __restrict__ hist,
                                                        const uint32_t* __restrict__ hist_prefix, int nblocks) {
+  constexpr int kRadix = 1 << kDigitBits;
+  constexpr int kWaves = RadixShape<kDigitBits>::kWaves, kThreads = RadixShape<kDigitBits>::kThreads;
+  constexpr int kItems = RadixShape<kDigitBits>::kItems;
+  constexpr int kWords = kRadix / 2;           // counter words per wave
+  static_assert(kRadix == kThreads || kRadix == 2 * kThreads, "one or two digits per thread");
+  constexpr int kPer = kRadix / kThreads;      // digits per thread in the digit-wise steps (1 or 2)
   int src, dst;
   if (!route_sets<kPlan>(R, src, dst)) return;  // a pass the plan skips (uniform)
   const uint32_t* __restrict__ keys = route_pick(R.k, src);
@@ -281,26 +352,29 @@ __global__ __launch_bounds__(256) void k_radix_scatter(SortRoute R, int identity
   uint32_t* __restrict__ keys_out = route_pick(R.k, dst);
   uint32_t* __restrict__ vals_out = route_pick(R.v, dst);
   uint32_t* __restrict__ vals2_out = route_pick(R.w, dst);
-  const uint32_t key_base = (kPlan && R.pass == 0) ? *R.key_base : 0u;
+  const KeyRewrite rewrite = key_rewrite<kPlan>(R);
   const long long n = item_count(n_cap, n_dev);
-  if ((long long)blockIdx.x * kScanTile >= n) return;  // whole block past the items (uniform)
-  __shared__ uint32_t s_cnt[4][256];   // per-wave digit counts, then per-wave digit offsets
-  __shared__ uint32_t s_start[256];    // block-local start of each digit
-  __shared__ uint32_t s_global[256];   // global start of each digit for this block
-  __shared__ uint32_t s_count;         // items of the block kept
-  __shared__ uint32_t s_keys[kScanTile];
-  __shared__ uint32_t s_vals[kScanTile];
+  if ((long long)blockIdx.x * kItems >= n) return;  // whole block past the items (uniform)
+  __shared__ uint32_t s_cnt[kWaves][kWords];  // per-wave digit counts, then per-wave digit offsets (16-bit halves)
+  __shared__ uint32_t s_start[kRadix];        // block-local start of each digit, then global start less it
+  __shared__ uint32_t s_count;                // items of the block kept
+  __shared__ uint32_t s_keys[kItems];
+  __shared__ uint32_t s_vals[kItems];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t mask = (1u << bits) - 1u;
-  const long long base = (long long)blockIdx.x * kScanTile;
+  const long long base = (long long)blockIdx.x * kItems;
   const long long wbase = base + (long long)wave * kRsWaveItems;
+  for (int c = tid; c < kWords; c += kThreads)
 #pragma unroll
-  for (int w = 0; w < 4; ++w) s_cnt[w][tid] = 0;
-  {
-    // the block-level scan of the digit table, plus its second level's
-    // prefix when the table spans several scan blocks
-    const long long e = (long long)tid * nblocks + blockIdx.x;
-    s_global[tid] = hist[e] + (hist_prefix ? hist_prefix[e / kScanTile] : 0u);
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][c] = 0;
+  // the global start of this thread's digits: the block-level scan of the
+  // digit table, plus its second level's prefix when the table spans several
+  // scan blocks
+  uint32_t gstart[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const long long e = (long long)(tid * kPer + i) * nblocks + blockIdx.x;
+    gstart[i] = hist[e] + (hist_prefix ? hist_prefix[e / kScanTile] : 0u);
   }
   __syncthreads();
   const uint64_t below = __lanemask_lt();
@@ -310,8 +384,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(SortRoute R, int identity
 #pragma unroll
   for (int r = 0; r < kRsRounds; ++r) {
     const long long j = wbase + (long long)r * 64 + lane;
-    k[r] = j < n ? keys[j] : kDropKey;
-    if (kPlan && k[r] != kDropKey) k[r] -= key_base;
+    k[r] = j < n ? rewrite(keys[j], j) : kDropKey;
     const bool valid = j < n && !(drop && k[r] == kDropKey);
     v[r] = valid ? (vals ? vals[j] : (uint32_t)j) : 0u;
     if constexpr (kV2) v2[r] = valid ? vals2[j] : 0u;
@@ -329,7 +402,8 @@ __global__ __launch_bounds__(256) void k_radix_scatter(SortRoute R, int identity
     rank[r] = valid ? (uint32_t)__popcll(peers & below) : 0xFFFFFFFFu;
     leader[r] = valid ? __builtin_ctzll(peers) : lane;
     old[r] = 0u;
-    if (valid && (peers & below) == 0ull) old[r] = atomicAdd(&s_cnt[wave][d], (uint32_t)__popcll(peers));
+    if (valid && (peers & below) == 0ull)
+      old[r] = half_of(atomicAdd(&s_cnt[wave][d >> 1], (uint32_t)__popcll(peers) << (16u * (d & 1u))), d);
   }
 #pragma unroll
   for (int r = 0; r < kRsRounds; ++r) {
@@ -337,61 +411,88 @@ __global__ __launch_bounds__(256) void k_radix_scatter(SortRoute R, int identity
     if (rank[r] != 0xFFFFFFFFu) rank[r] += base;
   }
   __syncthreads();
-  // per digit: offsets of each wave inside the digit, and the digit's total
-  uint32_t total = 0;
+  // per digit: offsets of each wave inside the digit, and the digit's total;
+  // thread t takes digits kPer * t .. kPer * t + kPer - 1
+  uint32_t tot[kPer], sum = 0;
 #pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    const uint32_t c = s_cnt[w][tid];
-    s_cnt[w][tid] = total;
-    total += c;
+  for (int i = 0; i < kPer; ++i) tot[i] = 0;
+  if (kPer == 2) {  // word t holds both of this thread's digits
+    uint32_t run = 0;  // both halves at once: each stays below 65536
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_cnt[w][tid];
+      s_cnt[w][tid] = run;
+      run += c;
+    }
+    tot[0] = run & 0xFFFFu;
+    tot[kPer - 1] = run >> 16;
+  } else if ((tid & 1) == 0) {  // the even thread of a pair updates the shared word
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_cnt[w][tid >> 1];
+      s_cnt[w][tid >> 1] = run;
+      run += c;
+    }
+    tot[0] = run & 0xFFFFu;
+    s_start[tid + 1] = run >> 16;  // the odd digit's total, handed over below
   }
-  // block-wide exclusive scan of the digit totals (thread = digit)
+  if (kPer == 1) {
+    __syncthreads();
+    if (tid & 1) tot[0] = s_start[tid];
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) sum += tot[i];
+  // block-wide exclusive scan of the digit totals in digit order
   {
-    uint32_t x = total;
+    uint32_t x = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const uint32_t y = __shfl_up(x, d, 64);
       if (lane >= d) x += y;
     }
-    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_wsum[kWaves];
     if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
     uint32_t add = 0;
     for (int w = 0; w < wave; ++w) add += s_wsum[w];
-    s_start[tid] = x - total + add;
-    if (tid == 255) s_count = x + add;
+    uint32_t run = x - sum + add;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      s_start[tid * kPer + i] = run;
+      run += tot[i];
+    }
+    if (tid == kThreads - 1) s_count = x + add;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRsRounds; ++r) {
     if (rank[r] == 0xFFFFFFFFu) continue;
     const uint32_t d = (k[r] >> shift) & mask;
-    const uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
-    s_keys[pos] = k[r];
-    s_vals[pos] = v[r];
+    rank[r] += s_start[d] + half_of(s_cnt[wave][d >> 1], d);  // from here: the item's place in the block
+    s_keys[rank[r]] = k[r];
+    s_vals[rank[r]] = v[r];
   }
   __syncthreads();
+  // from here s_start[d] is the digit's global start less its block start
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) s_start[tid * kPer + i] = gstart[i] - s_start[tid * kPer + i];
+  __syncthreads();
   const int count = (int)s_count;
-  for (int p = tid; p < count; p += 256) {
+  for (int p = tid; p < count; p += kThreads) {
     const uint32_t kk = s_keys[p];
-    const uint32_t d = (kk >> shift) & mask;
-    const uint32_t dst = s_global[d] + (uint32_t)p - s_start[d];
+    const uint32_t dst = s_start[(kk >> shift) & mask] + (uint32_t)p;
     keys_out[dst] = kk;
     vals_out[dst] = s_vals[p];
   }
   if constexpr (kV2) {
     __syncthreads();  // every value is out of s_vals: it takes the second values
 #pragma unroll
-    for (int r = 0; r < kRsRounds; ++r) {
-      if (rank[r] == 0xFFFFFFFFu) continue;
-      const uint32_t d = (k[r] >> shift) & mask;
-      s_vals[s_start[d] + s_cnt[wave][d] + rank[r]] = v2[r];
-    }
+    for (int r = 0; r < kRsRounds; ++r)
+      if (rank[r] != 0xFFFFFFFFu) s_vals[rank[r]] = v2[r];
     __syncthreads();
-    for (int p = tid; p < count; p += 256) {
-      const uint32_t d = (s_keys[p] >> shift) & mask;
-      vals2_out[s_global[d] + (uint32_t)p - s_start[d]] = s_vals[p];
-    }
+    for (int p = tid; p < count; p += kThreads)
+      vals2_out[s_start[(s_keys[p] >> shift) & mask] + (uint32_t)p] = s_vals[p];
   }
 }
 
@@ -448,7 +549,7 @@ long long scan_workspace_elems(long long max_n) {
   return total + (max_n + kScanTile - 1) / kScanTile + 16;
 }
 
-long long sort_hist_elems(long long max_n) { return 256LL * ((max_n + kScanTile - 1) / kScanTile + 1); }
+long long sort_hist_elems(long long max_n) { return 512LL * ((max_n + kScanTile - 1) / kScanTile + 1); }
 
 hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx, uint32_t* out, long long n,
                                  uint32_t* total, const ScanWorkspace& ws, hipStream_t stream, int xform,
@@ -480,45 +581,64 @@ static hipError_t scan_no_add(uint32_t* data, long long n, uint32_t* total, cons
   return hipGetLastError();
 }
 
+namespace {
+
+template <bool kPlan, int kDigitBits>
+auto hist_kernel(bool aggregate) {
+  return aggregate ? k_radix_hist<true, kPlan, kDigitBits> : k_radix_hist<false, kPlan, kDigitBits>;
+}
+template <bool kPlan, int kDigitBits>
+auto scatter_kernel(bool v2) {
+  return v2 ? k_radix_scatter<true, kPlan, kDigitBits> : k_radix_scatter<false, kPlan, kDigitBits>;
+}
+
+}  // namespace
+
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
                              hipStream_t stream, const uint32_t* n_dev, uint32_t* vals2, uint32_t* vals2_alt,
                              bool identity_vals, uint32_t* n_kept, const SortPlan* plan) {
   *result_in_alt = false;
   if (end_bit <= begin_bit || n <= 0) return hipSuccess;
-  const long long nb = (n + kScanTile - 1) / kScanTile;
-  if (256 * nb > ws.hist_capacity) return hipErrorInvalidValue;
-  if (plan && (begin_bit != 0 || end_bit != 32 || (plan->vals2_c == nullptr) != (vals2 == nullptr)))
-    return hipErrorInvalidValue;  // a plan covers a full 32-bit key
+  const int digit = plan ? plan->digit_bits : 8;
+  if (digit != 8 && digit != 9) return hipErrorInvalidValue;
+  const int threads = digit == 9 ? RadixShape<9>::kThreads : RadixShape<8>::kThreads;
+  const long long items = digit == 9 ? RadixShape<9>::kItems : RadixShape<8>::kItems;
+  const long long nb = (n + items - 1) / items;
+  if ((1LL << digit) * nb > ws.hist_capacity) return hipErrorInvalidValue;
+  if (plan && (begin_bit != 0 || (plan->keys_c && (plan->vals2_c == nullptr) != (vals2 == nullptr))))
+    return hipErrorInvalidValue;
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt, *wi = vals2, *wo = vals2_alt;
   int pass = 0;
-  for (int shift = begin_bit; shift < end_bit; shift += 8, ++pass) {
-    const int bits = (end_bit - shift) < 8 ? (end_bit - shift) : 8;
+  for (int shift = begin_bit; shift < end_bit; shift += digit, ++pass) {
+    const int bits = (end_bit - shift) < digit ? (end_bit - shift) : digit;
     const bool first = shift == begin_bit;
     // with n_kept, the first pass drops kDropKey and the later ones sort what it kept
     const int drop = n_kept && first ? 1 : 0;
     const uint32_t* count = n_kept && !first ? n_kept : n_dev;
     SortRoute R{};
     if (plan) {
-      R = SortRoute{{keys, keys_alt, plan->keys_c}, {vals, vals_alt, plan->vals_c}, {vals2, vals2_alt, plan->vals2_c},
-                    plan->passes, plan->key_base, pass};
+      const uint32_t size = plan->view_size ? plan->view_size : 1u;
+      R = SortRoute{{keys, keys_alt, plan->keys_c}, {vals, vals_alt, plan->vals_c},
+                    {vals2, vals2_alt, plan->vals2_c}, plan->passes, plan->key_base, plan->view_shift,
+                    size, 1.f / (float)size, pass};
     } else {
-      R = SortRoute{{ki, ko, nullptr}, {vi, vo, nullptr}, {wi, wo, nullptr}, nullptr, nullptr, pass};
+      R = SortRoute{{ki, ko, nullptr}, {vi, vo, nullptr}, {wi, wo, nullptr}, nullptr, nullptr, nullptr, 1u, 1.f, pass};
     }
     const bool aggregate = bits <= 4 || shift + bits == 32;
-    auto hist_kernel = plan ? (aggregate ? k_radix_hist<true, true> : k_radix_hist<false, true>)
-                            : (aggregate ? k_radix_hist<true, false> : k_radix_hist<false, false>);
-    hipLaunchKernelGGL(hist_kernel, dim3((unsigned)nb), dim3(256), 0, stream, R, n, count, shift, bits, drop, ws.hist,
+    auto hk = plan ? (digit == 9 ? hist_kernel<true, 9>(aggregate) : hist_kernel<true, 8>(aggregate))
+                   : hist_kernel<false, 8>(aggregate);
+    hipLaunchKernelGGL(hk, dim3((unsigned)nb), dim3(threads), 0, stream, R, n, count, shift, bits, drop, ws.hist,
                        (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     const uint32_t* prefix = nullptr;
-    SPLAT_HIP_CHECK(scan_no_add(ws.hist, 256 * nb, drop ? n_kept : nullptr, ws.scan, &prefix, stream,
+    SPLAT_HIP_CHECK(scan_no_add(ws.hist, (1LL << digit) * nb, drop ? n_kept : nullptr, ws.scan, &prefix, stream,
                                 plan ? plan->active + pass : nullptr));
     const int identity = (identity_vals && first) ? 1 : 0;
-    auto scatter_kernel = plan ? (vals2 ? k_radix_scatter<true, true> : k_radix_scatter<false, true>)
-                               : (vals2 ? k_radix_scatter<true, false> : k_radix_scatter<false, false>);
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)nb), dim3(256), 0, stream, R, identity, n, count, shift, bits,
-                       drop, ws.hist, prefix, (int)nb);
+    auto sk = plan ? (digit == 9 ? scatter_kernel<true, 9>(vals2 != nullptr) : scatter_kernel<true, 8>(vals2 != nullptr))
+                   : scatter_kernel<false, 8>(vals2 != nullptr);
+    hipLaunchKernelGGL(sk, dim3((unsigned)nb), dim3(threads), 0, stream, R, identity, n, count, shift, bits, drop,
+                       ws.hist, prefix, (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     if (!plan) {
       uint32_t* t = ki; ki = ko; ko = t;
@@ -532,11 +652,17 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
 
 namespace {
 
-// The depth sort's plan from the projection's per-wave ranges of kept keys:
-// every kept key less the minimum fits the low 8 * passes bits, so the
-// passes above them would only copy.
+// The plans of the depth and super-tile sorts from the projection's per-wave
+// ranges of kept keys (layout: the header's kPlan* words). Every kept key
+// less the minimum fits span_bits bits. When view_bits >= 0 and the view id
+// fits above them (span_bits + view_bits <= 31, so no key becomes the drop
+// key), the depth sort also orders by view, above depth: its output, and the
+// super-tile pairs emitted in that order, are grouped by view, and one pass
+// over the 8-bit super-tile id within a view finishes the super-tile sort;
+// otherwise the super-tile sort takes a second pass, by view. The depth sort
+// runs 9-bit digits, as many passes as the key's bits need (at least 2).
 __global__ __launch_bounds__(1024) void k_sort_plan(const uint32_t* __restrict__ key_ranges, int n_waves,
-                                                    uint32_t* __restrict__ plan) {
+                                                    int view_bits, uint32_t* __restrict__ plan) {
   __shared__ uint32_t s_lo[16], s_hi[16];
   uint32_t lo = 0xFFFFFFFFu, hi = 0u;
   for (int w = threadIdx.x; w < n_waves; w += 1024) {
@@ -555,22 +681,30 @@ __global__ __launch_bounds__(1024) void k_sort_plan(const uint32_t* __restrict__
     lo = min(lo, s_lo[k]);
     hi = max(hi, s_hi[k]);
   }
-  uint32_t passes = 2, base = 0;
+  uint32_t base = 0;
+  int span_bits = 0;
   if (hi >= lo) {  // some key was kept
     base = lo;
     const uint32_t span = hi - lo;
-    const int bits = span ? 32 - __clz((int)span) : 0;
-    passes = (uint32_t)max(2, (bits + 7) / 8);
+    span_bits = span ? 32 - __clz((int)span) : 0;
   }
-  plan[0] = passes;
-  for (int i = 0; i < 4; ++i) plan[1 + i] = (uint32_t)i < passes ? 0xFFFFFFFFu : 0u;
-  plan[5] = base;
+  const bool grouped = view_bits >= 0 && span_bits + view_bits <= 31;
+  const int key_bits = span_bits + (grouped ? view_bits : 0);
+  const uint32_t passes = (uint32_t)max(2, (key_bits + kDepthDigitBits - 1) / kDepthDigitBits);
+  plan[kPlanPasses] = passes;
+  for (int i = 0; i < 4; ++i) plan[kPlanActive + i] = (uint32_t)i < passes ? 0xFFFFFFFFu : 0u;
+  plan[kPlanKeyBase] = base;
+  plan[kPlanViewShift] = grouped ? (uint32_t)span_bits : 32u;
+  const uint32_t super_passes = grouped || view_bits <= 0 ? 1u : 2u;
+  plan[kPlanSuperPasses] = super_passes;
+  for (int i = 0; i < 2; ++i) plan[kPlanSuperActive + i] = (uint32_t)i < super_passes ? 0xFFFFFFFFu : 0u;
 }
 
 }  // namespace
 
-hipError_t launch_sort_plan(const uint32_t* key_ranges, int n_waves, uint32_t* plan, hipStream_t stream) {
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(1024), 0, stream, key_ranges, n_waves, plan);
+hipError_t launch_sort_plan(const uint32_t* key_ranges, int n_waves, int view_bits, uint32_t* plan,
+                            hipStream_t stream) {
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(1024), 0, stream, key_ranges, n_waves, view_bits, plan);
   return hipGetLastError();
 }
 

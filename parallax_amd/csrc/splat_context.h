@@ -84,6 +84,7 @@ struct splat_context {
   // every view at most kPackedTiles tiles a side: packed rects ride along with
   // the depth sort (in scounts), and the emitter reads them in depth order
   bool packed = false;
+  int st_view_bits = -1;  // super-tiles numbered view * 256 + id within the view: bits of the last view index; else -1
   uint32_t* sorted_prects = nullptr;
   uint32_t* sorted_slots = nullptr;
   uint32_t* sorted_pair_vals = nullptr;

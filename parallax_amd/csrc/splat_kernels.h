@@ -56,7 +56,7 @@ struct SortWorkspace {
 long long sort_hist_elems(long long max_n);
 
 // Stable LSD radix sort of (key, value) pairs over key bits [begin_bit,
-// end_bit), 8-bit digits; key bits above end_bit ride along. Ping-pongs
+// end_bit), 8-bit digits (a plan may take 9); key bits above end_bit ride along. Ping-pongs
 // between (keys, vals) and (keys_alt, vals_alt); *result_in_alt tells which
 // pair holds the sorted output. With n_dev, the item count is read on the
 // device (clamped to n, which then only sizes the launches), so the sort can
@@ -67,29 +67,42 @@ long long sort_hist_elems(long long max_n);
 // n_kept (device), items whose key is 0xFFFFFFFF are dropped, *n_kept is set
 // to the count of the rest, and the sorted output holds those only.
 //
-// With a plan (a full 32-bit key only), the number of 8-bit passes is read on
-// the device: the first pass takes each kept key less *key_base, and the
-// passes at or past *passes do nothing (each of their kernels returns at
-// once). The output then always ends in (keys, vals, vals2), through the
-// third buffers when three passes run; *result_in_alt is false.
+// With a plan (begin_bit 0), the passes to run are read on the device:
+// each pass covers digit_bits bits (8 or 9), and a pass at or past *passes
+// does nothing (each of its kernels returns at once). The output always ends
+// in (keys, vals, vals2) and *result_in_alt is false. With third buffers
+// (keys_c ...) the input is in (keys, vals, vals2); without them it must be
+// in the first set for an even number of passes and in the second
+// (keys_alt ...) for an odd one. With key_base, the first pass rewrites each
+// kept key k as k - *key_base, with, when *view_shift < 32, the item's view
+// (item index / view_size) placed at bit *view_shift.
 struct SortPlan {
-  const uint32_t* passes;    // device: the passes to run, 2 to 4
-  const uint32_t* active;    // device, per pass: 0 when it is skipped (caps its scans' item count)
-  const uint32_t* key_base;  // device: subtracted from every kept key
-  uint32_t* keys_c;          // third buffers, each of n elements
+  const uint32_t* passes;      // device: the passes to run
+  const uint32_t* active;      // device, per pass: 0 when it is skipped (caps its scans' item count)
+  const uint32_t* key_base;    // device, or null: no rewrite
+  const uint32_t* view_shift;  // device, or null
+  uint32_t view_size;
+  int digit_bits;
+  uint32_t* keys_c;            // third buffers, each of n elements, or null
   uint32_t* vals_c;
-  uint32_t* vals2_c;         // with vals2 only
+  uint32_t* vals2_c;           // with vals2 only
 };
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, long long n,
                              int begin_bit, int end_bit, const SortWorkspace& ws, bool* result_in_alt,
                              hipStream_t stream, const uint32_t* n_dev = nullptr, uint32_t* vals2 = nullptr,
                              uint32_t* vals2_alt = nullptr, bool identity_vals = false, uint32_t* n_kept = nullptr,
                              const SortPlan* plan = nullptr);
-// The plan of a depth sort (plan[0] passes, plan[1..4] active, plan[5] key
-// base, as SortPlan reads them) from the projection's per-wave key ranges
-// (n_waves (min, max) pairs).
-constexpr int kSortPlanWords = 6;
-hipError_t launch_sort_plan(const uint32_t* key_ranges, int n_waves, uint32_t* plan, hipStream_t stream);
+// The plans of the binning's two sorts, from the projection's per-wave key
+// ranges (n_waves (min, max) pairs), into plan[kPlanWords]: the depth sort's
+// passes, per-pass active words, key base and view shift (32: no view bits),
+// and the super-tile sort's passes and active words. view_bits: bits of the
+// batch's largest view index when super-tiles are numbered view * 256 +
+// super-tile within the view (every view at most 256 of them), else -1.
+constexpr int kDepthDigitBits = 9;
+constexpr int kPlanPasses = 0, kPlanActive = 1, kPlanKeyBase = 5, kPlanViewShift = 6, kPlanSuperPasses = 7,
+              kPlanSuperActive = 8, kPlanWords = 10;
+hipError_t launch_sort_plan(const uint32_t* key_ranges, int n_waves, int view_bits, uint32_t* plan,
+                            hipStream_t stream);
 // *out (device) += the sum of in[0..n).
 hipError_t launch_sum_u32(const uint32_t* in, long long n, uint32_t* out, hipStream_t stream);
 
@@ -105,11 +118,14 @@ constexpr uint32_t kStMask = (1u << kStBits) - 1u;
 // pair_base[slot] = offsets[s]. Pairs at or past s_cap are dropped (the
 // capacity check flags the step).
 // The rects are read from sorted_prects (the packed rects in depth order)
-// when given, else gathered from rects[slot].
+// when given, else gathered from rects[slot]. The pairs go to (skeys,
+// svals), or to (skeys_odd, svals_odd) when sort_passes (device, nullable:
+// the next sort's pass count under its plan) is odd.
 hipError_t launch_emit_super(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* soffsets,
                              const uint32_t* s_total, const uint2* rects, const uint32_t* sorted_prects,
                              const ViewDesc* views, long long per_view, long long m, const uint32_t* m_dev,
-                             uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* pair_base,
+                             uint32_t s_cap, uint32_t* skeys, uint32_t* svals, uint32_t* skeys_odd,
+                             uint32_t* svals_odd, const uint32_t* sort_passes, uint32_t* pair_base,
                              hipStream_t stream);
 // sranges[2*st] / [2*st+1] = [begin, end) of super-tile st in the sorted
 // pairs; the pair count is min(*n_dev, n) with n_dev, else n.
