@@ -10,6 +10,17 @@ each against the oracle's buildTiles (SplatRaster.cpp:114-151), bit for bit:
 
 The same views binned one at a time and in the big batch must also give the
 same lists, and deterministic mode must repeat bit for bit on each path.
+
+The sorts' plans, each against the checker too:
+
+* depths from 0.05 to 400 m (a 27-bit span of float keys) in 8 views: the
+  view bits fit above the depth bits (30 of 31), and the depth sort runs all
+  four 9-bit passes;
+* the same depths in 64 views: 33 bits do not fit, so the slots are not
+  grouped by view and the super-tile sort runs its second pass, by view;
+* a view of 18 x 18 super-tiles: more than 256, so super-tiles are numbered
+  across the batch and the super-tile sort takes the plain route;
+* every splat behind the cameras: no kept key, empty lists.
 """
 import numpy as np
 import pytest
@@ -96,3 +107,63 @@ def test_deterministic_mode_on_every_binning_path(ctx):
     rel = np.linalg.norm(alone[0] - mixed[0]) / np.linalg.norm(alone[0])
     print(f"[bin] packed vs gathered-rect batch: gradients rel-L2 {rel:.3g}", flush=True)
     assert rel <= 1e-6
+
+
+def deep_cloud(n, seed, near=0.05, far=400.0):
+    """Splats inside a +-0.45 (x/z, y/z) cone in front of the origin, depths
+    log-uniform in [near, far], sizes growing with depth (the cameras below
+    look along +z from the origin)."""
+    rng = np.random.default_rng(seed)
+    cloud = syn.random_cloud(n, seed, 1, (-1, -1, 1), (1, 1, 2))
+    z = np.exp(rng.uniform(np.log(near), np.log(far), size=n))
+    uv = rng.uniform(-0.45, 0.45, size=(n, 2))
+    cloud.pos[:] = np.stack([uv[:, 0] * z, uv[:, 1] * z, z], axis=1).astype(np.float32)
+    cloud.log_scale[:] = (np.log(0.004 * z)[:, None] + rng.normal(0.0, 0.3, size=(n, 3))).astype(np.float32)
+    return cloud
+
+
+def batch_against_checker(ctx, cloud, views, check, sh=1):
+    ctx.set_capacity(cloud.n, sh)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views)
+    opts = S.options(sh_degree=sh)
+    ctx.render_forward(list(range(len(views))), opts)
+    batch = {k: ctx.tiles(k) for k in check}
+    lists_equal_checker(ctx, cloud, views, [(k, views[k]) for k in check], sh)
+    return batch, opts
+
+
+@pytest.mark.parametrize("n_views", [8, 64])
+def test_depth_keys_over_27_bits(ctx, n_views):
+    cloud = deep_cloud(3000, 31)
+    keys = cloud.pos[:, 2].astype(np.float32).view(np.uint32)
+    span = int(keys.max()) - int(keys.min())
+    assert span.bit_length() == 27  # with 3 view bits: 30, four 9-bit passes; with 6: 33, not grouped
+    views = [syn.pinhole(320, 200, 300.0)] * n_views
+    check = [0, n_views // 2, n_views - 1]
+    print(f"[bin] depths 0.05..400 m in {n_views} views", flush=True)
+    batch, opts = batch_against_checker(ctx, cloud, views, check)
+    ctx.render_forward([0], opts)
+    one = ctx.tiles(0)
+    assert np.array_equal(one[2], batch[0][2]) and np.array_equal(one[3], batch[0][3])
+
+
+def test_super_tiles_numbered_across_the_batch(ctx):
+    cloud = syn.random_cloud(6000, 32, 1, (-2, -2, -2), (2, 2, 2), log_scale_mu=-2.5)
+    big = syn.orbit_views(1, 6.0, [(2304, 2304)], seed=5)[0]  # 18 x 18 super-tiles
+    small = syn.orbit_views(1, 6.0, [(320, 200)], seed=6)[0]
+    print("[bin] 2304^2 + 320x200 (more than 256 super-tiles in a view)", flush=True)
+    batch_against_checker(ctx, cloud, [big, small], [0, 1])
+
+
+def test_every_splat_culled(ctx):
+    cloud = syn.random_cloud(2000, 33, 1, (-1, -1, -3), (1, 1, -2))  # behind the cameras
+    views = [syn.pinhole(320, 200, 300.0)] * 3
+    print("[bin] nothing in front of the cameras", flush=True)
+    ctx.set_capacity(cloud.n, 1)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views)
+    ctx.render_forward([0, 1, 2], S.options(sh_degree=1))
+    for k in range(3):
+        cols, rows, offs, ents = ctx.tiles(k)
+        assert len(ents) == 0 and not offs.any(), k
