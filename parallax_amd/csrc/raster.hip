@@ -58,7 +58,6 @@ __device__ __forceinline__ float falloff(float power) {
 }
 
 constexpr int kChunk = 64;    // forward: splats fetched per wave step
-constexpr int kVals = 11;  // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3, touched
 
 __device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int tile) {
   int v = 0;
@@ -335,83 +334,154 @@ __global__ __launch_bounds__(FwdShape<kShape>::kThreads) void k_forward(
   }
 }
 
-// Cross-lane exchange with the partner whose lane index differs in bit
-// kBit (0..3), through DPP: quad_perm for bits 0 and 1, row_half_mirror then
-// quad_perm [3,2,1,0] for bit 2, row_ror:8 for bit 3.
+// DPP move within a row of 16 lanes (quad_perm controls only here).
 template <int kCtrl>
 __device__ __forceinline__ float dpp(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
 }
-template <int kBit>
-__device__ __forceinline__ float xchg(float v) {
-  if constexpr (kBit == 0) return dpp<0xB1>(v);
-  else if constexpr (kBit == 1) return dpp<0x4E>(v);
-  else if constexpr (kBit == 2) return dpp<0x1B>(dpp<0x141>(v));
-  else return dpp<0x128>(v);
+
+// The adjoint's per-splat reduction goes through LDS: each lane writes its 10
+// accumulated terms into a [term][lane] array, then lanes 4i..4i+3 each read
+// a quarter of term i's row (16 lanes, four 16-byte loads), sum it in a fixed
+// tree, and combine the four quarters with two quad DPP adds (commutative, so
+// all four lanes hold the same bits). Lane 4i then holds term i's wave total.
+// The array is laid out for the LDS banks: element (term i, lane l) at
+// i * kRedPitch + (l / 16) * kRedGroup + l % 16, which keeps the quarter-row
+// reads free of bank conflicts.
+constexpr int kAcc = 10;       // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3
+constexpr int kRedGroup = 20;  // floats per 16-lane group in a row
+constexpr int kRedPitch = 80;  // floats per term row
+
+__device__ __forceinline__ float lds_quarter_sum(const float* s_red, int lane) {
+  const int i = min(lane >> 2, kAcc - 1);  // lanes 40..63 repeat term 9's reads; their sums are not used
+  const float4* src = reinterpret_cast<const float4*>(s_red + i * kRedPitch + (lane & 3) * kRedGroup);
+  const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
+  const float t0 = (q0.x + q0.y) + (q0.z + q0.w), t1 = (q1.x + q1.y) + (q1.z + q1.w);
+  const float t2 = (q2.x + q2.y) + (q2.z + q2.w), t3 = (q3.x + q3.y) + (q3.z + q3.w);
+  float t = (t0 + t1) + (t2 + t3);
+  t = t + dpp<0xB1>(t);  // quad_perm [1,0,3,2]
+  t = t + dpp<0x4E>(t);  // quad_perm [2,3,0,1]
+  return t;
 }
 
-// One transposed-butterfly step across lane bit 5 (or 4) with no lane
-// selects: v_permlane32_swap (v_permlane16_swap) exchanges the upper half
-// (odd rows) of x with the lower half (even rows) of y, so x' + y' holds,
-// in the lower lanes, the pair sum of x, and in the upper lanes the pair sum
-// of y.
-__device__ __forceinline__ float swap32_sum(float x, float y) {
-  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
-  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// A staged list entry of the adjoint: the compositing fields of its record
+// (one 48-byte slot, so the three loads share one address).
+struct alignas(16) Staged {
+  float4 a;  // mx, my, conic a, conic b
+  float4 b;  // conic c, opacity, r, g
+  float c;   // b
+};
+
+// The lane's four pixels (one per 8x8 quadrant of the tile) in the engine's
+// backward walk (SplatRaster.cpp:346-433): dL/drgb, the background term, the
+// transmittance and the suffix colour, kept folded (the value the next
+// contributor's step reads, updated as soon as this one is done) so the
+// previous contributor's alpha and colour need no registers.
+struct AdjPixels {
+  float dL0[4], dL1[4], dL2[4], tb[4], T[4], suf0[4], suf1[4], suf2[4];
+  int lastp[4];
+};
+
+// Lane masks (64-bit, in scalar registers) of per-lane float and integer
+// compares, and selects by such a mask. The adjoint combines its decisions as
+// masks: a ballot of a combined condition, or a select on one, would copy the
+// mask through a vector register first.
+constexpr int kFcmpOlt = 4;   // ordered less than
+constexpr int kFcmpUge = 11;  // unordered or greater or equal: !(a < b)
+constexpr int kFcmpUle = 13;  // unordered or less or equal: !(a > b)
+constexpr int kIcmpSlt = 40;  // signed less than
+__device__ __forceinline__ float select_lanes(uint64_t lanes, float if_set, float if_clear) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_clear), "v"(if_set), "s"(lanes));
+  return r;
 }
-__device__ __forceinline__ float swap16_sum(float x, float y) {
-  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
-  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+__device__ __forceinline__ float keep_lanes(uint64_t lanes, float x) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(x), "s"(lanes));
+  return r;
 }
 
-// Sum 11 per-lane values over the wave64 in one transposed butterfly: each
-// step exchanges half of the values a lane holds with its partner and keeps
-// the other half. The 11 values (and a zero) go through 6, then 3 registers
-// across lane bits 5 and 4 (permlane swaps, no selects), then 2 and 1 across
-// bits 0 and 1 (DPP; the third register pairs with itself); bits 2 and 3
-// then sum the one remaining value. Lane l ends with the wave total of value
-// wave_sum_slot(l) (slot 11 is the zero); lanes with bits 0 and 1 both set
-// repeat their bit-0 partner's slot. The summation order is fixed, so it is
-// deterministic.
-__device__ __forceinline__ float wave_sum11(const float (&v)[kVals], int lane) {
-  float r1[6], r2[3];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) r1[i] = swap32_sum(v[2 * i], v[2 * i + 1]);
-  r1[5] = swap32_sum(v[10], 0.f);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) r2[m] = swap16_sum(r1[2 * m], r1[2 * m + 1]);
-  const bool b0 = lane & 1, b1 = lane & 2;
-  const float send0 = b0 ? r2[0] : r2[1];
-  const float keep0 = b0 ? r2[1] : r2[0];
-  const float r3a = keep0 + xchg<0>(send0);
-  const float r3b = r2[2] + xchg<0>(r2[2]);
-  const float send1 = b1 ? r3a : r3b;
-  const float keep1 = b1 ? r3b : r3a;
-  float e = keep1 + xchg<1>(send1);
-  e = e + xchg<2>(e);
-  e = e + xchg<3>(e);
-  return e;
-}
-
-// The value whose total wave_sum11 leaves in lane `lane`.
-__device__ __forceinline__ int wave_sum_slot(int lane) {
-  const int b0 = lane & 1, b1 = (lane >> 1) & 1, b4 = (lane >> 4) & 1, b5 = (lane >> 5) & 1;
-  return 4 * (b1 ? 2 : b0) + b5 + 2 * b4;
+// Splat (a, b, cblue) at list index idx over quadrant k's pixels: the forward's
+// decisions recomputed with its exact float operations, then the gradient
+// terms added into acc. Every lane computes and non-contributing lanes add
+// exact zeros; a quadrant with no contributing lane returns early. Returns
+// whether any lane contributed; adds the pixels that shaped the splat (alpha
+// below the clamp) to `shaped`.
+//
+// kLowCut false: the falloff's exponent is clamped to [-80, 0] with no test of
+// power <= -80. The forward sets G = 0 there; here G = e^-80 instead, whose
+// alpha (<= 1.9e-35) is below any min_alpha >= 1e-30, so the pixel does not
+// contribute either way and, G being read only where it does, nothing else
+// changes. kLowCut true keeps the forward's test, for smaller min_alpha.
+template <int k, bool kLowCut>
+__device__ __forceinline__ bool adjoint_quad(float (&acc)[kAcc], uint32_t& shaped, AdjPixels& s, const float4& a,
+                                             const float4& b, float cblue, int idx, float fx0, float fy0,
+                                             float min_alpha) {
+  const float dx = (k & 1 ? fx0 + 8.f : fx0) - a.x, dy = (k >> 1 ? fy0 + 8.f : fy0) - a.y;
+  const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+  const float G = kLowCut ? falloff(power) : splat_det::exp_core(__builtin_amdgcn_fmed3f(power, -80.0f, 0.0f));
+  const float raw = b.y * G;
+  const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
+  // the pixels that contribute: idx < last, !(power > 0), !(alpha < min_alpha)
+  const uint64_t cmask = __builtin_amdgcn_sicmp(idx, s.lastp[k], kIcmpSlt) &
+                         __builtin_amdgcn_fcmpf(power, 0.f, kFcmpUle) &
+                         __builtin_amdgcn_fcmpf(alpha, min_alpha, kFcmpUge);
+  if (cmask == 0ull) return false;
+  {
+  // the engine's expressions (SplatRaster.cpp:403-428) arranged for fused
+  // multiply-adds: alpha_c is alpha where this pixel contributes and 0
+  // elsewhere, so a dropped step changes nothing; the suffix colour
+  // alpha c + (1 - alpha) suf is suf + alpha_c (c - suf), reusing dL/dalpha's
+  // differences
+#pragma clang fp contract(fast)
+  // the contributing pixels whose alpha is below the clamp shape the splat
+  const uint64_t smask = cmask & __builtin_amdgcn_fcmpf(raw, kMaxAlpha, kFcmpOlt);
+  const float alpha_c = keep_lanes(cmask, alpha);
+  const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+  const float Tn = s.T[k] * inv;
+  const float weight = alpha_c * Tn;
+  const float d0 = b.z - s.suf0[k], d1 = b.w - s.suf1[k], d2 = cblue - s.suf2[k];
+  const float dL_dalpha = (d0 * s.dL0[k] + d1 * s.dL1[k] + d2 * s.dL2[k]) * Tn - s.tb[k] * inv;
+  const float gd = keep_lanes(smask, G) * dL_dalpha;  // dL/dopacity term
+  const float pw = b.y * gd;                            // dL/dpower
+  const float px = pw * dx, py = pw * dy;
+  const float u = px * a.z + py * a.w;  // -dL/dmean_x
+  const float w = py * b.x + px * a.w;  // -dL/dmean_y
+  acc[0] += weight * s.dL0[k];
+  acc[1] += weight * s.dL1[k];
+  acc[2] += weight * s.dL2[k];
+  acc[3] += gd;
+  acc[4] += u;
+  acc[5] += w;
+  acc[6] += __builtin_amdgcn_sqrtf(u * u + w * w);  // densify statistic: hardware sqrt
+  acc[7] -= px * dx;  // dL/dconic xx and yy are halved after the wave sum
+  acc[8] -= px * dy;
+  acc[9] -= py * dy;
+  shaped += (uint32_t)__popcll(smask);
+  s.T[k] = select_lanes(cmask, Tn, s.T[k]);
+  s.suf0[k] += alpha_c * d0;
+  s.suf1[k] += alpha_c * d1;
+  s.suf2[k] += alpha_c * d2;
+  }
+  return true;
 }
 
 // The adjoint of one 16x16 tile by one wave: each lane owns four pixels, one
 // per 8x8 quadrant (lane & 7, lane >> 3 within each), so a splat's terms over
-// the whole tile are reduced by a single wave butterfly and the four
-// quadrants never meet in LDS or at a barrier. The tile's list is walked back
-// to front from the tile's last contributor in chunks of kBB entries: the
-// lanes stage the chunk's records in LDS with a 4-bit mask of the quadrants
-// whose cull box they meet below that quadrant's last contributor; then, per
-// marked splat from the back, each marked quadrant's pixels are evaluated
-// (a wave-uniform branch per quadrant), the lane adds its pixels' terms, the
-// wave sums them, and after the chunk each lane issues its splat's result: 11
-// hardware float atomics (atomic mode) or one 64-byte partial store
+// the whole tile are reduced once per wave and the four quadrants never meet
+// at a barrier. The tile's list is walked back to front from the tile's last
+// contributor in chunks of kBB entries: the lanes stage the chunk's records
+// in LDS with a 4-bit mask of the quadrants whose cull box they meet below
+// that quadrant's last contributor; then, per marked splat from the back,
+// each marked quadrant's pixels are evaluated (a wave-uniform branch per
+// quadrant), the lane adds its pixels' terms, the wave sums them through LDS,
+// and lanes 0, 4, .., 40 issue the splat's result: up to 11 hardware float
+// atomics (atomic mode), or 16 lanes one 64-byte partial store
 // (deterministic mode, summation order fixed by the code alone).
-template <int kBB, bool kQueue>
+template <int kBB, bool kQueue, bool kLowCut>
 __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
                                                  const uint32_t* __restrict__ ranges,
                                                  const uint32_t* __restrict__ pair_vals,
@@ -422,23 +492,28 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
                                                  const uint32_t* __restrict__ epoch,
                                                  const uint32_t* __restrict__ order, uint32_t* __restrict__ counter) {
   static_assert(kBB == 64 || kBB == 128, "chunks of one or two entries per lane");
+  static_assert(kScreenStride >= kAcc + 1 && kPartialStride == 16, "record slots");
   constexpr int kHalves = kBB / 64;
-  __shared__ float4 s_a[kBB];  // mx, my, ca, cb
-  __shared__ float4 s_b[kBB];  // cc, opacity, r, g
-  __shared__ float s_c[kBB];   // b
+  __shared__ Staged s_rec[kBB];
+  __shared__ __attribute__((aligned(16))) float s_red[kAcc * kRedPitch];
+  __shared__ f32x4 s_zero[2];  // the accumulators' initial zeros
+  __shared__ f32x2 s_zero2;
   __shared__ int s_unit;
   const int lane = threadIdx.x;
-  const int my_slot = wave_sum_slot(lane);
+  if (lane < 2) s_zero[lane] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (lane == 2) s_zero2 = f32x2{0.f, 0.f};
+  wave_sync_lds();
+  // after the reduction lane 4i holds term i (i < 10); lane 40 takes the
+  // shaped-pixel count (slot 10), which is known in a scalar register
+  const int my_slot = lane >> 2;
+  const bool slot_lane = (lane & 3) == 0;
+  const bool writer = slot_lane && my_slot <= kAcc;
   // terms 7 and 9 (dL/dconic xx, yy) are summed without their factor 1/2,
   // applied to the total (exact: a power of two)
   const float slot_scale = (my_slot == 7 || my_slot == 9) ? 0.5f : 1.f;
-  const bool lead = (lane & 12) == 0 && (lane & 3) != 3;  // one lane per slot after wave_sum11
-  const bool writer = lead && my_slot < kVals;
-  // the 16 lanes that store a deterministic partial record, one slot each
-  const bool rec_lane = (lane & 12) == 0;
-  const int rec_slot = (lane & 3) == 3 ? 12 + ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) : my_slot;
-  // slot 11 carries this pass's tag; slots 12-15 are zero padding
-  const float rec_pad = rec_slot == 11 ? __uint_as_float(deterministic ? *epoch : 0u) : 0.f;
+  // deterministic records: slots 0-10 the terms, 11 this pass's tag, 12-15 zero
+  const float rec_pad = my_slot == 11 ? __uint_as_float(deterministic ? *epoch : 0u) : 0.f;
+  const int red_base = (lane >> 4) * kRedGroup + (lane & 15);
   for (int iter = 0;; ++iter) {
     int tile;
     if (kQueue) {
@@ -457,35 +532,31 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
     const int tx = local % V.cols, ty = local / V.cols;
     const TileQuads tq = tile_quads(V, tx, ty);
     const uint32_t begin = ranges[2 * tile];  // the walk starts at the last contributor, before the list's end
-    // the lane's four pixels: state of the engine's backward walk
-    // (SplatRaster.cpp:346-433), with the suffix colour kept folded (the
-    // value the next contributor's step reads, updated as soon as this one is
-    // done) so the previous contributor's alpha and colour need no registers
     // pixel centres: quadrant 0's, +8 (exact) for the others
     const float fx0 = (float)(tx * kTile + (lane & 7)) + 0.5f, fy0 = (float)(ty * kTile + (lane >> 3)) + 0.5f;
-    float dL0[4], dL1[4], dL2[4], tb[4], T[4], suf0[4], suf1[4], suf2[4];
-    int lastp[4], stops[4];
+    AdjPixels s;
+    int stops[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int px = tx * kTile + 8 * (k & 1) + (lane & 7), py = ty * kTile + 8 * (k >> 1) + (lane >> 3);
-      dL0[k] = dL1[k] = dL2[k] = 0.f;
+      s.dL0[k] = s.dL1[k] = s.dL2[k] = 0.f;
       float T_final = 1.f;
-      lastp[k] = 0;
+      s.lastp[k] = 0;
       if (px < V.width && py < V.height) {
         const long long pixel = V.pixel_base + (long long)py * V.width + px;
-        dL0[k] = dL_drgb[3 * pixel];
-        dL1[k] = dL_drgb[3 * pixel + 1];
-        dL2[k] = dL_drgb[3 * pixel + 2];
+        s.dL0[k] = dL_drgb[3 * pixel];
+        s.dL1[k] = dL_drgb[3 * pixel + 1];
+        s.dL2[k] = dL_drgb[3 * pixel + 2];
         T_final = T_in[pixel];
-        lastp[k] = last_in[pixel];
+        s.lastp[k] = last_in[pixel];
       }
-      tb[k] = T_final * (p.bg[0] * dL0[k] + p.bg[1] * dL1[k] + p.bg[2] * dL2[k]);
-      T[k] = T_final;
-      suf0[k] = suf1[k] = suf2[k] = 0.f;
-      int s = lastp[k];
+      s.tb[k] = T_final * (p.bg[0] * s.dL0[k] + p.bg[1] * s.dL1[k] + p.bg[2] * s.dL2[k]);
+      s.T[k] = T_final;
+      s.suf0[k] = s.suf1[k] = s.suf2[k] = 0.f;
+      int m = s.lastp[k];
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) s = max(s, __shfl_xor(s, off, 64));
-      stops[k] = __builtin_amdgcn_readfirstlane(s);
+      for (int off = 32; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+      stops[k] = __builtin_amdgcn_readfirstlane(m);
     }
     // pairs past every pixel's last contributor are not visited: their
     // partials keep an older tag and read as zeros
@@ -508,9 +579,9 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           const SplatRec* r = rec + slot;
           const float4* r4 = reinterpret_cast<const float4*>(r);
           const float4 a = r4[0], b = r4[1], c = r4[2];
-          s_a[j] = a;
-          s_b[j] = b;
-          s_c[j] = c.x;
+          s_rec[j].a = a;
+          s_rec[j].b = b;
+          s_rec[j].c = c.x;
           const int idx = (int)(e - begin);
           m = quad_hits(a.x, a.y, c.y, c.z, tq);
 #pragma unroll
@@ -536,86 +607,44 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           const int idx = (int)(lo + j - begin);
           const uint32_t m = (uint32_t)((qbits[h][0] >> bit) & 1u) | (uint32_t)((qbits[h][1] >> bit) & 1u) << 1 |
                              (uint32_t)((qbits[h][2] >> bit) & 1u) << 2 | (uint32_t)((qbits[h][3] >> bit) & 1u) << 3;
-          const float4 a = s_a[j];
-          const float4 b = s_b[j];
-          const float cblue = s_c[j];
-          // the accumulators start as zeros held in registers (the empty asm
-          // keeps the compiler from folding them into the first quadrant's
-          // adds, which would cost a copy of all eleven on every path that
-          // skips that quadrant); term 10, the pixels that shaped the splat,
-          // is counted in a scalar register from ballots
-          float acc[kVals];
-#pragma unroll
-          for (int i = 0; i < kVals - 1; ++i) {
-            acc[i] = 0.f;
-            asm volatile("" : "+v"(acc[i]));
+          const float4 a = s_rec[j].a;
+          const float4 b = s_rec[j].b;
+          const float cblue = s_rec[j].c;
+          // the accumulators start as zeros loaded from LDS (a broadcast read:
+          // no vector ALU instruction, and a value the compiler cannot fold
+          // into the first quadrant's adds, which would cost a copy of all ten
+          // on every path that skips that quadrant)
+          float acc[kAcc];
+          {
+            asm volatile("" ::: "memory");  // the loads stay here, once per splat
+            const f32x4 z0 = s_zero[0], z1 = s_zero[1];
+            const f32x2 z2 = s_zero2;
+            acc[0] = z0.x, acc[1] = z0.y, acc[2] = z0.z, acc[3] = z0.w;
+            acc[4] = z1.x, acc[5] = z1.y, acc[6] = z1.z, acc[7] = z1.w;
+            acc[8] = z2.x, acc[9] = z2.y;
           }
-          uint32_t shaped = 0u;
+          uint32_t shaped = 0u;  // pixels that shaped the splat, from ballots: a scalar
           bool any = false;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            if (!((m >> k) & 1u)) continue;  // wave-uniform
-            // the forward's decisions, recomputed with its exact float operations
-            const float dx = (k & 1 ? fx0 + 8.f : fx0) - a.x, dy = (k >> 1 ? fy0 + 8.f : fy0) - a.y;
-            const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-            const float G = falloff(power);
-            const float raw = b.y * G;
-            const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-            const bool contrib = idx < lastp[k] && !(power > 0.f) && !(alpha < p.min_alpha);
-            if (__ballot(contrib) == 0ull) continue;
-            any = true;
-            {
-            // every lane computes, contributing lanes keep: the engine's
-            // expressions with exact zeros and unchanged state elsewhere
-#pragma clang fp contract(fast)
-            // (algebraically the engine's expressions, SplatRaster.cpp:403-428,
-            // arranged for fused multiply-adds: alpha_c is alpha where this
-            // pixel contributes and 0 elsewhere, so a dropped step changes
-            // nothing; the suffix colour alpha c + (1 - alpha) suf is
-            // suf + alpha_c (c - suf), reusing dL/dalpha's differences)
-            const bool shape = contrib && raw < kMaxAlpha;
-            const float alpha_c = contrib ? alpha : 0.f;
-            const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-            const float Tn = T[k] * inv;
-            const float weight = alpha_c * Tn;
-            const float d0 = b.z - suf0[k], d1 = b.w - suf1[k], d2 = cblue - suf2[k];
-            const float dL_dalpha = (d0 * dL0[k] + d1 * dL1[k] + d2 * dL2[k]) * Tn - tb[k] * inv;
-            const float Gs = shape ? G : 0.f;
-            const float gd = Gs * dL_dalpha;  // dL/dopacity term
-            const float pw = b.y * gd;        // dL/dpower
-            const float px = pw * dx, py = pw * dy;
-            const float u = px * a.z + py * a.w;  // -dL/dmean_x
-            const float w = py * b.x + px * a.w;  // -dL/dmean_y
-            acc[0] += weight * dL0[k];
-            acc[1] += weight * dL1[k];
-            acc[2] += weight * dL2[k];
-            acc[3] += gd;
-            acc[4] += u;
-            acc[5] += w;
-            acc[6] += __builtin_amdgcn_sqrtf(u * u + w * w);  // densify statistic: hardware sqrt
-            acc[7] -= px * dx;  // halved after the wave sum
-            acc[8] -= px * dy;
-            acc[9] -= py * dy;  // halved after the wave sum
-            shaped += (uint32_t)__popcll(__ballot(shape));
-            T[k] = contrib ? Tn : T[k];
-            suf0[k] += alpha_c * d0;
-            suf1[k] += alpha_c * d1;
-            suf2[k] += alpha_c * d2;
-            }
-          }
+          if (m & 1u) any = adjoint_quad<0, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
+          if (m & 2u) any = adjoint_quad<1, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
+          if (m & 4u) any = adjoint_quad<2, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
+          if (m & 8u) any = adjoint_quad<3, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
           float z = 0.f;
           if (any) {
-            acc[10] = lane == 0 ? (float)shaped : 0.f;  // an exact integer through the wave sum
-            z = wave_sum11(acc, lane) * slot_scale;
+#pragma unroll
+            for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + red_base] = acc[i];
+            wave_sync_lds();
+            z = lds_quarter_sum(s_red, lane) * slot_scale;
+            z = my_slot == kAcc ? (float)shaped : z;  // an exact integer
+            wave_sync_lds();  // the next splat's writes follow these reads
           }
-          // the 12 lead lanes hold the splat's wave totals: one instruction
-          // covers the splat's contiguous record -- atomics (nonzero terms
-          // only) or the partial's 64-byte store (lanes 3, 19, 35, 51 add
-          // slots 12-15, so the store fills its whole memory sector)
+          // lanes 0, 4, .., 60 hold the splat's record: atomics (nonzero
+          // terms only) or the partial's 64-byte store (slots 11-15 the tag
+          // and zero padding, so the store fills its whole memory sector)
           const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
           if (deterministic) {
-            if (rec_lane && dsti < p_cap)
-              partial[(long long)dsti * kPartialStride + rec_slot] = rec_slot < kVals ? z : rec_pad;
+            if (slot_lane && dsti < p_cap)
+              partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
           } else if (any && writer && z != 0.f) {
             unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
           }
@@ -667,15 +696,15 @@ hipError_t forward_variant(const RasterParams& p, const ViewDesc* views, const u
   return hipGetLastError();
 }
 
-template <int kBB, bool kQueue>
+template <int kBB, bool kQueue, bool kLowCut>
 hipError_t backward_variant(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
                             const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
                             const int* last, float* screen, const uint32_t* pair_base, float* partial,
                             uint32_t p_cap, bool deterministic, const uint32_t* epoch, const uint32_t* order,
                             uint32_t* counter, int cus, hipStream_t stream) {
-  static const int per_cu = resident_blocks(k_backward<kBB, kQueue>, 64);
+  static const int per_cu = resident_blocks(k_backward<kBB, kQueue, kLowCut>, 64);
   const int blocks = kQueue ? std::min(p.total_tiles, per_cu * cus) : p.total_tiles;
-  hipLaunchKernelGGL((k_backward<kBB, kQueue>), dim3(blocks), dim3(64), 0, stream, p, views, ranges, pair_vals, rec,
+  hipLaunchKernelGGL((k_backward<kBB, kQueue, kLowCut>), dim3(blocks), dim3(64), 0, stream, p, views, ranges, pair_vals, rec,
                      dL_drgb, T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0, epoch, order,
                      counter);
   return hipGetLastError();
@@ -720,20 +749,23 @@ hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const
   if (p.total_tiles <= 0) return hipSuccess;
   if (cfg.bwd_queue) SPLAT_HIP_CHECK(launch_fill_u32(counter, 0u, 1, stream));
   const int pb = cfg.cus;
-  switch ((cfg.bwd_batch == 64 ? 0 : 2) + (cfg.bwd_queue ? 1 : 0)) {
-    case 0:
-      return backward_variant<64, false>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                         partial, p_cap, deterministic, epoch, order, counter, pb, stream);
-    case 1:
-      return backward_variant<64, true>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                        partial, p_cap, deterministic, epoch, order, counter, pb, stream);
-    case 2:
-      return backward_variant<128, false>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                          partial, p_cap, deterministic, epoch, order, counter, pb, stream);
-    default:
-      return backward_variant<128, true>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base,
-                                         partial, p_cap, deterministic, epoch, order, counter, pb, stream);
+  // the falloff's power <= -80 test is needed only for min_alpha below the
+  // alpha of e^-80 (see adjoint_quad)
+  const bool low_cut = !(p.min_alpha >= 1e-30f);
+#define SPLAT_BWD(BB, Q, LC)                                                                                     \
+  backward_variant<BB, Q, LC>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base, partial, \
+                              p_cap, deterministic, epoch, order, counter, pb, stream)
+  switch ((low_cut ? 4 : 0) + (cfg.bwd_batch == 64 ? 0 : 2) + (cfg.bwd_queue ? 1 : 0)) {
+    case 0: return SPLAT_BWD(64, false, false);
+    case 1: return SPLAT_BWD(64, true, false);
+    case 2: return SPLAT_BWD(128, false, false);
+    case 3: return SPLAT_BWD(128, true, false);
+    case 4: return SPLAT_BWD(64, false, true);
+    case 5: return SPLAT_BWD(64, true, true);
+    case 6: return SPLAT_BWD(128, false, true);
+    default: return SPLAT_BWD(128, true, true);
   }
+#undef SPLAT_BWD
 }
 
 }  // namespace splat
