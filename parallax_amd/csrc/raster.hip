@@ -57,6 +57,39 @@ __device__ __forceinline__ float falloff(float power) {
   return power <= -80.0f ? 0.0f : e;
 }
 
+// The falloff without its power <= -80 test: for power <= -80 it gives
+// e^-80 where falloff gives 0. Either way alpha = opacity * G <= 1.9e-35, below
+// any min_alpha >= 1e-30, so a compositing step that goes on to test
+// alpha < min_alpha skips the splat at that pixel with either value, and G is
+// read nowhere else. With smaller min_alpha the loops use falloff itself.
+__device__ __forceinline__ float falloff_above_cut(float power) {
+  return splat_det::exp_core(__builtin_amdgcn_fmed3f(power, -80.0f, 0.0f));
+}
+
+// Lane masks (64-bit, in scalar registers) of per-lane float and integer
+// compares, and selects by such a mask. The compositing loops combine their
+// decisions as masks: a ballot of a combined condition, or a select on one,
+// would copy the mask through a vector register first.
+constexpr int kFcmpOlt = 4;   // ordered less than
+constexpr int kFcmpUge = 11;  // unordered or greater or equal: !(a < b)
+constexpr int kFcmpUle = 13;  // unordered or less or equal: !(a > b)
+constexpr int kIcmpSlt = 40;  // signed less than
+__device__ __forceinline__ float select_lanes(uint64_t lanes, float if_set, float if_clear) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_clear), "v"(if_set), "s"(lanes));
+  return r;
+}
+__device__ __forceinline__ int select_lanes(uint64_t lanes, int if_set, int if_clear) {
+  int r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_clear), "v"(if_set), "s"(lanes));
+  return r;
+}
+__device__ __forceinline__ float keep_lanes(uint64_t lanes, float x) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(x), "s"(lanes));
+  return r;
+}
+
 constexpr int kChunk = 64;    // forward: splats fetched per wave step
 
 __device__ __forceinline__ int find_view(const ViewDesc* views, int n_views, int tile) {
@@ -160,14 +193,25 @@ __device__ __forceinline__ uint32_t quad_hits(float mx, float my, float ex, floa
   return (uint32_t)(r0 && c0) | (uint32_t)(r0 && c1) << 1 | (uint32_t)(r1 && c0) << 2 | (uint32_t)(r1 && c1) << 3;
 }
 
+// A staged list entry of the forward: the compositing fields of its record
+// and its list position + 1 (one 48-byte slot, so the loads share one address).
+struct alignas(16) FwdStaged {
+  float4 a;  // mx, my, conic a, conic b
+  float4 b;  // conic c, opacity, r, g
+  float c;   // b
+  int pos;   // list position + 1
+};
+
 // One wave's forward over one 8x8 quadrant (0..3, row-major) of one tile. The
-// LDS slices are the wave's own.
+// LDS slice is the wave's own. kLowCut: the falloff with its power <= -80
+// test (min_alpha below 1e-30).
+template <bool kLowCut>
 __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDesc* __restrict__ views,
                                              const uint32_t* __restrict__ ranges,
                                              const uint32_t* __restrict__ pair_vals,
                                              const SplatRec* __restrict__ rec, float* __restrict__ rgb,
                                              float* __restrict__ T_out, int* __restrict__ last_out, int tile,
-                                             int quadrant, float4* s_a, float4* s_b, float2* s_c) {
+                                             int quadrant, FwdStaged* s_st) {
   const int lane = threadIdx.x & 63;
   const int v = find_view(views, p.n_views, tile);
   const ViewDesc& V = views[v];
@@ -181,9 +225,10 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
   const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
   float T = 1.f, acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
   int last = 0;
-  bool done = !inside;
+  // terminated pixels as a lane mask (pixels outside the image start there)
+  uint64_t done = __ballot(!inside);
   for (uint32_t base = begin; base < end; base += kChunk) {
-    if (__ballot(!done) == 0ull) break;
+    if (done == ~0ull) break;
     const uint32_t e = base + lane;
     bool hit = false;
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
@@ -198,9 +243,10 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
     const uint64_t m = __ballot(hit);
     if (hit) {
       const int pos = lane_rank(m);
-      s_a[pos] = a;
-      s_b[pos] = b;
-      s_c[pos] = make_float2(c.x, __int_as_float((int)(e - begin) + 1));  // list position + 1
+      s_st[pos].a = a;
+      s_st[pos].b = b;
+      s_st[pos].c = c.x;
+      s_st[pos].pos = (int)(e - begin) + 1;
     }
     wave_sync_lds();
     const int cnt = __popcll(m);
@@ -209,24 +255,26 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
     // power <= 0, alpha >= min_alpha); a kept step is the same float
     // operations on the same operands, a dropped one adds an exact zero.
     for (int j = 0; j < cnt; ++j) {
-      const float4 sa = s_a[j];
-      const float4 sb = s_b[j];
-      const float2 sc = s_c[j];
+      const float4 sa = s_st[j].a;
+      const float4 sb = s_st[j].b;
+      const float sc = s_st[j].c;
+      const int spos = s_st[j].pos;
       const float dx = fx - sa.x, dy = fy - sa.y;
       const float power = -0.5f * (sa.z * dx * dx + sb.x * dy * dy) - sa.w * dx * dy;
-      const float raw = sb.y * falloff(power);
+      const float raw = sb.y * (kLowCut ? falloff(power) : falloff_above_cut(power));
       const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
-      const bool use = !done & !(power > 0.f) & !(alpha < p.min_alpha);
-      const float w = use ? T * alpha : 0.f;
+      const uint64_t use = ~done & __builtin_amdgcn_fcmpf(power, 0.f, kFcmpUle) &
+                           __builtin_amdgcn_fcmpf(alpha, p.min_alpha, kFcmpUge);
+      const float w = keep_lanes(use, T * alpha);
       acc0 += w * sb.z;
       acc1 += w * sb.w;
-      acc2 += w * sc.x;
-      T = use ? T * (1.f - alpha) : T;
-      last = use ? __float_as_int(sc.y) : last;
-      done = done | (use & (T < p.cutoff));
-      // leave once all the wave's pixels have terminated (the flags are lane
-      // masks, so the test is a scalar compare after every splat)
-      if (__ballot(!done) == 0ull) break;
+      acc2 += w * sc;
+      T = select_lanes(use, T * (1.f - alpha), T);
+      last = select_lanes(use, spos, last);
+      done |= use & __builtin_amdgcn_fcmpf(T, p.cutoff, kFcmpOlt);
+      // leave once all the wave's pixels have terminated (a scalar compare
+      // of the lane mask after every splat)
+      if (done == ~0ull) break;
     }
     wave_sync_lds();
   }
@@ -296,7 +344,7 @@ struct FwdShape<2> {
 // contiguous runs. kQueue true: a persistent grid; each workgroup takes the
 // next unit from a counter over `order` (tiles, longest list first), so the
 // longest tiles start first and the launch ends on short ones.
-template <int kShape, bool kQueue>
+template <int kShape, bool kQueue, bool kLowCut>
 __global__ __launch_bounds__(FwdShape<kShape>::kThreads) void k_forward(
     RasterParams p, const ViewDesc* __restrict__ views, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ pair_vals, const SplatRec* __restrict__ rec, float* __restrict__ rgb,
@@ -304,9 +352,7 @@ __global__ __launch_bounds__(FwdShape<kShape>::kThreads) void k_forward(
     uint32_t* __restrict__ counter) {
   using Sh = FwdShape<kShape>;
   constexpr int kWaves = Sh::kThreads / 64;
-  __shared__ float4 s_a[kWaves][kChunk];  // mx, my, ca, cb
-  __shared__ float4 s_b[kWaves][kChunk];  // cc, opacity, r, g
-  __shared__ float2 s_c[kWaves][kChunk];  // b, list index (int bits)
+  __shared__ FwdStaged s_st[kWaves][kChunk];
   __shared__ int s_unit;
   const int wave = threadIdx.x >> 6;
   const int n_units = Sh::units(p.total_tiles);
@@ -314,8 +360,7 @@ __global__ __launch_bounds__(FwdShape<kShape>::kThreads) void k_forward(
     const int unit = xcd_remap(blockIdx.x, n_units);
     int tile, quadrant;
     if (Sh::wave_work(views, p.n_views, unit, wave, &tile, &quadrant))
-      forward_quad(p, views, ranges, pair_vals, rec, rgb, T_out, last_out, tile, quadrant, s_a[wave], s_b[wave],
-                   s_c[wave]);
+      forward_quad<kLowCut>(p, views, ranges, pair_vals, rec, rgb, T_out, last_out, tile, quadrant, s_st[wave]);
     return;
   }
   for (;;) {
@@ -329,8 +374,7 @@ __global__ __launch_bounds__(FwdShape<kShape>::kThreads) void k_forward(
     const int unit = (int)order[idx / per] * per + idx % per;
     int tile, quadrant;
     if (Sh::wave_work(views, p.n_views, unit, wave, &tile, &quadrant))
-      forward_quad(p, views, ranges, pair_vals, rec, rgb, T_out, last_out, tile, quadrant, s_a[wave], s_b[wave],
-                   s_c[wave]);
+      forward_quad<kLowCut>(p, views, ranges, pair_vals, rec, rgb, T_out, last_out, tile, quadrant, s_st[wave]);
   }
 }
 
@@ -385,44 +429,20 @@ struct AdjPixels {
   int lastp[4];
 };
 
-// Lane masks (64-bit, in scalar registers) of per-lane float and integer
-// compares, and selects by such a mask. The adjoint combines its decisions as
-// masks: a ballot of a combined condition, or a select on one, would copy the
-// mask through a vector register first.
-constexpr int kFcmpOlt = 4;   // ordered less than
-constexpr int kFcmpUge = 11;  // unordered or greater or equal: !(a < b)
-constexpr int kFcmpUle = 13;  // unordered or less or equal: !(a > b)
-constexpr int kIcmpSlt = 40;  // signed less than
-__device__ __forceinline__ float select_lanes(uint64_t lanes, float if_set, float if_clear) {
-  float r;
-  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_clear), "v"(if_set), "s"(lanes));
-  return r;
-}
-__device__ __forceinline__ float keep_lanes(uint64_t lanes, float x) {
-  float r;
-  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(x), "s"(lanes));
-  return r;
-}
-
 // Splat (a, b, cblue) at list index idx over quadrant k's pixels: the forward's
 // decisions recomputed with its exact float operations, then the gradient
 // terms added into acc. Every lane computes and non-contributing lanes add
 // exact zeros; a quadrant with no contributing lane returns early. Returns
 // whether any lane contributed; adds the pixels that shaped the splat (alpha
 // below the clamp) to `shaped`.
-//
-// kLowCut false: the falloff's exponent is clamped to [-80, 0] with no test of
-// power <= -80. The forward sets G = 0 there; here G = e^-80 instead, whose
-// alpha (<= 1.9e-35) is below any min_alpha >= 1e-30, so the pixel does not
-// contribute either way and, G being read only where it does, nothing else
-// changes. kLowCut true keeps the forward's test, for smaller min_alpha.
+// kLowCut: the falloff with its power <= -80 test (min_alpha below 1e-30).
 template <int k, bool kLowCut>
 __device__ __forceinline__ bool adjoint_quad(float (&acc)[kAcc], uint32_t& shaped, AdjPixels& s, const float4& a,
                                              const float4& b, float cblue, int idx, float fx0, float fy0,
                                              float min_alpha) {
   const float dx = (k & 1 ? fx0 + 8.f : fx0) - a.x, dy = (k >> 1 ? fy0 + 8.f : fy0) - a.y;
   const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-  const float G = kLowCut ? falloff(power) : splat_det::exp_core(__builtin_amdgcn_fmed3f(power, -80.0f, 0.0f));
+  const float G = kLowCut ? falloff(power) : falloff_above_cut(power);
   const float raw = b.y * G;
   const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
   // the pixels that contribute: idx < last, !(power > 0), !(alpha < min_alpha)
@@ -683,15 +703,15 @@ int resident_blocks(K kernel, int threads) {
   return n;
 }
 
-template <int kShape, bool kQueue>
+template <int kShape, bool kQueue, bool kLowCut>
 hipError_t forward_variant(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
                            const uint32_t* pair_vals, const SplatRec* rec, float* rgb, float* T, int* last,
                            const uint32_t* order, uint32_t* counter, int cus, hipStream_t stream) {
   constexpr int kThreads = kShape == 1 ? 64 : 256;
   const int units = forward_units<kShape>(p.total_tiles);
-  static const int per_cu = resident_blocks(k_forward<kShape, kQueue>, kThreads);
+  static const int per_cu = resident_blocks(k_forward<kShape, kQueue, kLowCut>, kThreads);
   const int blocks = kQueue ? std::min(units, per_cu * cus) : units;
-  hipLaunchKernelGGL((k_forward<kShape, kQueue>), dim3(blocks), dim3(kThreads), 0, stream, p, views, ranges,
+  hipLaunchKernelGGL((k_forward<kShape, kQueue, kLowCut>), dim3(blocks), dim3(kThreads), 0, stream, p, views, ranges,
                      pair_vals, rec, rgb, T, last, order, counter);
   return hipGetLastError();
 }
@@ -730,15 +750,27 @@ hipError_t launch_forward(const RasterParams& p, const RasterConfig& cfg, const 
   if (p.total_tiles <= 0) return hipSuccess;
   if (cfg.fwd_queue) SPLAT_HIP_CHECK(launch_fill_u32(counter, 0u, 1, stream));
   const int pb = cfg.cus;
-  switch (cfg.fwd_shape * 2 + (cfg.fwd_queue ? 1 : 0)) {
-    case 0: return forward_variant<0, false>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
-    case 1: return forward_variant<0, true>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
-    case 2: return forward_variant<1, false>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
-    case 3: return forward_variant<1, true>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
-    case 4: return forward_variant<2, false>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
-    case 5: return forward_variant<2, true>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream);
+  // the falloff's power <= -80 test is needed only for min_alpha below the
+  // alpha of e^-80 (falloff_above_cut)
+  const bool low_cut = !(p.min_alpha >= 1e-30f);
+#define SPLAT_FWD(SH, Q, LC) \
+  forward_variant<SH, Q, LC>(p, views, ranges, pair_vals, rec, rgb, T, last, order, counter, pb, stream)
+  switch ((low_cut ? 6 : 0) + cfg.fwd_shape * 2 + (cfg.fwd_queue ? 1 : 0)) {
+    case 0: return SPLAT_FWD(0, false, false);
+    case 1: return SPLAT_FWD(0, true, false);
+    case 2: return SPLAT_FWD(1, false, false);
+    case 3: return SPLAT_FWD(1, true, false);
+    case 4: return SPLAT_FWD(2, false, false);
+    case 5: return SPLAT_FWD(2, true, false);
+    case 6: return SPLAT_FWD(0, false, true);
+    case 7: return SPLAT_FWD(0, true, true);
+    case 8: return SPLAT_FWD(1, false, true);
+    case 9: return SPLAT_FWD(1, true, true);
+    case 10: return SPLAT_FWD(2, false, true);
+    case 11: return SPLAT_FWD(2, true, true);
     default: return hipErrorInvalidValue;
   }
+#undef SPLAT_FWD
 }
 
 hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
