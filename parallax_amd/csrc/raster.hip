@@ -385,23 +385,28 @@ __device__ __forceinline__ float dpp(float v) {
 }
 
 // The adjoint's per-splat reduction goes through LDS: each lane writes its 10
-// accumulated terms into a [term][lane] array, then lanes 4i..4i+3 each read
+// accumulated terms into a [term][lane] array, then lanes 4i..4i+3 each load
 // a quarter of term i's row (16 lanes, four 16-byte loads), sum it in a fixed
 // tree, and combine the four quarters with two quad DPP adds (commutative, so
 // all four lanes hold the same bits). Lane 4i then holds term i's wave total.
-// The array is laid out for the LDS banks: element (term i, lane l) at
-// i * kRedPitch + (l / 16) * kRedGroup + l % 16, which keeps the quarter-row
-// reads free of bank conflicts.
+// Rows are 68 floats apart, which keeps both the row writes and the
+// quarter-row loads free of LDS bank conflicts. The loads of one splat are
+// summed only after the next splat's pixels have been evaluated, which hides
+// their latency.
 constexpr int kAcc = 10;       // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3
-constexpr int kRedGroup = 20;  // floats per 16-lane group in a row
-constexpr int kRedPitch = 80;  // floats per term row
+constexpr int kRedPitch = 68;  // floats per term row
 
-__device__ __forceinline__ float lds_quarter_sum(const float* s_red, int lane) {
-  const int i = min(lane >> 2, kAcc - 1);  // lanes 40..63 repeat term 9's reads; their sums are not used
-  const float4* src = reinterpret_cast<const float4*>(s_red + i * kRedPitch + (lane & 3) * kRedGroup);
-  const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
-  const float t0 = (q0.x + q0.y) + (q0.z + q0.w), t1 = (q1.x + q1.y) + (q1.z + q1.w);
-  const float t2 = (q2.x + q2.y) + (q2.z + q2.w), t3 = (q3.x + q3.y) + (q3.z + q3.w);
+struct Quarter {
+  float4 q0, q1, q2, q3;
+};
+__device__ __forceinline__ Quarter lds_quarter_load(const float* s_red, int lane) {
+  const int i = min(lane >> 2, kAcc - 1);  // lanes 40..63 repeat term 9's loads; their sums are not used
+  const float4* src = reinterpret_cast<const float4*>(s_red + i * kRedPitch + (lane & 3) * 16);
+  return Quarter{src[0], src[1], src[2], src[3]};
+}
+__device__ __forceinline__ float quarter_sum(const Quarter& r) {
+  const float t0 = (r.q0.x + r.q0.y) + (r.q0.z + r.q0.w), t1 = (r.q1.x + r.q1.y) + (r.q1.z + r.q1.w);
+  const float t2 = (r.q2.x + r.q2.y) + (r.q2.z + r.q2.w), t3 = (r.q3.x + r.q3.y) + (r.q3.z + r.q3.w);
   float t = (t0 + t1) + (t2 + t3);
   t = t + dpp<0xB1>(t);  // quad_perm [1,0,3,2]
   t = t + dpp<0x4E>(t);  // quad_perm [2,3,0,1]
@@ -533,7 +538,6 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
   const float slot_scale = (my_slot == 7 || my_slot == 9) ? 0.5f : 1.f;
   // deterministic records: slots 0-10 the terms, 11 this pass's tag, 12-15 zero
   const float rec_pad = my_slot == 11 ? __uint_as_float(deterministic ? *epoch : 0u) : 0.f;
-  const int red_base = (lane >> 4) * kRedGroup + (lane & 15);
   for (int iter = 0;; ++iter) {
     int tile;
     if (kQueue) {
@@ -581,6 +585,26 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
     // pairs past every pixel's last contributor are not visited: their
     // partials keep an older tag and read as zeros
     const uint32_t stop = begin + (uint32_t)max(max(stops[0], stops[1]), max(stops[2], stops[3]));
+    // a splat's result goes out one splat later (pend_*): its totals in lanes
+    // 0, 4, .., 40, then atomics (nonzero terms only) or the partial's 64-byte
+    // store from lanes 0, 4, .., 60 (slots 11-15 the tag and zero padding, so
+    // the store fills its whole memory sector)
+    bool pend = false, pend_any = false;
+    uint32_t pend_dst = 0u, pend_shaped = 0u;
+    Quarter red;
+    auto finish = [&](bool any, uint32_t dsti, uint32_t shaped, const Quarter& r) {
+      float z = 0.f;
+      if (any) {
+        z = quarter_sum(r) * slot_scale;
+        z = my_slot == kAcc ? (float)shaped : z;  // an exact integer
+      }
+      if (deterministic) {
+        if (slot_lane && dsti < p_cap)
+          partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
+      } else if (any && writer && z != 0.f) {
+        unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
+      }
+    };
     for (uint32_t hi = stop; hi > begin;) {
       const uint32_t lo = hi - min((uint32_t)kBB, hi - begin);
       const int cnt = (int)(hi - lo);
@@ -649,30 +673,26 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           if (m & 2u) any = adjoint_quad<1, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
           if (m & 4u) any = adjoint_quad<2, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
           if (m & 8u) any = adjoint_quad<3, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
-          float z = 0.f;
+          // the previous splat's totals (its loads have landed meanwhile),
+          // then this splat's terms into LDS and its loads
+          if (pend) finish(pend_any, pend_dst, pend_shaped, red);
           if (any) {
+            wave_sync_lds();  // these writes follow the previous loads
 #pragma unroll
-            for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + red_base] = acc[i];
+            for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + lane] = acc[i];
             wave_sync_lds();
-            z = lds_quarter_sum(s_red, lane) * slot_scale;
-            z = my_slot == kAcc ? (float)shaped : z;  // an exact integer
-            wave_sync_lds();  // the next splat's writes follow these reads
+            red = lds_quarter_load(s_red, lane);
           }
-          // lanes 0, 4, .., 60 hold the splat's record: atomics (nonzero
-          // terms only) or the partial's 64-byte store (slots 11-15 the tag
-          // and zero padding, so the store fills its whole memory sector)
-          const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
-          if (deterministic) {
-            if (slot_lane && dsti < p_cap)
-              partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
-          } else if (any && writer && z != 0.f) {
-            unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
-          }
+          pend = true;
+          pend_any = any;
+          pend_dst = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
+          pend_shaped = shaped;
         }
       }
       wave_sync_lds();  // the next chunk overwrites the staging
       hi = lo;
     }
+    if (pend) finish(pend_any, pend_dst, pend_shaped, red);
   }
 }
 
