@@ -62,8 +62,13 @@ __device__ __forceinline__ float falloff(float power) {
 // any min_alpha >= 1e-30, so a compositing step that goes on to test
 // alpha < min_alpha skips the splat at that pixel with either value, and G is
 // read nowhere else. With smaller min_alpha the loops use falloff itself.
+//
+// Nor is the clamp at 0 needed: a step that keeps the splat has power <= 0,
+// where max(power, -80) is the two-sided clamp (NaN power, too, becomes -80,
+// as with med3). Where power > 0 the pixel is skipped and G is garbage that
+// every use masks out by a select (never a multiply by zero).
 __device__ __forceinline__ float falloff_above_cut(float power) {
-  return splat_det::exp_core(__builtin_amdgcn_fmed3f(power, -80.0f, 0.0f));
+  return splat_det::exp_core(fmaxf(power, -80.0f));
 }
 
 // Lane masks (64-bit, in scalar registers) of per-lane float and integer
@@ -441,12 +446,23 @@ struct AdjPixels {
 // whether any lane contributed; adds the pixels that shaped the splat (alpha
 // below the clamp) to `shaped`.
 // kLowCut: the falloff with its power <= -80 test (min_alpha below 1e-30).
+//
+// The falloff's exponent -0.5 (a dx^2 + c dy^2) - b dx dy is evaluated with
+// the forward's operations: a dx dx and b dx depend on the pixel's column
+// only and c dy dy on its row, and a lane's four pixels share two columns and
+// two rows, so those products come in per splat (SplatCols, SplatRows).
+struct SplatCols {
+  float dx[2], axx[2], bx[2];  // per pixel column of the lane: dx, (a dx) dx, b dx
+};
+struct SplatRows {
+  float dy[2], cyy[2];  // per pixel row of the lane: dy, (c dy) dy
+};
 template <int k, bool kLowCut>
 __device__ __forceinline__ bool adjoint_quad(float (&acc)[kAcc], uint32_t& shaped, AdjPixels& s, const float4& a,
-                                             const float4& b, float cblue, int idx, float fx0, float fy0,
-                                             float min_alpha) {
-  const float dx = (k & 1 ? fx0 + 8.f : fx0) - a.x, dy = (k >> 1 ? fy0 + 8.f : fy0) - a.y;
-  const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                                             const float4& b, float cblue, int idx, const SplatCols& X,
+                                             const SplatRows& Y, float min_alpha) {
+  const float dx = X.dx[k & 1], dy = Y.dy[k >> 1];
+  const float power = -0.5f * (X.axx[k & 1] + Y.cyy[k >> 1]) - X.bx[k & 1] * dy;
   const float G = kLowCut ? falloff(power) : falloff_above_cut(power);
   const float raw = b.y * G;
   const float alpha = raw < kMaxAlpha ? raw : kMaxAlpha;
@@ -492,6 +508,17 @@ __device__ __forceinline__ bool adjoint_quad(float (&acc)[kAcc], uint32_t& shape
   s.suf2[k] += alpha_c * d2;
   }
   return true;
+}
+
+// acc = 0 from the LDS zero block (volatile-free loads kept in place by an
+// empty asm that clobbers memory, so they are neither hoisted nor merged).
+__device__ __forceinline__ void load_zeros(float (&acc)[kAcc], const f32x4* s_zero, const f32x2& s_zero2) {
+  asm volatile("" ::: "memory");
+  const f32x4 z0 = s_zero[0], z1 = s_zero[1];
+  const f32x2 z2 = s_zero2;
+  acc[0] = z0.x, acc[1] = z0.y, acc[2] = z0.z, acc[3] = z0.w;
+  acc[4] = z1.x, acc[5] = z1.y, acc[6] = z1.z, acc[7] = z1.w;
+  acc[8] = z2.x, acc[9] = z2.y;
 }
 
 // The adjoint of one 16x16 tile by one wave: each lane owns four pixels, one
@@ -592,6 +619,12 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
     bool pend = false, pend_any = false;
     uint32_t pend_dst = 0u, pend_shaped = 0u;
     Quarter red;
+    // the accumulators start as zeros loaded from LDS (a broadcast read: no
+    // vector ALU instruction, and a value the compiler cannot fold into the
+    // first quadrant's adds, which would cost a copy of all ten on every path
+    // that skips that quadrant)
+    float acc[kAcc];
+    load_zeros(acc, s_zero, s_zero2);
     auto finish = [&](bool any, uint32_t dsti, uint32_t shaped, const Quarter& r) {
       float z = 0.f;
       if (any) {
@@ -651,38 +684,40 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           const int idx = (int)(lo + j - begin);
           const uint32_t m = (uint32_t)((qbits[h][0] >> bit) & 1u) | (uint32_t)((qbits[h][1] >> bit) & 1u) << 1 |
                              (uint32_t)((qbits[h][2] >> bit) & 1u) << 2 | (uint32_t)((qbits[h][3] >> bit) & 1u) << 3;
+          // LDS loads in the order their waits need: this splat's record, then
+          // the previous splat's quarter rows (its terms went to LDS at the end
+          // of the previous step), so waiting for the record does not wait
+          // for them; they land while this splat's pixels are evaluated
           const float4 a = s_rec[j].a;
           const float4 b = s_rec[j].b;
           const float cblue = s_rec[j].c;
-          // the accumulators start as zeros loaded from LDS (a broadcast read:
-          // no vector ALU instruction, and a value the compiler cannot fold
-          // into the first quadrant's adds, which would cost a copy of all ten
-          // on every path that skips that quadrant)
-          float acc[kAcc];
-          {
-            asm volatile("" ::: "memory");  // the loads stay here, once per splat
-            const f32x4 z0 = s_zero[0], z1 = s_zero[1];
-            const f32x2 z2 = s_zero2;
-            acc[0] = z0.x, acc[1] = z0.y, acc[2] = z0.z, acc[3] = z0.w;
-            acc[4] = z1.x, acc[5] = z1.y, acc[6] = z1.z, acc[7] = z1.w;
-            acc[8] = z2.x, acc[9] = z2.y;
+          red = lds_quarter_load(s_red, lane);
+          SplatCols X;
+          SplatRows Y;
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            X.dx[c] = (c ? fx0 + 8.f : fx0) - a.x;
+            X.axx[c] = a.z * X.dx[c] * X.dx[c];
+            X.bx[c] = a.w * X.dx[c];
+            Y.dy[c] = (c ? fy0 + 8.f : fy0) - a.y;
+            Y.cyy[c] = b.x * Y.dy[c] * Y.dy[c];
           }
           uint32_t shaped = 0u;  // pixels that shaped the splat, from ballots: a scalar
           bool any = false;
-          if (m & 1u) any = adjoint_quad<0, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
-          if (m & 2u) any = adjoint_quad<1, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
-          if (m & 4u) any = adjoint_quad<2, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
-          if (m & 8u) any = adjoint_quad<3, kLowCut>(acc, shaped, s, a, b, cblue, idx, fx0, fy0, p.min_alpha) | any;
-          // the previous splat's totals (its loads have landed meanwhile),
-          // then this splat's terms into LDS and its loads
+          if (m & 1u) any = adjoint_quad<0, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
+          if (m & 2u) any = adjoint_quad<1, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
+          if (m & 4u) any = adjoint_quad<2, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
+          if (m & 8u) any = adjoint_quad<3, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
+          // the previous splat's totals, then this splat's terms into LDS and
+          // the accumulators back to zero. The LDS traffic is the same on
+          // every path (whether or not a splat added anything), so the
+          // compiler's waits for it need not assume the worst path.
           if (pend) finish(pend_any, pend_dst, pend_shaped, red);
-          if (any) {
-            wave_sync_lds();  // these writes follow the previous loads
+          wave_sync_lds();  // these writes follow the previous loads
 #pragma unroll
-            for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + lane] = acc[i];
-            wave_sync_lds();
-            red = lds_quarter_load(s_red, lane);
-          }
+          for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + lane] = acc[i];
+          wave_sync_lds();
+          load_zeros(acc, s_zero, s_zero2);
           pend = true;
           pend_any = any;
           pend_dst = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
@@ -692,7 +727,10 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
       wave_sync_lds();  // the next chunk overwrites the staging
       hi = lo;
     }
-    if (pend) finish(pend_any, pend_dst, pend_shaped, red);
+    if (pend) {
+      red = lds_quarter_load(s_red, lane);
+      finish(pend_any, pend_dst, pend_shaped, red);
+    }
   }
 }
 
