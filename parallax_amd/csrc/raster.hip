@@ -395,9 +395,7 @@ __device__ __forceinline__ float dpp(float v) {
 // tree, and combine the four quarters with two quad DPP adds (commutative, so
 // all four lanes hold the same bits). Lane 4i then holds term i's wave total.
 // Rows are 68 floats apart, which keeps both the row writes and the
-// quarter-row loads free of LDS bank conflicts. The loads of one splat are
-// summed only after the next splat's pixels have been evaluated, which hides
-// their latency.
+// quarter-row loads free of LDS bank conflicts.
 constexpr int kAcc = 10;       // dcolour 3, dopacity, dmean 2, |dmean|, dconic 3
 constexpr int kRedPitch = 68;  // floats per term row
 
@@ -533,8 +531,13 @@ __device__ __forceinline__ void load_zeros(float (&acc)[kAcc], const f32x4* s_ze
 // and lanes 0, 4, .., 40 issue the splat's result: up to 11 hardware float
 // atomics (atomic mode), or 16 lanes one 64-byte partial store
 // (deterministic mode, summation order fixed by the code alone).
+//
+// Five waves per SIMD (at most 96 VGPRs, one value spilled outside the splat
+// loop): the per-pixel chains are long and dependent, and the fifth wave
+// hides more of their latency than the registers a fourth-wave budget would
+// allow (a pipelined LDS reduction, 118 VGPRs at four waves, was 4% slower).
 template <int kBB, bool kQueue, bool kLowCut>
-__global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
+__global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
                                                  const uint32_t* __restrict__ ranges,
                                                  const uint32_t* __restrict__ pair_vals,
                                                  const SplatRec* __restrict__ rec, const float* __restrict__ dL_drgb,
@@ -612,32 +615,12 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
     // pairs past every pixel's last contributor are not visited: their
     // partials keep an older tag and read as zeros
     const uint32_t stop = begin + (uint32_t)max(max(stops[0], stops[1]), max(stops[2], stops[3]));
-    // a splat's result goes out one splat later (pend_*): its totals in lanes
-    // 0, 4, .., 40, then atomics (nonzero terms only) or the partial's 64-byte
-    // store from lanes 0, 4, .., 60 (slots 11-15 the tag and zero padding, so
-    // the store fills its whole memory sector)
-    bool pend = false, pend_any = false;
-    uint32_t pend_dst = 0u, pend_shaped = 0u;
-    Quarter red;
     // the accumulators start as zeros loaded from LDS (a broadcast read: no
     // vector ALU instruction, and a value the compiler cannot fold into the
     // first quadrant's adds, which would cost a copy of all ten on every path
     // that skips that quadrant)
     float acc[kAcc];
     load_zeros(acc, s_zero, s_zero2);
-    auto finish = [&](bool any, uint32_t dsti, uint32_t shaped, const Quarter& r) {
-      float z = 0.f;
-      if (any) {
-        z = quarter_sum(r) * slot_scale;
-        z = my_slot == kAcc ? (float)shaped : z;  // an exact integer
-      }
-      if (deterministic) {
-        if (slot_lane && dsti < p_cap)
-          partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
-      } else if (any && writer && z != 0.f) {
-        unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
-      }
-    };
     for (uint32_t hi = stop; hi > begin;) {
       const uint32_t lo = hi - min((uint32_t)kBB, hi - begin);
       const int cnt = (int)(hi - lo);
@@ -684,14 +667,9 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           const int idx = (int)(lo + j - begin);
           const uint32_t m = (uint32_t)((qbits[h][0] >> bit) & 1u) | (uint32_t)((qbits[h][1] >> bit) & 1u) << 1 |
                              (uint32_t)((qbits[h][2] >> bit) & 1u) << 2 | (uint32_t)((qbits[h][3] >> bit) & 1u) << 3;
-          // LDS loads in the order their waits need: this splat's record, then
-          // the previous splat's quarter rows (its terms went to LDS at the end
-          // of the previous step), so waiting for the record does not wait
-          // for them; they land while this splat's pixels are evaluated
           const float4 a = s_rec[j].a;
           const float4 b = s_rec[j].b;
           const float cblue = s_rec[j].c;
-          red = lds_quarter_load(s_red, lane);
           SplatCols X;
           SplatRows Y;
 #pragma unroll
@@ -708,28 +686,31 @@ __global__ __launch_bounds__(64) void k_backward(RasterParams p, const ViewDesc*
           if (m & 2u) any = adjoint_quad<1, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
           if (m & 4u) any = adjoint_quad<2, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
           if (m & 8u) any = adjoint_quad<3, kLowCut>(acc, shaped, s, a, b, cblue, idx, X, Y, p.min_alpha) | any;
-          // the previous splat's totals, then this splat's terms into LDS and
-          // the accumulators back to zero. The LDS traffic is the same on
-          // every path (whether or not a splat added anything), so the
-          // compiler's waits for it need not assume the worst path.
-          if (pend) finish(pend_any, pend_dst, pend_shaped, red);
-          wave_sync_lds();  // these writes follow the previous loads
+          float z = 0.f;
+          if (any) {
 #pragma unroll
-          for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + lane] = acc[i];
-          wave_sync_lds();
-          load_zeros(acc, s_zero, s_zero2);
-          pend = true;
-          pend_any = any;
-          pend_dst = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
-          pend_shaped = shaped;
+            for (int i = 0; i < kAcc; ++i) s_red[i * kRedPitch + lane] = acc[i];
+            wave_sync_lds();
+            z = quarter_sum(lds_quarter_load(s_red, lane)) * slot_scale;
+            z = my_slot == kAcc ? (float)shaped : z;  // an exact integer
+            wave_sync_lds();  // the next splat's writes follow these loads
+            load_zeros(acc, s_zero, s_zero2);  // a splat that added nothing left them at zero
+          }
+          // lanes 0, 4, .., 40 hold the splat's totals: atomics (nonzero terms
+          // only), or the partial's 64-byte store from lanes 0, 4, .., 60
+          // (slots 11-15 the tag and zero padding, so the store fills its
+          // whole memory sector)
+          const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
+          if (deterministic) {
+            if (slot_lane && dsti < p_cap)
+              partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
+          } else if (any && writer && z != 0.f) {
+            unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
+          }
         }
       }
       wave_sync_lds();  // the next chunk overwrites the staging
       hi = lo;
-    }
-    if (pend) {
-      red = lds_quarter_load(s_red, lane);
-      finish(pend_any, pend_dst, pend_shaped, red);
     }
   }
 }
