@@ -680,11 +680,16 @@ splat_status enqueue_backward(splat_context* ctx, bool fuse_adam = false, const 
     fuse.v = ctx->v.f();
     fuse.stat_sg = ctx->stat_sg.f();
     fuse.stat_vc = ctx->stat_vc.i();
+    fuse.stored_clear = ctx->grads_clear;
     TIMED("adjoint_adam", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
                                          ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream, &fuse));
+    // the update leaves them at zero (a step the device skips on overflow is
+    // cleared by the host before its redo)
+    ctx->grads_clear = true;
   } else {
     TIMED("adjoint", launch_adjoint(ap, ctx->views_dev(), ctx->params.f(), ctx->recs(), ctx->screen.f(),
                                     ctx->grads.f(), ctx->sg.f(), ctx->vc.i(), ctx->stream));
+    ctx->grads_clear = false;
   }
   return SPLAT_OK;
 }
@@ -728,7 +733,9 @@ splat_status enqueue_adam(splat_context* ctx, const uint32_t* skip) {
 
 splat_status adam_impl(splat_context* ctx, const splat_adam_params* a) {
   if (splat_status s = stage_adam(ctx, a)) return s;
-  return enqueue_adam(ctx, nullptr);
+  if (splat_status s = enqueue_adam(ctx, nullptr)) return s;
+  ctx->grads_clear = true;  // Adam clears the gradients and folds sg / vc
+  return SPLAT_OK;
 }
 
 // One training step as a plain launch sequence (one host read of the pair
@@ -758,6 +765,8 @@ std::string graph_key(const splat_context* ctx, float w) {
   put(head, sizeof head);
   put(&ctx->opts, sizeof ctx->opts);
   put(&w, sizeof w);
+  const char clear = ctx->grads_clear ? 1 : 0;  // which fused update the graph holds
+  put(&clear, 1);
   const int cfg[4] = {ctx->rcfg.fwd_shape, ctx->rcfg.fwd_queue ? 1 : 0, ctx->rcfg.bwd_batch, ctx->rcfg.bwd_queue ? 1 : 0};
   put(cfg, sizeof cfg);
   for (const ViewDesc& d : ctx->batch_desc) {
@@ -865,6 +874,7 @@ splat_status train_step_graph(splat_context* ctx, int n_views, const int32_t* vi
     CK(launch_fill_f32(ctx->grads.f(), 0.f, ctx->cap * (14 + ctx->rs), ctx->stream), "clear");
     CK(launch_fill_f32(ctx->sg.f(), 0.f, ctx->cap, ctx->stream), "clear");
     CK(launch_fill_u32(ctx->vc.u(), 0u, ctx->cap, ctx->stream), "clear");
+    ctx->grads_clear = true;
     return train_step_plain(ctx, n_views, view_ids, o, w, adam, losses);
   }
   ctx->forward_done = true;
@@ -1005,6 +1015,7 @@ splat_status splat_set_capacity(splat_context* ctx, int32_t max_gaussians, int32
   CK(launch_fill_u32(ctx->stat_vc.u(), 0u, max_gaussians, ctx->stream), "clear");
   CK(hipStreamSynchronize(ctx->stream), "sync");
   ctx->forward_done = ctx->loss_done = false;
+  ctx->grads_clear = true;
   return SPLAT_OK;
 }
 
@@ -1037,6 +1048,7 @@ splat_status splat_upload_cloud(splat_context* ctx, const splat_cloud* c) {
   CK(launch_fill_f32(ctx->v.f(), 0.f, floats, ctx->stream), "clear");
   CK(launch_fill_f32(ctx->sg.f(), 0.f, cap, ctx->stream), "clear");
   CK(launch_fill_u32(ctx->vc.u(), 0u, cap, ctx->stream), "clear");
+  ctx->grads_clear = true;
   CK(launch_fill_f32(ctx->stat_sg.f(), 0.f, cap, ctx->stream), "clear");
   CK(launch_fill_u32(ctx->stat_vc.u(), 0u, cap, ctx->stream), "clear");
   CK(hipStreamSynchronize(ctx->stream), "sync");
@@ -1275,6 +1287,7 @@ splat_status splat_zero_gradients(splat_context* ctx) {
   CK(launch_fill_f32(ctx->sg.f(), 0.f, ctx->cap, ctx->stream), "clear");
   CK(launch_fill_u32(ctx->vc.u(), 0u, ctx->cap, ctx->stream), "clear");
   CK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->grads_clear = true;
   return SPLAT_OK;
 }
 

@@ -414,6 +414,7 @@ splat_status splat_densify(splat_context* ctx, const splat_densify_params* p, fl
   CKD(launch_fill_f32(ctx->grads.f(), 0.f, floats, ctx->stream), "clear");
   CKD(launch_fill_f32(ctx->sg.f(), 0.f, cap, ctx->stream), "clear");
   CKD(launch_fill_u32(ctx->vc.u(), 0u, cap, ctx->stream), "clear");
+  ctx->grads_clear = true;
   CKD(launch_fill_f32(ctx->stat_sg.f(), 0.f, cap, ctx->stream), "clear");
   CKD(launch_fill_u32(ctx->stat_vc.u(), 0u, cap, ctx->stream), "clear");
   uint32_t result[4];

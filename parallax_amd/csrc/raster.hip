@@ -254,12 +254,14 @@ __device__ __forceinline__ void forward_quad(const RasterParams& p, const ViewDe
       s_st[pos].pos = (int)(e - begin) + 1;
     }
     wave_sync_lds();
-    const int cnt = __popcll(m);
+    const int cnt = __builtin_amdgcn_readfirstlane(__popcll(m));  // wave-uniform
     // Branch-free over the chunk: every lane evaluates every splat and keeps
     // the result only where the engine's loop would (not yet terminated,
     // power <= 0, alpha >= min_alpha); a kept step is the same float
     // operations on the same operands, a dropped one adds an exact zero.
-    for (int j = 0; j < cnt; ++j) {
+    // (the counter passes through readfirstlane so that loop strength
+    // reduction cannot turn it into a vector down-counter with a carry)
+    for (int j = 0; j < cnt; j = __builtin_amdgcn_readfirstlane(j + 1)) {
       const float4 sa = s_st[j].a;
       const float4 sb = s_st[j].b;
       const float sc = s_st[j].c;

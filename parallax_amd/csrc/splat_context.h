@@ -68,6 +68,10 @@ struct splat_context {
   splat::RasterParams raster{};
   bool forward_done = false;
   bool loss_done = false;
+  // the stored gradient (grads) and the step's sg / vc are all zeros: true
+  // after allocation, upload, densify, a clear, an Adam step and every
+  // training step; false after a standalone adjoint, which adds into them
+  bool grads_clear = true;
   float last_ssim_weight = 0.f;
   float ssim_kernel[11] = {};
 

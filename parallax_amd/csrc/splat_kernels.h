@@ -240,6 +240,7 @@ struct AdamFuse {
   float* v = nullptr;
   float* stat_sg = nullptr;
   int* stat_vc = nullptr;
+  bool stored_clear = false;  // the stored gradient, sg and vc are all zeros: not read, not cleared
 };
 hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const float* params, const SplatRec* rec,
                           const float* screen, float* grads, float* sg, int* vc, hipStream_t stream,

@@ -131,3 +131,53 @@ def test_fused_adam_matches_separate_calls(ctx, sh):
             assert np.array_equal(getattr(a[0], k), getattr(b[0], k)), f"step {it}: {k}"
         for j, name in ((1, "screen gradient"), (2, "visible count"), (3, "m"), (4, "v")):
             assert np.array_equal(np.asarray(a[j]), np.asarray(b[j])), f"step {it}: {name}"
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["plain", "graph"])
+def test_step_adds_a_stored_gradient(ctx, graph):
+    """A standalone adjoint leaves its gradient stored; the next training step
+    adds it to its own before Adam (the fused update's general form), exactly
+    as the separate calls do, and the step after that starts from a clear
+    gradient again (the form that neither reads nor clears it)."""
+    cloud, views, targets, batches = scene()
+    opts = S.options(sh_degree=3, deterministic=True)
+
+    def start():
+        ctx.set_graph_mode(False)
+        ctx.set_capacity(cloud.n, 3)
+        ctx.upload_cloud(cloud)
+        ctx.upload_views(views, targets)
+
+    def snapshot():
+        sg, vc = ctx.stats()
+        m, v = ctx.moments()
+        return ctx.download_cloud(), sg, vc, m, v
+
+    print("[stored] separate calls", flush=True)
+    start()
+    ctx.render_forward([0], opts)
+    ctx.loss(0.2)
+    ctx.render_backward()  # stored: view 0's gradient
+    sep = []
+    for it, b in enumerate(([1], [2], [1], [2])):
+        ctx.render_forward(b, opts)
+        ctx.loss(0.2)
+        ctx.render_backward()
+        ctx.adam_step(S.adam_params(it, 1e-4))
+        sep.append(snapshot())
+    print(f"[stored] training steps, graph={graph}", flush=True)
+    start()
+    ctx.render_forward([0], opts)
+    ctx.loss(0.2)
+    ctx.render_backward()
+    ctx.set_graph_mode(graph)
+    fused = []
+    for it, b in enumerate(([1], [2], [1], [2])):
+        ctx.train_step(b, opts, 0.2, S.adam_params(it, 1e-4))
+        fused.append(snapshot())
+    ctx.set_graph_mode(False)
+    for it, (a, b) in enumerate(zip(sep, fused)):
+        for k in ("pos", "log_scale", "rot", "opacity", "dc", "rest"):
+            assert np.array_equal(getattr(a[0], k), getattr(b[0], k)), f"step {it}: {k}"
+        for j, name in ((1, "screen gradient"), (2, "visible count"), (3, "m"), (4, "v")):
+            assert np.array_equal(np.asarray(a[j]), np.asarray(b[j])), f"step {it}: {name}"
