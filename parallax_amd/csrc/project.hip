@@ -68,20 +68,33 @@ __device__ __forceinline__ int sh_basis(int degree, float x, float y, float z, f
 // within 1.2e-6 * (ac/det) * Q of the exact one. q is therefore raised by 1e-5
 // (the exponential's and the product's rounding), the box widened by
 // 1/(1 - 4e-6 ac/det) in q and by 1e-3 px, and a splat too elongated for the
-// bound to be meaningful gets no box (+inf). Evaluated in double.
-__device__ __forceinline__ void cull_extents(float a, float b, float c, float opacity, float min_alpha, float* ex,
-                                             float* ey) {
-  const double A = a, B = b, Cc = c;
-  const double det = A * Cc - B * B;
+// bound to be meaningful gets no box (+inf).
+//
+// q depends on the splat only (cull_q, once per gaussian, in double); the box
+// is evaluated per view in float. det comes from Kahan's fused form
+// fma(a, c, -b b) + fma(-b, b, b b), within 2 ulp of the exact a c - b^2 for
+// every conic the shrink test keeps (ac/det < 1.9e5, far from the 1/ulp where
+// it would fail); with the hardware reciprocal and square root (1 ulp each)
+// the float result is within 1e-6 of the exact one, and the factor 1 + 1e-5
+// (against 1 + 1e-6 in a double evaluation) keeps the box containing the
+// exact box. tests/test_cull_rules.py checks it against the double form.
+__device__ __forceinline__ float cull_q(float opacity, float min_alpha) {
+  if (!(min_alpha > 0.f)) return -1.f;  // thresholds off: no box
+  const double q = 2.0 * (log((double)opacity / (double)min_alpha) + 1e-5);
+  return q < 0.0 ? 0.f : (float)(q * (1.0 + 1e-7));  // rounded to float, up
+}
+__device__ __forceinline__ void cull_extents(float a, float b, float c, float q, float* ex, float* ey) {
   *ex = *ey = __builtin_inff();
-  if (!(det > 0.0) || !(min_alpha > 0.f)) return;
-  const double kappa = A * Cc / det;
-  const double shrink = 1.0 - 4e-6 * kappa;
-  if (!(shrink > 0.25)) return;
-  double q = 2.0 * (log((double)opacity / (double)min_alpha) + 1e-5);
-  if (q < 0.0) q = 0.0;
-  *ex = (float)(sqrt(q * Cc / (det * shrink)) * (1.0 + 1e-6) + 1e-3);
-  *ey = (float)(sqrt(q * A / (det * shrink)) * (1.0 + 1e-6) + 1e-3);
+  if (!(q >= 0.f)) return;
+  const float bb = b * b;
+  const float det = __builtin_fmaf(a, c, -bb) + __builtin_fmaf(-b, b, bb);
+  if (!(det > 0.f)) return;
+  const float rdet = __builtin_amdgcn_rcpf(det);
+  const float shrink = 1.f - 4e-6f * ((a * c) * rdet);
+  if (!(shrink > 0.25f)) return;
+  const float s = q * rdet * __builtin_amdgcn_rcpf(shrink);
+  *ex = __builtin_amdgcn_sqrtf(s * c) * 1.00001f + 1e-3f;
+  *ey = __builtin_amdgcn_sqrtf(s * a) * 1.00001f + 1e-3f;
 }
 
 template <int kDeg>
@@ -129,6 +142,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
   const float opacity = ((1.0f) / (1.0f + det_exp(-logit)));
+  const float q_cull = cull_q(opacity, min_alpha);  // the cull ellipse's level, the same in every view
   const float dc0 = dcp[0], dc1 = dcp[1], dc2 = dcp[2];
   const int per_channel = rest_stride / 3;
   // the SH coefficients this degree uses, loaded once for all views
@@ -193,7 +207,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
         if (opacity < min_alpha) ok = false;
         if (ok) {
           o.opacity = opacity;
-          cull_extents(o.ca, o.cb, o.cc, opacity, min_alpha, &o.ex, &o.ey);
+          cull_extents(o.ca, o.cb, o.cc, q_cull, &o.ex, &o.ey);
           const float lx2 = V.thx * kSlack, ly2 = V.thy * kSlack;
           if (fabsf(((pc[0]) / (z))) > lx2) o.flags |= 8u;
           if (fabsf(((pc[1]) / (z))) > ly2) o.flags |= 16u;

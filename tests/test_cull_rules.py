@@ -97,3 +97,87 @@ def test_splats_just_beyond_the_corner_are_never_wrongly_rejected():
     rejected, wrong = check(a, b, c, op, mx, my, rng)
     assert rejected > n // 8
     assert wrong == 0
+
+
+# ---- the projection's cull box (project.hip: cull_q, cull_extents) ----
+
+def cull_box_double(a, b, c, op, min_alpha):
+    """The box in double precision: the exact half extents widened by 1 + 1e-6
+    and 1e-3 px (+inf where the conic is not positive definite or too
+    elongated for the float error bound, ac/det > 187500)."""
+    A, B, C = (np.asarray(v, np.float64) for v in (a, b, c))
+    det = A * C - B * B
+    ex = np.full(A.shape, np.inf)
+    ey = np.full(A.shape, np.inf)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        kappa = A * C / det
+        shrink = 1.0 - 4e-6 * kappa
+        ok = (det > 0) & (shrink > 0.25)
+        q = np.maximum(0.0, 2.0 * (np.log(np.float64(op) / np.float64(min_alpha)) + 1e-5))
+        ex[ok] = np.sqrt(q * C / (det * shrink))[ok] * (1 + 1e-6) + 1e-3
+        ey[ok] = np.sqrt(q * A / (det * shrink))[ok] * (1 + 1e-6) + 1e-3
+    return ex, ey
+
+
+def cull_box_float(a, b, c, op, min_alpha, rcp_err, sqrt_err):
+    """float32 restatement of the kernel's box: q per gaussian in double,
+    rounded up; Kahan's fused determinant; the hardware reciprocal and square
+    root modelled as the correctly rounded ones times (1 + err)."""
+    a, b, c = (np.asarray(v, f32) for v in (a, b, c))
+    qd = np.maximum(0.0, 2.0 * (np.log(np.float64(op) / np.float64(min_alpha)) + 1e-5))
+    q = (qd * (1.0 + 1e-7)).astype(f32)
+
+    def fma(x, y, z):  # one rounding of x y + z (the double product of two floats is exact)
+        return (x.astype(np.float64) * y.astype(np.float64) + z.astype(np.float64)).astype(f32)
+
+    def rcp(x):
+        return ((f32(1) / x).astype(f32) * f32(1 + rcp_err)).astype(f32)
+
+    def hsqrt(x):
+        return (np.sqrt(x).astype(f32) * f32(1 + sqrt_err)).astype(f32)
+
+    bb = (b * b).astype(f32)
+    det = (fma(a, c, -bb) + fma(-b, b, bb)).astype(f32)
+    ex = np.full(a.shape, np.inf, f32)
+    ey = np.full(a.shape, np.inf, f32)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        rdet = rcp(det)
+        shrink = (f32(1) - (f32(4e-6) * ((a * c).astype(f32) * rdet).astype(f32)).astype(f32)).astype(f32)
+        ok = (det > 0) & (shrink > f32(0.25))
+        s = ((q * rdet).astype(f32) * rcp(shrink)).astype(f32)
+        bx = (hsqrt((s * c).astype(f32)) * f32(1.00001) + f32(1e-3)).astype(f32)
+        by = (hsqrt((s * a).astype(f32)) * f32(1.00001) + f32(1e-3)).astype(f32)
+    ex[ok], ey[ok] = bx[ok], by[ok]
+    return ex, ey, ok, (a.astype(np.float64) * c) / np.where(det > 0, det, 1)
+
+
+def test_the_float_cull_box_holds_the_double_box():
+    rng = np.random.default_rng(13)
+    n = 200_000
+    a, b, c = conics(rng, n)
+    # near-singular conics too: the minor axis up to 2000 times the major
+    s = np.exp(rng.uniform(-1, 3.5, n))
+    ratio = np.exp(rng.uniform(0, np.log(2000.0), n))
+    th = rng.uniform(0, np.pi, n)
+    cs, sn = np.cos(th), np.sin(th)
+    cxx = cs * cs * s ** 2 + sn * sn * (s * ratio) ** 2
+    cyy = sn * sn * s ** 2 + cs * cs * (s * ratio) ** 2
+    cxy = cs * sn * (s ** 2 - (s * ratio) ** 2)
+    det = cxx * cyy - cxy * cxy
+    a = np.concatenate([a, cyy / det]).astype(f32)
+    b = np.concatenate([b, -cxy / det]).astype(f32)
+    c = np.concatenate([c, cxx / det]).astype(f32)
+    op = rng.uniform(1e-3, 1, len(a))
+    for min_alpha in (MIN_ALPHA, 1e-3, 1e-20):
+        dex, dey = cull_box_double(a, b, c, op, min_alpha)
+        for rcp_err in (-2.4e-7, 2.4e-7):
+            for sqrt_err in (-2.4e-7, 2.4e-7):
+                fex, fey, ok, kappa = cull_box_float(a, b, c, op, min_alpha, rcp_err, sqrt_err)
+                # wherever the double box is finite the float box holds it,
+                # or is +inf (no cull); a finite float box where the double
+                # gave none is allowed only at the elongation limit
+                finite = np.isfinite(dex)
+                assert np.all((fex[finite] >= dex[finite]) & (fey[finite] >= dey[finite])), min_alpha
+                extra = ok & ~finite
+                assert np.all(np.abs(kappa[extra] / 187500.0 - 1) < 1e-4), min_alpha
+                assert finite.sum() > len(a) // 2
