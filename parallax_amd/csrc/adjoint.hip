@@ -415,21 +415,39 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjointParams p, const ViewDesc
       const int nv = __popcll(__ballot(true));
       const long long base = 14 * cap + (long long)rs * (g - lane);
       const float inv_rs = 1.f / (float)rs;
-      for (int e = lane; e < nv * rs; e += nv) {
-        int j = (int)((float)e * inv_rs);  // gaussian of element e (exact after the correction)
-        if (j * rs > e) --j;
-        else if ((j + 1) * rs <= e) ++j;
-        int k = e - j * rs;  // coefficient within the gaussian's row, then within its channel
-        k = k >= 2 * per_channel ? k - 2 * per_channel : (k >= per_channel ? k - per_channel : k);
-        const bool add = used[j] != 0 && k < kCount;  // as the plain adjoint adds (used bands, touched gaussians)
-        const long long o = base + e;
-        const float pp = pw[o], gg = kClear ? 0.f : gw[o];
-        float mm = mw[o], vv = vw[o];
-        const float gt = add ? gg + row[e] : gg;
-        pw[o] = pp + adam_delta(mm, vv, gt, P.lr[5], P);
-        mw[o] = mm;
-        vw[o] = vv;
-        if (!kClear) gw[o] = 0.f;
+      const int total = nv * rs;
+      // kU elements a round per lane (e, e + nv, ..), every load of the round
+      // issued before any update: a round keeps 3 kU loads in flight where
+      // one element at a time left two waves per SIMD waiting on each
+      constexpr int kU = 9;
+      for (int e0 = lane; e0 < total; e0 += kU * nv) {
+        float pp[kU], gg[kU], mm[kU], vv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int e = e0 + u * nv;
+          const long long o = base + (e < total ? e : lane);  // past the block: a valid element, not stored
+          pp[u] = pw[o];
+          gg[u] = kClear ? 0.f : gw[o];
+          mm[u] = mw[o];
+          vv[u] = vw[o];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int e = e0 + u * nv;
+          if (e >= total) break;
+          int j = (int)((float)e * inv_rs);  // gaussian of element e (exact after the correction)
+          if (j * rs > e) --j;
+          else if ((j + 1) * rs <= e) ++j;
+          int k = e - j * rs;  // coefficient within the gaussian's row, then within its channel
+          k = k >= 2 * per_channel ? k - 2 * per_channel : (k >= per_channel ? k - per_channel : k);
+          const bool add = used[j] != 0 && k < kCount;  // as the plain adjoint adds (used bands, touched gaussians)
+          const long long o = base + e;
+          const float gt = add ? gg[u] + row[e] : gg[u];
+          pw[o] = pp[u] + adam_delta(mm[u], vv[u], gt, P.lr[5], P);
+          mw[o] = mm[u];
+          vw[o] = vv[u];
+          if (!kClear) gw[o] = 0.f;
+        }
       }
     }
     const float sg_stored = kClear ? 0.f : sg[g];
