@@ -154,6 +154,11 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ param
     for (int k = 0; k < kCount; ++k) rc[c][k] = rest[c * per_channel + k];
 
   uint32_t key_min = 0xFFFFFFFFu, key_max = 0u;  // over this gaussian's kept slots
+  // two views at a time: their chains (transform, Jacobian, 2D covariance,
+  // conic, SH colour, each with its IEEE divisions) are independent, and
+  // interleaving them beats the occupancy they cost (three waves per SIMD at
+  // SH3 instead of four)
+#pragma unroll 2
   for (int v = 0; v < n_views; ++v) {
     const ViewDesc& V = views[v];
     const long long slot = (long long)v * n + g;
