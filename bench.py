@@ -234,7 +234,9 @@ def main():
     design = {
         # 64 B record + depth key + packed tile rect (views up to 4096 px; 84 B a slot beyond)
         "project": n * param_bytes + n * B * 72.0,
-        "adjoint_adam": n * B * (64 + 48) + 32.0 * n * (14 + rs),  # 64 B record, 48 B screen; + the gradient clear
+        # 64 B record, 48 B screen; read p, m, v and write them (a training step's stored gradient is known
+        # to be zero and neither read nor cleared)
+        "adjoint_adam": n * B * (64 + 48) + 24.0 * n * (14 + rs),
         "loss": px_batch * LOSS_DESIGN_BYTES_PER_PX,
         "tile_write": 4.0 * pairs,
     }
