@@ -261,3 +261,35 @@ def test_empty_cloud_renders_background(ctx):
     assert np.all(T == 1.0)
     assert np.all(rgb.reshape(-1, 3) == np.array([0.25, 0.5, 0.75], np.float32))
     assert np.all(last == 0)
+
+
+@pytest.mark.parametrize("min_alpha", [1e-31, 2e-30, 1e-3], ids=["below_cut", "above_cut", "1e-3"])
+def test_min_alpha_around_the_falloff_cut(ctx, min_alpha):
+    """The compositing loops drop the falloff's power <= -80 test for
+    min_alpha >= 1e-30 (a splat there has alpha <= 1.9e-35 either way) and keep
+    it below. On both sides of that cut, and with elongated splats whose tile
+    rects reach pixels with power far below -80, the image and last
+    contributors equal the oracle's bit for bit and the gradients match FP64."""
+    cloud = syn.random_cloud(600, 17, 2, (-1, -1, 2), (1, 1, 5), log_scale_mu=-2.2, log_scale_sigma=1.0)
+    view = syn.pinhole(192, 144, 220.0)
+    kw = dict(sh_degree=2, background=(0.1, 0.2, 0.3), min_alpha=min_alpha)
+    tgt = np.random.default_rng(61).uniform(size=view.pixels * 3).astype(np.float32)
+    print(f"[parity] min_alpha {min_alpha}", flush=True)
+    load(ctx, cloud, [view], [tgt])
+    ctx.zero_gradients()
+    ctx.render_forward([0], S.options(**kw))
+    f = O.Frame(cloud, view, O.default_options(**kw))
+    rgb, T, last = ctx.image(0)
+    orgb, oT, olast = f.image()
+    assert np.array_equal(rgb, orgb)
+    assert np.array_equal(T, oT)
+    assert np.array_equal(last, olast)
+    ctx.loss(0.2)
+    ctx.render_backward()
+    g = ctx.gradients()
+    gdl = ctx.image_gradient(0)
+    og64 = f.backward_f64(gdl)
+    og32 = f.backward(gdl)
+    assert rel_l2(flat_grads(g), oracle_flat(og64)) <= GRAD_REL_L2
+    assert np.array_equal(g["visible_count"], og32.visible_count)
+    assert rel_l2(g["screen_gradient"], og64.screen_gradient) <= GRAD_REL_L2
