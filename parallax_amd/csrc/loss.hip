@@ -185,13 +185,15 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
       float x0, y0, x1, y1;
     };
     Row q[kStatAhead];
-    // lanes read columns x0 - 5 + lane and x0 + 59 + lane (kOff outside the
-    // image); rows outside it read row 0 and are zeroed by a uniform select
+    // lanes read columns x0 - 5 + lane and, the first ten of them, x0 + 59 +
+    // lane: the 74 columns the windows use (kOff outside the image, or
+    // unused: no memory access); rows outside it read row 0 and are zeroed
+    // by a uniform select
     const __amdgpu_buffer_rsrc_t rimg = buffer_of(img, (uint32_t)(pixels * 12));
     const __amdgpu_buffer_rsrc_t rtgt = buffer_of(tgt, (uint32_t)(pixels * 12));
     const int cola = x0 - kR + lane, colb = cola + 64;
     const uint32_t la = cola >= 0 && cola < W ? (uint32_t)((3 * cola + c) * 4) : kOff;
-    const uint32_t lb = colb >= 0 && colb < W ? (uint32_t)((3 * colb + c) * 4) : kOff;
+    const uint32_t lb = lane < 2 * kR && colb < W ? (uint32_t)((3 * colb + c) * 4) : kOff;
     auto fetch = [&](int r) {
       const int row = by - kR + r;
       const bool row_ok = row >= 0 && row < H;
@@ -372,11 +374,14 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
   const __amdgpu_buffer_rsrc_t rp2 = buffer_of(plane0 + 2 * total_px, plane_bytes);
   const __amdgpu_buffer_rsrc_t rimg = buffer_of(img, (uint32_t)(pixels * 12));
   const __amdgpu_buffer_rsrc_t rtgt = buffer_of(tgt, (uint32_t)(pixels * 12));
-  uint32_t lp[3];  // plane columns sx - 5 + lane + 64 k (kOff outside the image)
+  // plane columns sx - 5 + lane + 64 k, the 138 the windows use (kOff
+  // outside the image or past them: such a lane makes no memory access)
+  uint32_t lp[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int colk = sx - kR + lane + 64 * k;
-    lp[k] = colk >= 0 && colk < W ? (uint32_t)(colk * 4) : kOff;
+    const bool used = k < 2 || lane < 2 * kR;
+    lp[k] = used && colk >= 0 && colk < W ? (uint32_t)(colk * 4) : kOff;
   }
   uint32_t lo[2];  // this lane's two pixels, channel c
 #pragma unroll
