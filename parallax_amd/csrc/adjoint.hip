@@ -527,6 +527,7 @@ __global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ m_dev, uint32_t p_cap,
                                                     const uint32_t* __restrict__ chunk_owner,
                                                     const float* __restrict__ partial,
+                                                    const uint8_t* __restrict__ visit,
                                                     const uint32_t* __restrict__ epoch, float* __restrict__ screen,
                                                     float* __restrict__ carry_first, float* __restrict__ carry_last) {
   const int lane = threadIdx.x & 63;
@@ -554,8 +555,10 @@ __global__ __launch_bounds__(256) void k_det_chunks(const uint32_t* __restrict__
   float v[11];
 #pragma unroll
   for (int k = 0; k < 11; ++k) v[k] = 0.f;
-  if (valid) {
-    // a pair the adjoint did not visit this pass holds an older tag: zeros
+  // a pair the adjoint did not visit this pass holds an older tag: zeros.
+  // Its visit byte is then (but for a wrap of the byte) another than the
+  // tag's low byte, and the record is not loaded; the tag decides.
+  if (valid && visit[r] == (uint8_t)tag) {
     const float4* src = reinterpret_cast<const float4*>(partial + r * kPartialStride);
     const float4 a = src[0], b = src[1], cc = src[2];
     if (__float_as_uint(cc.w) == tag) {
@@ -648,7 +651,8 @@ long long det_reduce_carry_floats(unsigned long long p_cap) {
 
 hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
                              long long m, const uint32_t* m_dev, uint32_t p_cap, const float* partial,
-                             const uint32_t* epoch, float* screen, float* carry, hipStream_t stream) {
+                             const uint8_t* visit, const uint32_t* epoch, float* screen, float* carry,
+                             hipStream_t stream) {
   if (m <= 0) return hipSuccess;
   const long long chunks = ((long long)p_cap + kDrChunk - 1) / kDrChunk + 1;
   float* carry_first = carry;
@@ -659,7 +663,8 @@ hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offse
                        m, m_dev, p_cap, chunk_owner);
     SPLAT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_det_chunks, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, stream, sorted_slots, offsets,
-                       p_total, m, m_dev, p_cap, chunk_owner, partial, epoch, screen, carry_first, carry_last);
+                       p_total, m, m_dev, p_cap, chunk_owner, partial, visit, epoch, screen, carry_first,
+                       carry_last);
     SPLAT_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_det_owners, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, sorted_slots, offsets,

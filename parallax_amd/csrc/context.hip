@@ -129,7 +129,7 @@ std::vector<DeviceBuffer*> splat_context::all_buffers() {
   return {&params, &grads, &m, &v, &sg, &vc, &stat_sg, &stat_vc, &targets, &d_views, &rec, &keys, &vals, &keys_alt,
           &vals_alt, &counts, &offsets, &pair_base, &scounts, &scounts_alt, &soffsets, &keys_c, &vals_c, &scounts_c, &key_ranges, &rects, &skeys, &svals, &skeys_alt, &svals_alt,
           &sranges, &nseg, &seg_base, &seg_st, &bin_table, &pair_vals, &ranges,
-          &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &carry, &scan_ws, &hist_ws, &scalars,
+          &rgb, &T, &last, &dL, &terms, &block_sums, &loss_sums, &screen, &partial, &visit, &carry, &scan_ws, &hist_ws, &scalars,
           &scratch_params, &scratch_m, &scratch_v, &dens_flags, &dens_a, &dens_b, &dens_c, &dens_src, &d_adam,
           &order_keys, &order_vals, &order_keys_alt, &order_vals_alt};
 }
@@ -301,14 +301,23 @@ splat_status upload_batch(splat_context* ctx) {
   return SPLAT_OK;
 }
 
-// The deterministic adjoint's per-pair partials. A new allocation is
-// cleared: tag 0 is no epoch, so every record reads as zero until written.
+// The deterministic adjoint's per-pair partials and their visit bytes. A new
+// allocation is cleared: tag 0 is no epoch, so every record reads as zero
+// until written (a visit byte of 0 only sends the reduction to the record's
+// own tag). A reallocation always grows `bytes`; the allocator may return
+// the freed address, so the pointer cannot tell.
 splat_status reserve_partials(splat_context* ctx, unsigned long long p_cap) {
-  const void* before = ctx->partial.ptr;
-  CK(ctx->partial.reserve(sizeof(float) * kPartialStride * std::max<unsigned long long>(1, p_cap)), "alloc partials");
-  if (ctx->partial.ptr != before)
+  p_cap = std::max<unsigned long long>(1, p_cap);
+  const size_t before = ctx->partial.bytes;
+  CK(ctx->partial.reserve(sizeof(float) * kPartialStride * p_cap), "alloc partials");
+  if (ctx->partial.bytes != before)
     CK(launch_fill_f32(ctx->partial.f(), 0.f, (long long)(ctx->partial.bytes / sizeof(float)), ctx->stream),
        "clear partials");
+  const size_t visit_before = ctx->visit.bytes;
+  CK(ctx->visit.reserve((p_cap + 3) & ~3ULL), "alloc visit bytes");
+  if (ctx->visit.bytes != visit_before)
+    CK(launch_fill_u32(ctx->visit.u(), 0u, (long long)(ctx->visit.bytes / sizeof(uint32_t)), ctx->stream),
+       "clear visit bytes");
   return SPLAT_OK;
 }
 
@@ -634,7 +643,8 @@ splat_status enqueue_backward(splat_context* ctx, bool fuse_adam = false, const 
   auto launch = [&](const RasterConfig& c) {
     return launch_backward(ctx->raster, c, ctx->views_dev(), ctx->ranges.u(), ctx->sorted_pair_vals, ctx->recs(),
                            ctx->dL.f(), ctx->T.f(), ctx->last.i(), ctx->screen.f(), ctx->pair_base.u(),
-                           ctx->partial.f(), p_cap, det, epoch, ctx->tile_order, ctx->scalars.u() + 4, ctx->stream);
+                           ctx->partial.f(), static_cast<uint8_t*>(ctx->visit.ptr), p_cap, det, epoch,
+                           ctx->tile_order, ctx->scalars.u() + 4, ctx->stream);
   };
   if (ctx->tuning_backward && !ctx->capturing) {
     // the adjoint's launch shape for this batch shape: each variant timed on
@@ -669,8 +679,9 @@ splat_status enqueue_backward(splat_context* ctx, bool fuse_adam = false, const 
   ap.deterministic = det ? 1 : 0;
   if (det)
     TIMED("det_reduce", launch_det_reduce(ctx->sorted_slots, ctx->offsets.u(), ctx->scalars.u(), M,
-                                          ctx->scalars.u() + kSortedScalar, p_cap, ctx->partial.f(), epoch,
-                                          ctx->screen.f(), ctx->carry.f(), ctx->stream));
+                                          ctx->scalars.u() + kSortedScalar, p_cap, ctx->partial.f(),
+                                          static_cast<const uint8_t*>(ctx->visit.ptr), epoch, ctx->screen.f(),
+                                          ctx->carry.f(), ctx->stream));
   if (fuse_adam) {
     AdamFuse fuse;
     fuse.step = static_cast<const AdamStep*>(ctx->d_adam.ptr);

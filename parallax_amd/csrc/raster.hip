@@ -545,8 +545,8 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
                                                  const SplatRec* __restrict__ rec, const float* __restrict__ dL_drgb,
                                                  const float* __restrict__ T_in, const int* __restrict__ last_in,
                                                  float* __restrict__ screen, const uint32_t* __restrict__ pair_base,
-                                                 float* __restrict__ partial, uint32_t p_cap, int deterministic,
-                                                 const uint32_t* __restrict__ epoch,
+                                                 float* __restrict__ partial, uint8_t* __restrict__ visit,
+                                                 uint32_t p_cap, int deterministic, const uint32_t* __restrict__ epoch,
                                                  const uint32_t* __restrict__ order, uint32_t* __restrict__ counter) {
   static_assert(kBB == 64 || kBB == 128, "chunks of one or two entries per lane");
   static_assert(kScreenStride >= kAcc + 1 && kPartialStride == 16, "record slots");
@@ -570,6 +570,8 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
   const float slot_scale = (my_slot == 7 || my_slot == 9) ? 0.5f : 1.f;
   // deterministic records: slots 0-10 the terms, 11 this pass's tag, 12-15 zero
   const float rec_pad = my_slot == 11 ? __uint_as_float(deterministic ? *epoch : 0u) : 0.f;
+  // and beside each record its visit byte, the tag's low byte
+  const uint8_t visit_tag = (uint8_t)(deterministic ? *epoch : 0u);
   for (int iter = 0;; ++iter) {
     int tile;
     if (kQueue) {
@@ -701,11 +703,13 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
           // lanes 0, 4, .., 40 hold the splat's totals: atomics (nonzero terms
           // only), or the partial's 64-byte store from lanes 0, 4, .., 60
           // (slots 11-15 the tag and zero padding, so the store fills its
-          // whole memory sector)
+          // whole memory sector) and the visit byte from lane 0
           const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
           if (deterministic) {
-            if (slot_lane && dsti < p_cap)
+            if (slot_lane && dsti < p_cap) {
               partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
+              if (lane == 0) visit[dsti] = visit_tag;
+            }
           } else if (any && writer && z != 0.f) {
             unsafeAtomicAdd(screen + (long long)dsti * kScreenStride + my_slot, z);
           }
@@ -761,13 +765,14 @@ template <int kBB, bool kQueue, bool kLowCut>
 hipError_t backward_variant(const RasterParams& p, const ViewDesc* views, const uint32_t* ranges,
                             const uint32_t* pair_vals, const SplatRec* rec, const float* dL_drgb, const float* T,
                             const int* last, float* screen, const uint32_t* pair_base, float* partial,
-                            uint32_t p_cap, bool deterministic, const uint32_t* epoch, const uint32_t* order,
+                            uint8_t* visit, uint32_t p_cap, bool deterministic, const uint32_t* epoch,
+                            const uint32_t* order,
                             uint32_t* counter, int cus, hipStream_t stream) {
   static const int per_cu = resident_blocks(k_backward<kBB, kQueue, kLowCut>, 64);
   const int blocks = kQueue ? std::min(p.total_tiles, per_cu * cus) : p.total_tiles;
   hipLaunchKernelGGL((k_backward<kBB, kQueue, kLowCut>), dim3(blocks), dim3(64), 0, stream, p, views, ranges, pair_vals, rec,
-                     dL_drgb, T, last, screen, pair_base, partial, p_cap, deterministic ? 1 : 0, epoch, order,
-                     counter);
+                     dL_drgb, T, last, screen, pair_base, partial, visit, p_cap, deterministic ? 1 : 0, epoch,
+                     order, counter);
   return hipGetLastError();
 }
 
@@ -817,9 +822,11 @@ hipError_t launch_forward(const RasterParams& p, const RasterConfig& cfg, const 
 hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
                            const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec,
                            const float* dL_drgb, const float* T, const int* last, float* screen,
-                           const uint32_t* pair_base, float* partial, uint32_t p_cap, bool deterministic,
-                           const uint32_t* epoch, const uint32_t* order, uint32_t* counter, hipStream_t stream) {
+                           const uint32_t* pair_base, float* partial, uint8_t* visit, uint32_t p_cap,
+                           bool deterministic, const uint32_t* epoch, const uint32_t* order, uint32_t* counter,
+                           hipStream_t stream) {
   if (p.total_tiles <= 0) return hipSuccess;
+  if (deterministic && (partial == nullptr || visit == nullptr)) return hipErrorInvalidValue;
   if (cfg.bwd_queue) SPLAT_HIP_CHECK(launch_fill_u32(counter, 0u, 1, stream));
   const int pb = cfg.cus;
   // the falloff's power <= -80 test is needed only for min_alpha below the
@@ -827,7 +834,7 @@ hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const
   const bool low_cut = !(p.min_alpha >= 1e-30f);
 #define SPLAT_BWD(BB, Q, LC)                                                                                     \
   backward_variant<BB, Q, LC>(p, views, ranges, pair_vals, rec, dL_drgb, T, last, screen, pair_base, partial, \
-                              p_cap, deterministic, epoch, order, counter, pb, stream)
+                              visit, p_cap, deterministic, epoch, order, counter, pb, stream)
   switch ((low_cut ? 4 : 0) + (cfg.bwd_batch == 64 ? 0 : 2) + (cfg.bwd_queue ? 1 : 0)) {
     case 0: return SPLAT_BWD(64, false, false);
     case 1: return SPLAT_BWD(64, true, false);

@@ -124,7 +124,7 @@ struct splat_context {
   size_t readback_bytes = 0;
   void drop_graphs();
   DeviceBuffer rgb, T, last, dL, terms, block_sums, loss_sums;
-  DeviceBuffer screen, partial, carry;
+  DeviceBuffer screen, partial, visit, carry;
   DeviceBuffer scan_ws, hist_ws, scalars;  // scalars: [0] tile pairs, [1] super-tile pairs, [2] segments, [3] overflow
   splat::SortWorkspace sort_ws;
 

@@ -191,11 +191,14 @@ hipError_t launch_forward(const RasterParams& p, const RasterConfig& cfg, const 
 // wave-reduced atomics. Deterministic mode: per-pair partials written to
 // partial[pair_base[slot] + local pair] (kPartialStride floats each), for the
 // pairs the adjoint visits only, each tagged in slot 11 with *epoch; a record
-// whose tag is not the current epoch reads as zero.
+// whose tag is not the current epoch reads as zero. visit[pair] (p_cap
+// bytes) takes the epoch's low byte beside each record written, so the
+// reduction loads only records whose byte matches.
 hipError_t launch_backward(const RasterParams& p, const RasterConfig& cfg, const ViewDesc* views,
                            const uint32_t* ranges, const uint32_t* pair_vals, const SplatRec* rec,
                            const float* dL_drgb, const float* T, const int* last, float* screen,
-                           const uint32_t* pair_base, float* partial, uint32_t p_cap, bool deterministic,
+                           const uint32_t* pair_base, float* partial, uint8_t* visit, uint32_t p_cap,
+                           bool deterministic,
                            const uint32_t* epoch, const uint32_t* order, uint32_t* counter, hipStream_t stream);
 
 // ---- loss ----
@@ -249,11 +252,13 @@ hipError_t launch_adjoint(const AdjointParams& p, const ViewDesc* views, const f
 // [offsets[i], offsets[i+1]) of depth-sorted position i < min(m, *m_dev),
 // the last run ending at *p_total, all clamped to p_cap) summed in a fixed
 // order. `carry` holds det_reduce_carry_floats(p_cap) floats.
-// Records tagged with another epoch than *epoch count as zeros.
+// Records tagged with another epoch than *epoch count as zeros; a record
+// whose visit byte is not the epoch's low byte is one and is not loaded.
 long long det_reduce_carry_floats(unsigned long long p_cap);
 hipError_t launch_det_reduce(const uint32_t* sorted_slots, const uint32_t* offsets, const uint32_t* p_total,
                              long long m, const uint32_t* m_dev, uint32_t p_cap, const float* partial,
-                             const uint32_t* epoch, float* screen, float* carry, hipStream_t stream);
+                             const uint8_t* visit, const uint32_t* epoch, float* screen, float* carry,
+                             hipStream_t stream);
 // The next epoch of the deterministic partials: *epoch + 1, skipping 0 (the
 // tag of a cleared buffer). A device-side counter, so each replay of a
 // captured step takes a new one; a record can be mistaken for current only
