@@ -200,9 +200,11 @@ def main():
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
     rc = cfg_batched
     # the compositing kernels keep the falloff's power <= -80 test only for
-    # min_alpha below 1e-30 (third template argument)
+    # min_alpha below 1e-30 (third template argument); the adjoint's fourth
+    # is deterministic mode
     low_cut = str(not (opts.min_alpha >= 1e-30)).lower()
-    variant = (f"k_backward<{rc['backward_batch']},{str(rc['backward_queue']).lower()},{low_cut}>"
+    det = str(bool(opts.deterministic)).lower()
+    variant = (f"k_backward<{rc['backward_batch']},{str(rc['backward_queue']).lower()},{low_cut},{det}>"
                if roofline["kernel"] == "backward" else
                f"k_forward<{['16x16', '8x8', '32x8'].index(rc['forward_shape'])},{str(rc['forward_queue']).lower()},"
                f"{low_cut}>")

@@ -25,10 +25,10 @@
 //        per column: the five statistics planes (x, y, xx, yy, xy) per pixel,
 //        then the SSIM value, the three partial planes (to HBM) and the L1
 //        terms; per-half-unit L1 and SSIM sums in a fixed order.
-// grad:  three waves per unit, one per channel, two columns per lane: the
-//        three partial planes of the channel are blurred and combined into
-//        dL/drgb; the three channels of 12 output rows are assembled in LDS and
-//        written as whole interleaved rows.
+// grad:  three waves per half unit, one per channel, one lane per column:
+//        the three partial planes of the channel are blurred and combined
+//        into dL/drgb; the three channels of 12 output rows are assembled in
+//        LDS and written as whole interleaved rows.
 // The blur taps run in tap order -5..5 and the per-pixel SSIM expressions
 // are the engine's, with its divisions by c d, c^2 d and c d^2 taken as
 // products of two hardware reciprocals (1 ulp each); the loss and dL/drgb
@@ -50,8 +50,8 @@ constexpr int kRing = 12;
 constexpr int kUnitW = 128;               // columns per unit
 constexpr int kUnitH = 86;                // output rows per unit: 96 input rows, eight groups of 12
 constexpr int kStatWaves = 3;             // stats: one wave per channel over half a unit (64 columns)
-constexpr int kGradWaves = 3;             // grad: 1 wave per channel, 2 columns per lane
-constexpr int kGradPitch = 192;           // 138 staged columns per grad wave (three per lane written)
+constexpr int kGradWaves = 3;             // grad: one wave per channel over half a unit (64 columns)
+constexpr int kGradPitch = 128;           // 74 staged columns per grad wave (two per lane written)
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 
@@ -76,23 +76,23 @@ struct Unit {
   const ViewDesc* V;
 };
 
-// The block's unit, or with `half` its half unit (64 columns). The gradient
-// pass takes the units in the reverse of the statistics pass's order: the
-// partial planes written last are then read first, while the memory-side
-// cache may still hold them.
-__device__ __forceinline__ Unit unit_of(const LossParams& p, const ViewDesc* views, bool half) {
+// The block's half unit (64 columns). With `reverse` (the gradient pass)
+// the blocks take the half units in the reverse of the statistics pass's
+// order: the partial planes written last are then read first, while the
+// memory-side cache may still hold them.
+__device__ __forceinline__ Unit unit_of(const LossParams& p, const ViewDesc* views, bool reverse) {
   Unit u;
-  const int n = half ? p.total_tiles : p.total_tiles / 2;
+  const int n = p.total_tiles;
   u.block = xcd_remap(blockIdx.x, n);
-  if (!half) u.block = n - 1 - u.block;
+  if (reverse) u.block = n - 1 - u.block;
   int v = 0;
   for (int k = 1; k < p.n_views; ++k)
-    if ((half ? views[k].lt_base : views[k].lt_base / 2) <= u.block) v = k;
+    if (views[k].lt_base <= u.block) v = k;
   const ViewDesc& V = views[v];
-  const int local = u.block - (half ? V.lt_base : V.lt_base / 2);
-  const int unit = half ? local >> 1 : local;
+  const int local = u.block - V.lt_base;
+  const int unit = local >> 1;
   const int strips = (V.width + kUnitW - 1) / kUnitW;
-  u.sx = (unit % strips) * kUnitW + (half ? 64 * (local & 1) : 0);
+  u.sx = (unit % strips) * kUnitW + 64 * (local & 1);
   u.by = (unit / strips) * kUnitH;
   u.rows = min(kUnitH, V.height - u.by);
   u.V = &V;
@@ -112,7 +112,7 @@ __device__ __forceinline__ float pixel_or_zero(const float* img, int width, int 
 // Rows a wave's loads run ahead of its compositing of them: enough rows in
 // flight to cover an HBM round trip.
 constexpr int kStatAhead = 4;
-constexpr int kGradAhead = 3;
+constexpr int kGradAhead = 4;
 
 // A raw buffer over `bytes` bytes at `base`: stores at offsets past the end
 // are dropped by the hardware, so a lane is switched off by giving it the
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
   __shared__ float s_x[kStatWaves][2 * 128];
   __shared__ float s_y[kStatWaves][2 * 128];
   __shared__ float s_red[2][kStatWaves];
-  const Unit U = unit_of(p, views, true);
+  const Unit U = unit_of(p, views, false);
   const ViewDesc& V = *U.V;
   // the wave index read from lane 0 is known to be uniform: buffer
   // descriptors and plane bases built from it stay in scalar registers
@@ -336,18 +336,23 @@ __global__ __launch_bounds__(kStatWaves * 64) __attribute__((amdgpu_waves_per_eu
 }
 
 // ----------------------------------------------------------------- grad --
-__global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void k_loss_grad(LossParams p, const ViewDesc* __restrict__ views,
+// Three waves per half unit, one per channel, one lane per column: the
+// three partial planes of the channel are row-blurred from a 74-column LDS
+// slice, column-blurred from a 12-row register ring, and combined into
+// dL/drgb; the three channels of 12 output rows are assembled in LDS and
+// written as whole interleaved rows.
+__global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_loss_grad(LossParams p, const ViewDesc* __restrict__ views,
                                                                const float* __restrict__ rgb,
                                                                const float* __restrict__ targets,
                                                                const float* __restrict__ terms, long long total_px,
                                                                float* __restrict__ dL) {
-  __shared__ __align__(16) float s_t[kGradWaves][3][kGradPitch];  // 138 staged columns; every lane writes three
-  __shared__ __align__(16) float s_dl[kRing][kUnitW * 3];          // one group of output rows, interleaved
-  const Unit U = unit_of(p, views, false);
+  __shared__ __align__(16) float s_t[kGradWaves][3][kGradPitch];  // 74 staged columns; every lane writes two
+  __shared__ __align__(16) float s_dl[kRing][64 * 3];              // one group of output rows, interleaved
+  const Unit U = unit_of(p, views, true);
   const ViewDesc& V = *U.V;
   const int lane = threadIdx.x & 63;
   const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave = channel, wave-uniform
-  const int ca = U.sx + 2 * lane;                          // this lane's two columns: ca, ca + 1
+  const int col = U.sx + lane;                                     // this lane's column
   // the view's fields in registers (the buffer stores below are opaque)
   const int W = V.width, H = V.height, by = U.by, sx = U.sx, rows = U.rows;
   const long long pixels = (long long)W * H;
@@ -356,16 +361,16 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
   const float* img = rgb + 3 * V.pixel_base;
   const float* tgt = targets + V.target_base;
   const float* plane0 = terms + (long long)(3 * c) * total_px + V.pixel_base;
-  const int valid_floats = 3 * min(kUnitW, W - sx);
+  const int valid_floats = 3 * min(64, W - sx);
   const __amdgpu_buffer_rsrc_t rdl = buffer_of(dL + 3 * V.pixel_base, (uint32_t)(pixels * 12));
   float (*st)[kGradPitch] = s_t[c];
 
-  // for input row r: the three planes at staged columns sx - 5 + lane + 64 k,
-  // and x and y of this lane's two pixels in output row r - 10; loaded
-  // kGradAhead rows before they are used (clamped loads, no branches)
+  // for input row r: the three planes at staged columns sx - 5 + lane and
+  // sx + 59 + lane, and x and y of this lane's pixel in output row r - 10;
+  // loaded kGradAhead rows before they are used (clamped loads, no branches)
   struct Row {
-    float v[3][3];
-    float ox[2], oy[2];
+    float v[3][2];
+    float ox, oy;
   };
   Row q[kGradAhead];
   const uint32_t plane_bytes = (uint32_t)(pixels * 4);
@@ -374,25 +379,22 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
   const __amdgpu_buffer_rsrc_t rp2 = buffer_of(plane0 + 2 * total_px, plane_bytes);
   const __amdgpu_buffer_rsrc_t rimg = buffer_of(img, (uint32_t)(pixels * 12));
   const __amdgpu_buffer_rsrc_t rtgt = buffer_of(tgt, (uint32_t)(pixels * 12));
-  // plane columns sx - 5 + lane + 64 k, the 138 the windows use (kOff
-  // outside the image or past them: such a lane makes no memory access)
-  uint32_t lp[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int colk = sx - kR + lane + 64 * k;
-    const bool used = k < 2 || lane < 2 * kR;
-    lp[k] = used && colk >= 0 && colk < W ? (uint32_t)(colk * 4) : kOff;
+  // plane columns sx - 5 + lane and, the first ten lanes, sx + 59 + lane: the
+  // 74 the windows use (kOff outside the image or past them: no memory access)
+  uint32_t lp[2];
+  {
+    const int cola = sx - kR + lane, colb = cola + 64;
+    lp[0] = cola >= 0 && cola < W ? (uint32_t)(cola * 4) : kOff;
+    lp[1] = lane < 2 * kR && colb < W ? (uint32_t)(colb * 4) : kOff;
   }
-  uint32_t lo[2];  // this lane's two pixels, channel c
-#pragma unroll
-  for (int k = 0; k < 2; ++k) lo[k] = ca + k < W ? (uint32_t)((3 * (ca + k) + c) * 4) : kOff;
+  const uint32_t lo = col < W ? (uint32_t)((3 * col + c) * 4) : kOff;  // this lane's pixel, channel c
   auto fetch = [&](int r) {
     Row f;
     const int row = by - kR + r;
     const bool row_ok = row >= 0 && row < H;
     const uint32_t ro = row_ok ? (uint32_t)(row * W * 4) : 0u;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
       const uint32_t a = row_ok ? lp[k] : kOff;  // rows outside read 0
       f.v[0][k] = load_or_zero(rp0, a, ro);
       f.v[1][k] = load_or_zero(rp1, a, ro);
@@ -401,19 +403,16 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
     const int orow = by + r - 2 * kR;
     const bool orow_ok = orow >= 0 && orow < H;
     const uint32_t oro = orow_ok ? (uint32_t)(orow * W * 12) : 0u;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const uint32_t a = orow_ok ? lo[k] : kOff;
-      f.ox[k] = load_or_zero(rimg, a, oro);
-      f.oy[k] = load_or_zero(rtgt, a, oro);
-    }
+    const uint32_t a = orow_ok ? lo : kOff;
+    f.ox = load_or_zero(rimg, a, oro);
+    f.oy = load_or_zero(rtgt, a, oro);
     return f;
   };
-  float ring[3][2][kRing];
-  float ox[2] = {0.f, 0.f}, oy[2] = {0.f, 0.f};  // x and y of the lane's pixels in the current output row
+  float ring[3][kRing];
+  float ox = 0.f, oy = 0.f;  // x and y of the lane's pixel in the current output row
 
-  // one input row: stage the planes, row-blur the lane's two columns into
-  // ring slot `slot`, and take the output row's x and y
+  // one input row: stage the planes, row-blur the lane's column into ring
+  // slot `slot`, and take the output row's x and y
   auto row_in = [&](int r, int slot) {
     __builtin_amdgcn_sched_barrier(0);  // one scheduling region per row
     wave_sync_lds();
@@ -421,34 +420,19 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
     for (int pl = 0; pl < 3; ++pl) {
       st[pl][lane] = q[0].v[pl][0];
       st[pl][64 + lane] = q[0].v[pl][1];
-      st[pl][128 + lane] = q[0].v[pl][2];
     }
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      ox[k] = q[0].ox[k];
-      oy[k] = q[0].oy[k];
-    }
+    ox = q[0].ox;
+    oy = q[0].oy;
     wave_sync_lds();
 #pragma unroll
     for (int k = 0; k + 1 < kGradAhead; ++k) q[k] = q[k + 1];
     q[kGradAhead - 1] = fetch(r + kGradAhead);  // past the band: clamped loads, never used
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
-      float win[kTaps + 1];
+      float h = 0.f;
 #pragma unroll
-      for (int i = 0; i < kTaps + 1; i += 2) {
-        const float2 v2 = *reinterpret_cast<const float2*>(&st[pl][2 * lane + i]);
-        win[i] = v2.x;
-        win[i + 1] = v2.y;
-      }
-      float h0 = 0.f, h1 = 0.f;
-#pragma unroll
-      for (int i = 0; i < kTaps; ++i) {
-        h0 = __builtin_fmaf(p.kernel[i], win[i], h0);
-        h1 = __builtin_fmaf(p.kernel[i], win[i + 1], h1);
-      }
-      ring[pl][0][slot] = h0;
-      ring[pl][1][slot] = h1;
+      for (int i = 0; i < kTaps; ++i) h = __builtin_fmaf(p.kernel[i], st[pl][lane + i], h);
+      ring[pl][slot] = h;
     }
   };
 #pragma unroll
@@ -463,41 +447,32 @@ __global__ __launch_bounds__(kGradWaves * 64) __attribute__((amdgpu_waves_per_eu
       row_in(r, j);
       const int o = r - 2 * kR;
       if (o < 0) continue;  // the first group's first ten rows only fill the ring
-      float bl[3][2];
+      float bl[3];
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl)
+      for (int pl = 0; pl < 3; ++pl) {
+        float s = 0.f;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          float s = 0.f;
-#pragma unroll
-          for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][k][(j + kRing - 2 * kR + i) % kRing], s);
-          bl[pl][k] = s;
-        }
-      float* stage = s_dl[o % kRing];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        float d = bl[0][k];
-        d += 2.f * ox[k] * bl[1][k];
-        d += oy[k] * bl[2][k];
-        const float difference = ox[k] - oy[k];
-        const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
-        float g_ = (1.f - w) * sign * inverse;
-        g_ -= w * d;
-        stage[3 * (2 * lane + k) + c] = g_;
+        for (int i = 0; i < kTaps; ++i) s = __builtin_fmaf(p.kernel[i], ring[pl][(j + kRing - 2 * kR + i) % kRing], s);
+        bl[pl] = s;
       }
+      float d = bl[0];
+      d += 2.f * ox * bl[1];
+      d += oy * bl[2];
+      const float difference = ox - oy;
+      const float sign = difference > 0.f ? 1.f : (difference < 0.f ? -1.f : 0.f);
+      float g_ = (1.f - w) * sign * inverse;
+      g_ -= w * d;
+      s_dl[o % kRing][3 * lane + c] = g_;
     }
     // all three channels of the output rows this group completed are staged:
-    // write those inside the unit as whole interleaved rows
+    // write those inside the half unit as whole interleaved rows, one float
+    // per thread
     __syncthreads();
     const int o_first = max(0, g * kRing - 2 * kR), o_end = min(rows, g * kRing + kRing - 2 * kR);
+    const int e = (int)threadIdx.x;
     for (int o = o_first; o < o_end; ++o) {
       const int row_float = 3 * ((by + o) * W + sx);
-      const float* stage = s_dl[o % kRing];
-#pragma unroll
-      for (int e0 = 0; e0 < kUnitW * 3; e0 += kGradWaves * 64) {
-        const int e = e0 + (int)threadIdx.x;
-        store_or_drop(rdl, e < valid_floats ? (uint32_t)((row_float + e) * 4) : kOff, stage[e]);
-      }
+      store_or_drop(rdl, e < valid_floats ? (uint32_t)((row_float + e) * 4) : kOff, s_dl[o % kRing][e]);
     }
     __syncthreads();
   }
@@ -548,7 +523,7 @@ hipError_t launch_loss(const LossParams& p, const ViewDesc* views, const float* 
   hipLaunchKernelGGL(k_loss_stats<true>, dim3(p.total_tiles), dim3(kStatWaves * 64), 0, stream, p, views, rgb, targets,
                      dL, terms, total_px, block_sums);
   SPLAT_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_loss_grad, dim3(p.total_tiles / 2), dim3(kGradWaves * 64), 0, stream, p, views, rgb, targets,
+  hipLaunchKernelGGL(k_loss_grad, dim3(p.total_tiles), dim3(kGradWaves * 64), 0, stream, p, views, rgb, targets,
                      terms, total_px, dL);
   return hipGetLastError();
 }

@@ -538,7 +538,9 @@ __device__ __forceinline__ void load_zeros(float (&acc)[kAcc], const f32x4* s_ze
 // loop): the per-pixel chains are long and dependent, and the fifth wave
 // hides more of their latency than the registers a fourth-wave budget would
 // allow (a pipelined LDS reduction, 118 VGPRs at four waves, was 4% slower).
-template <int kBB, bool kQueue, bool kLowCut>
+// kDet selects the deterministic records at compile time, so the atomic
+// form's register allocation carries none of their code.
+template <int kBB, bool kQueue, bool kLowCut, bool kDet>
 __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDesc* __restrict__ views,
                                                  const uint32_t* __restrict__ ranges,
                                                  const uint32_t* __restrict__ pair_vals,
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
                                                  const float* __restrict__ T_in, const int* __restrict__ last_in,
                                                  float* __restrict__ screen, const uint32_t* __restrict__ pair_base,
                                                  float* __restrict__ partial, uint8_t* __restrict__ visit,
-                                                 uint32_t p_cap, int deterministic, const uint32_t* __restrict__ epoch,
+                                                 uint32_t p_cap, const uint32_t* __restrict__ epoch,
                                                  const uint32_t* __restrict__ order, uint32_t* __restrict__ counter) {
   static_assert(kBB == 64 || kBB == 128, "chunks of one or two entries per lane");
   static_assert(kScreenStride >= kAcc + 1 && kPartialStride == 16, "record slots");
@@ -569,9 +571,9 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
   // applied to the total (exact: a power of two)
   const float slot_scale = (my_slot == 7 || my_slot == 9) ? 0.5f : 1.f;
   // deterministic records: slots 0-10 the terms, 11 this pass's tag, 12-15 zero
-  const float rec_pad = my_slot == 11 ? __uint_as_float(deterministic ? *epoch : 0u) : 0.f;
+  const float rec_pad = my_slot == 11 ? __uint_as_float(kDet ? *epoch : 0u) : 0.f;
   // and beside each record its visit byte, the tag's low byte
-  const uint8_t visit_tag = (uint8_t)(deterministic ? *epoch : 0u);
+  const uint8_t visit_tag = (uint8_t)(kDet ? *epoch : 0u);
   for (int iter = 0;; ++iter) {
     int tile;
     if (kQueue) {
@@ -654,7 +656,7 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
                 ((m >> k) & 1u && !ellipse_meets_rect(a.z, a.w, b.x, b.y, p.min_alpha, tq.x0[k & 1] - a.x,
                                                       tq.x1[k & 1] - a.x, tq.y0[k >> 1] - a.y, tq.y1[k >> 1] - a.y)))
               m &= ~(1u << k);
-          dst = deterministic ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
+          dst = kDet ? pair_base[slot] + (uint32_t)((ty - r->y0) * (r->x1 - r->x0 + 1) + (tx - r->x0)) : slot;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) qbits[h][k] = __ballot((m >> k) & 1u);
@@ -705,7 +707,7 @@ __global__ __launch_bounds__(64, 5) void k_backward(RasterParams p, const ViewDe
           // (slots 11-15 the tag and zero padding, so the store fills its
           // whole memory sector) and the visit byte from lane 0
           const uint32_t dsti = (uint32_t)__builtin_amdgcn_readlane((int)dst_reg[h], bit);
-          if (deterministic) {
+          if constexpr (kDet) {
             if (slot_lane && dsti < p_cap) {
               partial[(long long)dsti * kPartialStride + my_slot] = my_slot <= kAcc ? z : rec_pad;
               if (lane == 0) visit[dsti] = visit_tag;
@@ -768,11 +770,17 @@ hipError_t backward_variant(const RasterParams& p, const ViewDesc* views, const 
                             uint8_t* visit, uint32_t p_cap, bool deterministic, const uint32_t* epoch,
                             const uint32_t* order,
                             uint32_t* counter, int cus, hipStream_t stream) {
-  static const int per_cu = resident_blocks(k_backward<kBB, kQueue, kLowCut>, 64);
+  // the two forms have the same launch bounds and LDS, so the same residency
+  static const int per_cu = resident_blocks(k_backward<kBB, kQueue, kLowCut, false>, 64);
   const int blocks = kQueue ? std::min(p.total_tiles, per_cu * cus) : p.total_tiles;
-  hipLaunchKernelGGL((k_backward<kBB, kQueue, kLowCut>), dim3(blocks), dim3(64), 0, stream, p, views, ranges, pair_vals, rec,
-                     dL_drgb, T, last, screen, pair_base, partial, visit, p_cap, deterministic ? 1 : 0, epoch,
-                     order, counter);
+  if (deterministic)
+    hipLaunchKernelGGL((k_backward<kBB, kQueue, kLowCut, true>), dim3(blocks), dim3(64), 0, stream, p, views, ranges,
+                       pair_vals, rec, dL_drgb, T, last, screen, pair_base, partial, visit, p_cap, epoch, order,
+                       counter);
+  else
+    hipLaunchKernelGGL((k_backward<kBB, kQueue, kLowCut, false>), dim3(blocks), dim3(64), 0, stream, p, views, ranges,
+                       pair_vals, rec, dL_drgb, T, last, screen, pair_base, partial, visit, p_cap, epoch, order,
+                       counter);
   return hipGetLastError();
 }
 
