@@ -331,13 +331,17 @@ __device__ __forceinline__ uint32_t half_of(uint32_t word, uint32_t d) { return 
 
 // With kV2, a second value array rides along with the first (staged through
 // LDS after the keys and values, so its stores are runs too). Without vals
-// (identity), item j's value is j.
+// (identity), item j's value is j. With chunk_sums, the digit table holds
+// the scan within each 4096-entry chunk and chunk_sums the chunks' totals,
+// which every block scans for itself; then, with kept_total, block 0 also
+// writes their sum (the items the dropping pass keeps).
 template <bool kV2, bool kPlan, int kDigitBits>
 __global__ __launch_bounds__(RadixShape<kDigitBits>::kThreads) void k_radix_scatter(SortRoute R, int identity,
                                                        long long n_cap,
                                                        const uint32_t* __restrict__ n_dev, int shift, int bits,
                                                        int drop, const uint32_t* __restrict__ hist,
-                                                       const uint32_t* __restrict__ hist_prefix, int nblocks) {
+                                                       const uint32_t* __restrict__ chunk_sums,
+                                                       uint32_t* __restrict__ kept_total, int nblocks) {
   constexpr int kRadix = 1 << kDigitBits;
   constexpr int kWaves = RadixShape<kDigitBits>::kWaves, kThreads = RadixShape<kDigitBits>::kThreads;
   constexpr int kItems = RadixShape<kDigitBits>::kItems;
@@ -354,8 +358,12 @@ __global__ __launch_bounds__(RadixShape<kDigitBits>::kThreads) void k_radix_scat
   uint32_t* __restrict__ vals2_out = route_pick(R.w, dst);
   const KeyRewrite rewrite = key_rewrite<kPlan>(R);
   const long long n = item_count(n_cap, n_dev);
-  if ((long long)blockIdx.x * kItems >= n) return;  // whole block past the items (uniform)
+  // a whole block past the items leaves (uniform), but for block 0 when it
+  // writes the kept total
+  const bool past = (long long)blockIdx.x * kItems >= n;
+  if (past && !(kept_total != nullptr && chunk_sums != nullptr && blockIdx.x == 0)) return;
   __shared__ uint32_t s_cnt[kWaves][kWords];  // per-wave digit counts, then per-wave digit offsets (16-bit halves)
+  __shared__ uint32_t s_cw[kWaves];           // wave totals of the chunk-sum scan
   __shared__ uint32_t s_start[kRadix];        // block-local start of each digit, then global start less it
   __shared__ uint32_t s_count;                // items of the block kept
   __shared__ uint32_t s_keys[kItems];
@@ -367,14 +375,54 @@ __global__ __launch_bounds__(RadixShape<kDigitBits>::kThreads) void k_radix_scat
   for (int c = tid; c < kWords; c += kThreads)
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) s_cnt[w][c] = 0;
-  // the global start of this thread's digits: the block-level scan of the
-  // digit table, plus its second level's prefix when the table spans several
-  // scan blocks
+  // the global start of this thread's digits: the digit table's scan, plus,
+  // when it spans several 4096-entry chunks, the sum of the chunks before
+  // (an exclusive scan of the at most kScanTile <= kItems chunk totals, in
+  // s_keys, which is free until the block's items are placed)
   uint32_t gstart[kPer];
+  if (chunk_sums) {  // uniform
+    constexpr int kPerT = kScanTile / kThreads;  // chunk totals per thread
+    static_assert(kScanTile % kThreads == 0 && kScanTile <= kItems, "chunk totals fit s_keys");
+    const int nchunks = (int)(((long long)kRadix * nblocks + kScanTile - 1) / kScanTile);
+    uint32_t v[kPerT], sum = 0;
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const long long e = (long long)(tid * kPer + i) * nblocks + blockIdx.x;
-    gstart[i] = hist[e] + (hist_prefix ? hist_prefix[e / kScanTile] : 0u);
+    for (int k = 0; k < kPerT; ++k) {
+      const int c = tid * kPerT + k;
+      v[k] = c < nchunks ? chunk_sums[c] : 0u;
+      sum += v[k];
+    }
+    uint32_t x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) s_cw[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t t = s_cw[w];
+      if (w < wave) before += t;
+      all += t;
+    }
+    uint32_t run = x - sum + before;
+#pragma unroll
+    for (int k = 0; k < kPerT; ++k) {
+      s_keys[tid * kPerT + k] = run;
+      run += v[k];
+    }
+    if (kept_total && blockIdx.x == 0 && tid == 0) *kept_total = all;
+    __syncthreads();
+    if (past) return;  // block 0 with no items of its own: the total is written (uniform)
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const long long e = (long long)(tid * kPer + i) * nblocks + blockIdx.x;
+      gstart[i] = hist[e] + s_keys[e / kScanTile];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) gstart[i] = hist[(long long)(tid * kPer + i) * nblocks + blockIdx.x];
   }
   __syncthreads();
   const uint64_t below = __lanemask_lt();
@@ -559,25 +607,24 @@ hipError_t launch_exclusive_scan(const uint32_t* in, const uint32_t* gather_idx,
   return scan_level(in, gather_idx, out, n, n_dev, xform, total, side_total, side_xform, ws.buf, stream);
 }
 
-// An in-place exclusive scan of data[0..n) whose readers add the second
-// level themselves: afterwards the scan of element i is data[i] +
-// (*prefix ? (*prefix)[i / kScanTile] : 0). Two levels when n fits them (the
-// last level's add is skipped); otherwise the full scan, and *prefix = null.
-// With n_dev (device), the scan covers min(n, *n_dev) items (a pass a sort
-// plan skips gives 0: nothing is read or written).
+// An in-place exclusive scan of data[0..n) whose readers finish it
+// themselves: when n spans 2 .. kScanTile chunks of kScanTile, only each
+// chunk is scanned and *chunk_sums gets the chunks' totals (unscanned; the
+// scan of element i is data[i] + the sum of the totals before chunk
+// i / kScanTile, and their sum is the total, which the readers then write).
+// Otherwise the full scan, *total written and *chunk_sums = null. With n_dev
+// (device), the scan covers min(n, *n_dev) items (a pass a sort plan skips
+// gives 0: nothing is read or written).
 static hipError_t scan_no_add(uint32_t* data, long long n, uint32_t* total, const ScanWorkspace& ws,
-                              const uint32_t** prefix, hipStream_t stream, const uint32_t* n_dev = nullptr) {
-  *prefix = nullptr;
+                              const uint32_t** chunk_sums, hipStream_t stream, const uint32_t* n_dev = nullptr) {
+  *chunk_sums = nullptr;
   const long long nb = (n + kScanTile - 1) / kScanTile;
   if (nb <= 1 || nb > kScanTile || n > ws.max_n)
     return launch_exclusive_scan(data, nullptr, data, n, total, ws, stream, kScanPlain, n_dev);
   uint32_t* sums = ws.buf;
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, data, nullptr, data, n, n_dev,
                      (int)kScanPlain, sums, nullptr, nullptr, (int)kScanPlain, nullptr);
-  SPLAT_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, stream, sums, nullptr, sums, nb, n_dev,
-                     (int)kScanPlain, nullptr, total, nullptr, (int)kScanPlain, nullptr);
-  *prefix = sums;
+  *chunk_sums = sums;
   return hipGetLastError();
 }
 
@@ -637,8 +684,11 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
     const int identity = (identity_vals && first) ? 1 : 0;
     auto sk = plan ? (digit == 9 ? scatter_kernel<true, 9>(vals2 != nullptr) : scatter_kernel<true, 8>(vals2 != nullptr))
                    : scatter_kernel<false, 8>(vals2 != nullptr);
+    // the dropping pass's kept total: written by the full scan, or with
+    // chunk totals by the scatter's block 0
+    uint32_t* kept_total = drop && prefix ? n_kept : nullptr;
     hipLaunchKernelGGL(sk, dim3((unsigned)nb), dim3(threads), 0, stream, R, identity, n, count, shift, bits, drop,
-                       ws.hist, prefix, (int)nb);
+                       ws.hist, prefix, kept_total, (int)nb);
     SPLAT_HIP_CHECK(hipGetLastError());
     if (!plan) {
       uint32_t* t = ki; ki = ko; ko = t;
