@@ -20,7 +20,8 @@ The sorts' plans, each against the checker too:
   grouped by view and the super-tile sort runs its second pass, by view;
 * a view of 18 x 18 super-tiles: more than 256, so super-tiles are numbered
   across the batch and the super-tile sort takes the plain route;
-* every splat behind the cameras: no kept key, empty lists.
+* every splat behind the cameras: no kept key, empty lists (with a digit
+  table of one scan chunk and of two).
 """
 import numpy as np
 import pytest
@@ -156,10 +157,14 @@ def test_super_tiles_numbered_across_the_batch(ctx):
     batch_against_checker(ctx, cloud, [big, small], [0, 1])
 
 
-def test_every_splat_culled(ctx):
-    cloud = syn.random_cloud(2000, 33, 1, (-1, -1, -3), (1, 1, -2))  # behind the cameras
+@pytest.mark.parametrize("n", [2000, 30000])
+def test_every_splat_culled(ctx, n):
+    # 6,000 slots: the depth sort's digit table is one scan chunk; 90,000
+    # slots (11 blocks of 8192): two chunks, whose zero total the scatter's
+    # block 0 writes as the kept count
+    cloud = syn.random_cloud(n, 33, 1, (-1, -1, -3), (1, 1, -2))  # behind the cameras
     views = [syn.pinhole(320, 200, 300.0)] * 3
-    print("[bin] nothing in front of the cameras", flush=True)
+    print(f"[bin] nothing in front of the cameras, {n} splats", flush=True)
     ctx.set_capacity(cloud.n, 1)
     ctx.upload_cloud(cloud)
     ctx.upload_views(views)
