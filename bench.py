@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="default compositing launch shape")
     ap.add_argument("--profile", action="store_true",
                     help="for rocprofv3: only the batched step, targets uploaded instead of rendered")
-    ap.add_argument("--workload", default="headline", choices=["headline", "c1", "c3", "c4", "c5", "formats"],
+    ap.add_argument("--workload", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5", "formats"],
                     help="headline: the benchmark line; c1/c3/c4/c5: the other configurations' measurements; "
                          "formats: seeding from a point cloud and the PLY, .spz and .splat files, against the "
                          "reference")
@@ -82,7 +82,7 @@ def main():
         raise SystemExit(f"no usable device: {reason}")
     log("device", name)
     if args.workload != "headline":
-        out = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+        out = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
                "formats": bench_formats}[args.workload](args, S, syn)
         print(json.dumps(out), flush=True)
         return
@@ -520,6 +520,27 @@ def bench_c1(args, S, syn):
             "graph_vs_cpu": round(cpu / (graph * 1e3), 1) if cpu else None}
 
 
+def bench_c2(args, S, syn):
+    """The single-view gradient-check configuration: 200k SH3 at 1280x720;
+    GPU step (plain and graph-captured) against the reference trainer's step
+    on the same view and target."""
+    cloud, truth, views = syn.config2()
+    ctx = S.Context(0)
+    tgt = device_targets(ctx, S, truth, views, 3)
+    ctx.set_capacity(cloud.n, 3)
+    ctx.upload_cloud(cloud)
+    ctx.upload_views(views, tgt)
+    opts = S.options(sh_degree=3)
+    plain = timed_steps(ctx, S, [[0]], opts, 50, 5)
+    ctx.set_graph_mode(True)
+    graph = timed_steps(ctx, S, [[0]], opts, 50, 5)
+    ctx.close()
+    cpu = None if args.no_cpu else cpu_step(cloud, views[0], tgt[0], 3, samples=2)
+    return {"workload": "c2: 200k SH3 gaussians, 1280x720, one view per step", "plain_ms": round(plain * 1e3, 4),
+            "graph_ms": round(graph * 1e3, 4), "cpu_reference_ms": cpu, "cpu_sample": "2 steps of the view",
+            "graph_vs_cpu": round(cpu / (graph * 1e3), 1) if cpu else None}
+
+
 def bench_c3(args, S, syn):
     """500k SH3, 32 orbit views at four resolutions: all 32 in one
     graph-captured step against the per-view launch loop."""
@@ -530,6 +551,13 @@ def bench_c3(args, S, syn):
     ctx.set_capacity(cloud.n, 3)
     ctx.upload_cloud(cloud)
     ctx.upload_views(views, tgt)
+    # the CPU leg: one reference step on the first view of each resolution
+    # the batch holds, weighted by how many views have it
+    sizes = {}
+    for k, v in enumerate(views):
+        sizes.setdefault((v.width, v.height), []).append(k)
+    cpu_views = {wh: (ks[0], len(ks)) for wh, ks in sizes.items()}
+    cpu_tgt = {wh: tgt[k] for wh, (k, _) in cpu_views.items()}
     del tgt
     opts = S.options(sh_degree=3)
     everything = [list(range(len(views)))]
@@ -542,11 +570,19 @@ def bench_c3(args, S, syn):
     plain = timed_steps(ctx, S, singles, opts, len(views), 4)
     ctx.close()
     px = sum(v.pixels for v in views)
+    cpu_view_ms = None
+    if not args.no_cpu:
+        per_size = {wh: cpu_step(cloud, views[k], cpu_tgt[wh], 3) for wh, (k, _) in cpu_views.items()}
+        if all(t is not None for t in per_size.values()):
+            cpu_view_ms = round(sum(per_size[wh] * count for wh, (_, count) in cpu_views.items()) / len(views), 1)
     return {"workload": "c3: 500k SH3 gaussians, 32 orbit views of 640x480..1920x1080 in one batch",
             "batched_ms_per_step": round(batched * 1e3, 3), "batched_ms_per_view": round(batched * 1e3 / len(views), 4),
             "plain_ms_per_view": round(plain * 1e3, 4), "batched_vs_plain_per_view": round(plain * len(views) / batched, 3),
             "pixels_per_step": px, "pairs_per_step": pairs, "raster_config": cfg,
-            "gaussian_pixels_per_s": float(f"{cloud.n * px / batched:.4g}")}
+            "gaussian_pixels_per_s": float(f"{cloud.n * px / batched:.4g}"),
+            "cpu_reference_ms_per_view": cpu_view_ms,
+            "cpu_sample": "one reference step per resolution in the batch, weighted by its views",
+            "batched_vs_cpu_per_view": round(cpu_view_ms / (batched * 1e3 / len(views)), 1) if cpu_view_ms else None}
 
 
 def orbit_extent(views):
@@ -642,6 +678,7 @@ def bench_c5(args, S, syn):
     ctx.set_capacity(cloud.n, 3)
     ctx.upload_cloud(cloud)
     ctx.upload_views(views, tgt)
+    tgt0 = tgt[0]  # the CPU leg's target
     del tgt
     out = {"workload": "c5: 3M SH3 gaussians in a 10x10x3 m room, batch of 8 interior 1920x1080 views"}
     for det in (True, False):
@@ -668,6 +705,13 @@ def bench_c5(args, S, syn):
         del grads
         log(f"c5 {key}: {out[key]}")
     ctx.close()
+    if not args.no_cpu:
+        cpu = cpu_step(cloud, views[0], tgt0, 3)
+        out["cpu_reference_ms_per_view"] = cpu
+        out["cpu_sample"] = "one reference step on view 0"
+        if cpu:
+            out["atomic_vs_cpu_per_view"] = round(cpu / (out["atomic"]["ms_per_step"] / len(views)), 1)
+        log(f"c5 cpu: {cpu} ms per view")
     return out
 
 
